@@ -1,0 +1,189 @@
+"""ctypes front end of the C restatement (oracle/lbm_oracle.c) and runner of the reference
+build (oracle/build_ref.py).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg, never by the product path.
+
+``OracleSim`` mirrors the reference driver's restart path and time loop:
+  readrestart -> fA; macrovars(fA); fB = fA; tau = 3*kinevisc + 0.5   (src/main.F90:255-262)
+  do it = nt0+1, nt1: the step sequence of src/main.F90:290-329       (or_run)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import shutil
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+class _Params(C.Structure):
+    pass
+
+
+def _params_type(real):
+    class P(C.Structure):
+        _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("nl", C.c_int),
+                    ("ibgk", C.c_int), ("ihrr", C.c_int), ("ivreman", C.c_int),
+                    ("ibnd", C.c_int), ("jbnd", C.c_int), ("kbnd", C.c_int),
+                    ("iturb_pos", C.c_int),
+                    ("kinevisc", real), ("smagorinsky", real), ("rho0", real)]
+    return P
+
+
+_LIBS = {}
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def _lib(prec):
+    if prec not in _LIBS:
+        path = os.path.join(HERE, f"liblbm_oracle_{prec}.so")
+        if not os.path.exists(path):
+            build()
+        _LIBS[prec] = C.CDLL(path)
+    return _LIBS[prec]
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class OracleSim:
+    """Reference-semantics simulation on the CPU (FP32 or FP64)."""
+
+    def __init__(self, q, dims, prec="f32", ibgk=2, ihrr=0, ivreman=0, smagorinsky=0.0,
+                 kinevisc=1.0 / 30.0, ibnd=0, jbnd=0, kbnd=0, rho0=1.0, iturb_pos=10):
+        self.prec = prec
+        self.dt = np.float32 if prec == "f32" else np.float64
+        real = C.c_float if prec == "f32" else C.c_double
+        self.lib = _lib(prec)
+        P = _params_type(real)
+        nx, ny, nz = dims
+        self.q, self.dims = q, (nx, ny, nz)
+        self.p = P(nx, ny, nz, q, ibgk, ihrr, ivreman, ibnd, jbnd, kbnd, iturb_pos,
+                   kinevisc, smagorinsky, rho0)
+        self.kinevisc = self.dt(kinevisc)
+        shp = (nz + 2, ny + 2, nx + 2)
+        self.fA = np.zeros(shp + (q,), self.dt)
+        self.fB = np.zeros(shp + (q,), self.dt)
+        self.rho, self.u, self.v, self.w = (np.zeros(shp, self.dt) for _ in range(4))
+        self.tau = np.zeros(shp, self.dt)
+        self.cur = 0   # 0: f1 is fA
+        self.lib.or_run.restype = C.c_int
+
+    # -- state ------------------------------------------------------------------------
+    @property
+    def f1(self):
+        return self.fA if self.cur == 0 else self.fB
+
+    @property
+    def f2(self):
+        return self.fB if self.cur == 0 else self.fA
+
+    def restart(self, f):
+        """readrestart + macrovars + fB=fA + tau (src/main.F90:255-262)."""
+        self.fA[...] = f
+        self.fB[...] = f
+        self.cur = 0
+        self.lib.or_macrovars(C.byref(self.p), _ptr(self.fA), _ptr(self.rho), _ptr(self.u),
+                              _ptr(self.v), _ptr(self.w))
+        self.tau[...] = self.dt(3.0) * self.kinevisc + self.dt(0.5)
+
+    def macros(self):
+        return self.rho.copy(), self.u.copy(), self.v.copy(), self.w.copy()
+
+    # -- time loop --------------------------------------------------------------------
+    def run(self, it0, it1, uini=0.0, udir=0.0, uvel_shear=None, uvel_time=None, udir_time=None,
+            blank=None, turb=None, forcing=None, ablvisc=None):
+        nx, ny, nz = self.dims
+        dt = self.dt
+        shear = np.ones(nz, dt) if uvel_shear is None else np.asarray(uvel_shear, dt)
+        ut = np.full(it1 + 1, uini, dt) if uvel_time is None else np.asarray(uvel_time, dt)
+        dtm = np.full(it1 + 1, udir, dt) if udir_time is None else np.asarray(udir_time, dt)
+        abl = np.zeros(nz, dt) if ablvisc is None else np.asarray(ablvisc, dt)
+        uu = vv = ww = None
+        nrturb, ampl = 1, 0.0
+        if turb is not None:
+            uu, vv, ww, nrturb, ampl = turb
+            uu, vv, ww = (np.ascontiguousarray(a, dt) for a in (uu, vv, ww))
+        F = None if forcing is None else np.ascontiguousarray(forcing, dt)
+        bl = None if blank is None else np.ascontiguousarray(blank, np.uint8)
+        real = C.c_float if self.prec == "f32" else C.c_double
+        sw = self.lib.or_run(C.byref(self.p), _ptr(self.f1), _ptr(self.f2), _ptr(self.tau),
+                             _ptr(self.rho), _ptr(self.u), _ptr(self.v), _ptr(self.w), _ptr(abl),
+                             _ptr(bl), _ptr(shear), _ptr(ut), _ptr(dtm), _ptr(uu), _ptr(vv), _ptr(ww),
+                             C.c_int(nrturb), real(ampl), _ptr(F), C.c_int(F is not None), C.c_int(it0),
+                             C.c_int(it1))
+        self.cur ^= sw
+        return self.f1
+
+
+# ------------------------------------------------------------------------------------------
+# Reference build runner
+# ------------------------------------------------------------------------------------------
+def infile_text(*, ibgk=2, ihrr=0, ivreman=0, smagorinsky=0.0, nt0=1, nt1=2, ibnd=0, jbnd=0, kbnd=0,
+                uini=0.0, udir=0.0, turb=False, ampl=0.0, nrturb=100, visc=1.0 / 30.0,
+                experiment="none", ltesting=True, ldump=True, ltiming=False, iout=None):
+    """An infile.in in the line order src/m_readinfile.F90:84-160 reads.  C_l = C_u = 1 so the
+    viscosity line is the non-dimensional kinevisc (src/m_readinfile.F90:182-186)."""
+    iout = iout if iout is not None else 10 ** 7
+    L = lambda b: "T" if b else "F"
+    smagorinsky, uini, udir, ampl, visc = (float(x) for x in (smagorinsky, uini, udir, ampl, visc))
+    rows = [
+        "# Development variables",
+        f" {L(ltiming)}", f" {L(ltesting)}", f" {L(ldump)}", " 64 2 1", " F",
+        "# Experiment configuration",
+        f" {experiment}", f" {ibgk}", f" {ihrr}", f" {ivreman} {smagorinsky!r}",
+        "# Atmospheric boundary layer",
+        " 0", " 400.0", " 1",
+        "# Time stepping and outputs",
+        f" {nt0}", f" {nt1}", f" {iout}", f" {10 ** 7}", f" 0 {10 ** 7} {10 ** 7}", " 2",
+        "# Boundary conditions",
+        f" {ibnd}", f" {jbnd}", f" {kbnd}",
+        "# Inflow variables",
+        f" {uini!r} {udir!r}", f" {L(turb)} {ampl!r} {nrturb}",
+        "# Physical variables",
+        f" {visc!r}", " 1.225", " 1.0", " 1.0",
+        "# Averaging variables",
+        " F F", " 20000", " 40000",
+        "# Turbine-definitions",
+        " 0",
+    ]
+    return "\n".join(rows) + "\n"
+
+
+def run_reference(exe, f_in, *, nt0, nt1, turb_fields=None, workdir=None, threads=1, timeout=3600, **opts):
+    """Run a reference executable (built by oracle/build_ref.py) from restart state ``f_in``
+    at step nt0 to nt1.  Returns (f_out, stdout).  f_out is the restart record written at
+    nt1 (post-streaming f1 with ghosts, src/main.F90:398)."""
+    import sys
+    sys.path.insert(0, os.path.dirname(HERE))
+    from lbm_amd import fortran_io as fio
+    own = workdir is None
+    wd = tempfile.mkdtemp(prefix="lbmref_") if own else workdir
+    try:
+        os.makedirs(os.path.join(wd, "restart"), exist_ok=True)
+        fio.write_restart(os.path.join(wd, "restart", fio.restart_name(nt0)), f_in)
+        if turb_fields is not None:
+            uu, vv, ww, rr = turb_fields
+            fio.write_turbulence(os.path.join(wd, "restart", fio.restart_name(nt0, prefix="turbulence")),
+                                 uu, vv, ww, rr)
+        with open(os.path.join(wd, "infile.in"), "w") as fh:
+            fh.write(infile_text(nt0=nt0, nt1=nt1, turb=turb_fields is not None, **opts))
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+        p = subprocess.run(["bash", "-c", f"ulimit -s unlimited; {exe}"], cwd=wd, env=env,
+                           capture_output=True, text=True, timeout=timeout)
+        if p.returncode != 0 or "infile.in problem" in p.stdout:
+            raise RuntimeError(f"reference run failed ({p.returncode}):\n{p.stdout[-3000:]}\n{p.stderr[-3000:]}")
+        f_out = fio.read_restart(os.path.join(wd, "restart", fio.restart_name(nt1)))
+        return f_out, p.stdout
+    finally:
+        if own:
+            shutil.rmtree(wd, ignore_errors=True)
