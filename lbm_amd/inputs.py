@@ -97,8 +97,9 @@ def smooth_field(seed: int, ny: int, nz: int, corr: float, offset: int = 0) -> n
     kz = np.fft.fftfreq(nz) * 2.0 * np.pi
     filt = np.exp(-((kz[:, None] * corr) ** 2 + (ky[None, :] * corr) ** 2) / 4.0)
     g = np.real(np.fft.ifft2(np.fft.fft2(noise) * filt))
+    g = g - g.mean()
     s = g.std()
-    return g / s if s > 0 else g
+    return g / s if s > 1e-12 else np.zeros_like(g)
 
 
 def turbulence_fields(seed: int, ny: int, nz: int, nrturb: int, corr: float = 8.0,

@@ -1,0 +1,90 @@
+"""Golden-vector cases: the reference's own build (oracle/build_ref.py) run on seeded inputs.
+
+Each case is a small grid chosen to exercise one part of the lattice update.  The inputs are
+regenerated from the seed by lbm_amd.inputs (their SHA-256 is stored to catch generator
+drift); turbulence fields are stored in full.  The stored output is the restart record the
+reference writes after ``steps`` steps: f1 including ghost layers (src/main.F90:398).
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+# name: (q, prec, dims, steps, seed, experiment, turbulence nrturb or 0, options)
+CASES = {
+    # config-1 physics: D3Q19 BGK, periodic i/j, no-slip k, tau = 0.6
+    "q19_bgk_pp11": (19, "sp", (16, 8, 6), 20, 1, "none", 0,
+                     dict(ibgk=2, ihrr=0, ivreman=0, ibnd=0, jbnd=0, kbnd=11, visc=1.0 / 30.0)),
+    # config-2 physics: D3Q27 3rd-order regularized + Vreman, inflow/outflow, periodic j,
+    # no-slip k=1, free-slip k=nz, the reference's cylinder (r=7 at i=50, j=ny/2), turbulence
+    # forcing at i=10
+    "q27_regvre_inflow_cyl": (27, "sp", (60, 14, 5), 20, 2, "cylinder", 25,
+                              dict(ibgk=3, ihrr=1, ivreman=1, smagorinsky=0.16733, ibnd=1, jbnd=0,
+                                   kbnd=12, uini=0.05, udir=0.0, visc=1e-4, ampl=1e-3)),
+    # the same physics in the FP64 build (no cylinder: grid too small for it)
+    "q27_regvre_inflow_dp": (27, "dp", (16, 8, 6), 20, 3, "none", 25,
+                             dict(ibgk=3, ihrr=1, ivreman=1, smagorinsky=0.16733, ibnd=1, jbnd=0,
+                                  kbnd=12, uini=0.05, udir=0.0, visc=1e-4, ampl=1e-3)),
+    # all walls closed, mixed no-slip/free-slip: closed-wall kernels, edges and corners
+    "q27_regvre_closed": (27, "sp", (12, 8, 6), 20, 5, "none", 0,
+                          dict(ibgk=3, ihrr=1, ivreman=1, smagorinsky=0.16733, ibnd=22, jbnd=11,
+                               kbnd=12, visc=1e-3)),
+    "q19_bgk_closed": (19, "sp", (12, 8, 6), 20, 3, "none", 0,
+                       dict(ibgk=2, ibnd=11, jbnd=21, kbnd=22, visc=1.0 / 30.0)),
+    # equal dimensions make every entry of the reference's corner table a real corner
+    "q27_bgk3_closed_cube": (27, "sp", (8, 8, 8), 12, 7, "none", 0,
+                             dict(ibgk=3, ihrr=0, ivreman=0, ibnd=12, jbnd=21, kbnd=11, visc=0.02)),
+    # D3Q19 regularized + Vreman, periodic i and k, closed j
+    "q19_regvre_pj12": (19, "sp", (12, 8, 6), 20, 4, "none", 0,
+                        dict(ibgk=3, ihrr=1, ivreman=1, smagorinsky=0.2, ibnd=0, jbnd=12, kbnd=0,
+                             visc=1e-3)),
+    # inflow at 20 degrees, free-slip j, no-slip k
+    "q19_bgk_inflow_dir": (19, "sp", (12, 8, 6), 20, 6, "none", 0,
+                           dict(ibgk=2, ibnd=1, jbnd=22, kbnd=11, uini=0.03, udir=20.0, visc=1.0 / 30.0)),
+}
+
+
+def case_inputs(name):
+    """Initial restart state and turbulence fields of a case (regenerated from its seed)."""
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+    from lbm_amd import inputs
+    q, prec, dims, steps, seed, exp, nrturb, o = CASES[name]
+    dt = np.float32 if prec == "sp" else np.float64
+    u0 = (o.get("uini", 0.05), 0.0, 0.0)
+    f0 = inputs.perturbed_uniform_flow(q, dims, seed=seed, u0=u0, order=o.get("ibgk", 2), dtype=dt)
+    turb = None
+    if nrturb:
+        turb = inputs.turbulence_fields(seed, dims[1], dims[2], nrturb, dtype=dt)
+    return f0, turb
+
+
+def blanking(name):
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+    from lbm_amd import inputs
+    q, prec, dims, steps, seed, exp, nrturb, o = CASES[name]
+    if exp == "cylinder":
+        # the reference's own geometry: radius 7 at (50, nyg/2) (src/m_cylinder.F90:10)
+        return inputs.cylinder_mask(dims, 50, dims[1] // 2, 7)
+    return None
+
+
+def fixture_path(name):
+    return os.path.join(HERE, f"{name}.npz")
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def load(name):
+    z = np.load(fixture_path(name), allow_pickle=False)
+    meta = json.loads(str(z["meta"]))
+    out = {k: z[k] for k in z.files if k != "meta"}
+    return meta, out

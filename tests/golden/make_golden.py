@@ -1,0 +1,45 @@
+#!/usr/bin/env python3
+"""Writes the golden fixtures of tests/golden/cases.py by running the REFERENCE's own build
+(amdflang, oracle/build_ref.py) on the seeded inputs.  Needs /root/reference; run from the
+repository root:  python tests/golden/make_golden.py [case ...]
+"""
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, HERE)
+
+import build_ref  # noqa: E402
+import oracle  # noqa: E402
+from cases import CASES, case_inputs, fixture_path, sha  # noqa: E402
+
+
+def make(name):
+    q, prec, dims, steps, seed, exp, nrturb, o = CASES[name]
+    f0, turb = case_inputs(name)
+    exe = build_ref.build(q, prec, dims)
+    opts = dict(o)
+    if nrturb:
+        opts["nrturb"] = nrturb
+        opts.setdefault("ampl", 0.0)
+    f_out, _ = oracle.run_reference(exe, f0, nt0=1, nt1=1 + steps, turb_fields=turb,
+                                    experiment=exp, **opts)
+    meta = dict(name=name, q=q, prec=prec, dims=list(dims), steps=steps, seed=seed, experiment=exp,
+                nrturb=nrturb, options=o, nt0=1, input_sha256=sha(f0),
+                producer="reference build (amdflang) via oracle/build_ref.py")
+    arrays = dict(meta=np.array(json.dumps(meta)), f_out=f_out)
+    if turb is not None:
+        arrays.update(uu=turb[0], vv=turb[1], ww=turb[2])
+    np.savez_compressed(fixture_path(name), **arrays)
+    print(f"{name}: {os.path.getsize(fixture_path(name)) / 1024:.0f} KiB")
+
+
+if __name__ == "__main__":
+    for n in (sys.argv[1:] or CASES):
+        make(n)
