@@ -1,0 +1,186 @@
+#!/usr/bin/env python3
+"""Benchmark of the fused lattice update on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): D3Q27, third-order Hermite
+equilibrium, regularized collision + Vreman SGS (c = 0.07, i.e. smagorinsky = sqrt(0.07/2.5)),
+FP32, 384x192x128 cells, inflow u=0.05 at i=1 / outflow at i=nx, periodic j, no-slip k=1,
+free-slip k=nz, solid cylinder of radius 12 along k at (96,96), smooth seeded inflow forcing at
+slice i=10 (amplitude 1e-3, correlation length 8), nu = 1e-4, 1000 steps by default.
+
+A "step" is one pass of the whole time-loop body (fused stream-collide kernel + boundary
+kernels) over the grid.  Inputs are synthetic and resident in HBM before timing starts.
+
+    python bench.py [--steps K] [--warmup W] [--storage f32|f16s|bf16s|f64] [--no-cpu-baseline]
+
+Prints ONE JSON line.  --gpus N > 1 is refused: this build targets one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import re
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+SMAG = math.sqrt(0.07 / 2.5)
+
+
+def workload(storage):
+    return dict(q=27, dims=(384, 192, 128), storage=storage, ibgk=3, ihrr=1, ivreman=1, smagorinsky=SMAG,
+                kinevisc=1e-4, ibnd=1, jbnd=0, kbnd=12, uini=0.05, udir=0.0, cyl=(96, 96, 12), ip=10,
+                ampl=1e-3, corr=8.0, seed=2)
+
+
+def bytes_per_cell(q, storage):
+    s = {"f32": 4, "f64": 8, "f16s": 2, "bf16s": 2}[storage]
+    return 2 * q * s
+
+
+def make_inputs(w, nrturb):
+    from lbm_amd import inputs
+    dt = np.float64 if w["storage"] == "f64" else np.float32
+    f0 = inputs.perturbed_uniform_flow(w["q"], w["dims"], seed=w["seed"], u0=(w["uini"], 0, 0), order=3, dtype=dt)
+    nx, ny, nz = w["dims"]
+    turb = inputs.turbulence_fields(w["seed"], ny, nz, nrturb, corr=w["corr"], dtype=dt)
+    ic, jc, r = w["cyl"]
+    mask = inputs.cylinder_mask(w["dims"], ic, jc, r)
+    return f0, turb, mask
+
+
+def traffic_from_profiles(name):
+    """HBM bytes per launch measured with rocprofv3 PMC counters (profiles/<name>.json)."""
+    p = os.path.join(ROOT, "profiles", name + ".json")
+    if not os.path.exists(p):
+        return None
+    try:
+        with open(p) as fh:
+            return json.load(fh).get("traffic_bytes_per_launch")
+    except Exception:
+        return None
+
+
+def cpu_baseline(w, f0, turb, sample_steps, threads):
+    """The reference's own OpenMP build (oracle/_ref, amdflang) timed on this host on a bounded
+    sample of the same workload; its "Total TS routines" timer (src/m_wtime.F90:75)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import build_ref  # noqa: E402
+    import oracle  # noqa: E402
+    exe = build_ref.exe_path(w["q"], "sp", w["dims"], omp=True)
+    if not os.path.exists(exe):
+        try:
+            exe = build_ref.build(w["q"], "sp", w["dims"], omp=True)
+        except Exception:
+            return None
+    nx, ny, nz = w["dims"]
+    nr = sample_steps + 5
+    tf = tuple(a[: nr + 1] for a in turb) + (np.zeros_like(turb[0][: nr + 1]),)
+    wd = tempfile.mkdtemp(prefix="lbm_cpu_")
+    try:
+        _, out = oracle.run_reference(exe, f0, nt0=1, nt1=1 + sample_steps, turb_fields=tf, workdir=wd,
+                                      threads=threads, ibgk=3, ihrr=1, ivreman=1, smagorinsky=float(np.float32(SMAG)),
+                                      ibnd=1, jbnd=0, kbnd=12, uini=w["uini"], udir=0.0, visc=w["kinevisc"],
+                                      ampl=w["ampl"], nrturb=nr, experiment="cylinder", ltiming=True,
+                                      ltesting=False, ldump=False)
+    finally:
+        shutil.rmtree(wd, ignore_errors=True)
+    m = re.search(r"Total TS routines\s+time\s*=\s*([0-9.Ee+-]+)", out)
+    if not m:
+        return None
+    secs = float(m.group(1))
+    mlups = nx * ny * nz * sample_steps / secs / 1e6
+    return {"value": round(mlups, 3), "unit": "MLUPS", "cores": threads, "kind": "reference",
+            "sample": f"{sample_steps} steps of the same 384x192x128 D3Q27 reg+Vreman workload "
+                      f"(reference's own cylinder r=7 at i=50), amdflang -O2 -fopenmp, OMP_NUM_THREADS={threads}, "
+                      f"'Total TS routines' timer {secs:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--storage", default="f32", choices=["f32", "f64", "f16s", "bf16s"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
+    ap.add_argument("--block", type=int, nargs=2, default=(0, 0))
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
+                                   "described in DESIGN.md, not built)"}))
+        sys.exit(2)
+
+    from lbm_amd.domain import Domain
+    w = workload(a.storage)
+    nx, ny, nz = w["dims"]
+    nrturb = max(1001, a.steps + a.warmup + 1)
+    print(f"[bench] building inputs {w['dims']} q={w['q']} storage={a.storage}", file=sys.stderr, flush=True)
+    f0, turb, mask = make_inputs(w, nrturb)
+
+    dom = Domain(w["q"], w["dims"], a.storage, ibgk=w["ibgk"], ihrr=w["ihrr"], ivreman=w["ivreman"],
+                 smagorinsky=w["smagorinsky"], kinevisc=w["kinevisc"], ibnd=w["ibnd"], jbnd=w["jbnd"],
+                 kbnd=w["kbnd"], iturb_pos=w["ip"], block=tuple(a.block))
+    dom.set_state(f0)
+    dom.set_inflow(w["uini"], w["udir"])
+    dom.set_blanking(mask)
+    dom.set_turbulence(turb[0], turb[1], turb[2], nrturb, w["ampl"])
+    print(f"[bench] warmup {a.warmup}", file=sys.stderr, flush=True)
+    dom.step(1, a.warmup)
+    dom.synchronize()
+    dom.set_timing(True)
+    print(f"[bench] timed {a.steps} steps", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    dom.step(1 + a.warmup, a.steps)
+    dom.synchronize()
+    t1 = time.perf_counter()
+    tm = dom.timing_read()
+    rho, u, v, ww = dom.macrovars()
+    finite = bool(np.isfinite(rho).all() and np.isfinite(u).all())
+    dom.close()
+
+    cells = nx * ny * nz
+    secs = t1 - t0
+    mlups = cells * a.steps / secs / 1e6
+    kern_ms = tm["step_ms"] / max(tm["n_step"], 1)
+    bpc = bytes_per_cell(w["q"], a.storage)
+    achieved = bpc * cells / (kern_ms * 1e-3) / 1e9
+    kname = "step_kernel_q27_reg_vreman_" + a.storage
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(kname),
+            "kernel": "step_kernel", "bytes_per_cell": bpc, "kernel_ms": round(kern_ms, 5),
+            "bc_ms_per_step": round(tm["bc_ms"] / max(tm["n_bc"], 1), 5)}
+    cpu = None
+    if not a.no_cpu_baseline and a.storage == "f32":
+        print(f"[bench] cpu baseline: reference OpenMP build, {a.cpu_steps} steps", file=sys.stderr, flush=True)
+        try:
+            cpu = cpu_baseline(w, f0, turb, a.cpu_steps, a.cpu_threads)
+        except Exception as e:  # the baseline is reported, never required
+            print(f"[bench] cpu baseline failed: {e}", file=sys.stderr)
+    line = {
+        "metric": "MLUPS + %HBM roofline (D3Q27 reg+Vreman), FP16-store vs FP32 at stated error",
+        "value": round(mlups, 1), "unit": "MLUPS", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(secs / a.steps * 1e3, 5), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": {"f32": "f32", "f64": "f64", "f16s": "f16-store/f32", "bf16s": "bf16-store/f32"}[a.storage],
+        "data": "synthetic (seeded perturbed uniform flow, seeded smooth inflow forcing)",
+        "config": {"workload": "D3Q27 3rd-order regularized + Vreman (c=0.07), 384x192x128, inflow/outflow i, "
+                               "periodic j, no-slip k=1 / free-slip k=nz, cylinder r=12 at (96,96), "
+                               "inflow forcing at i=10, nu=1e-4",
+                   "cells": cells, "storage": a.storage, "parallelism": "single GPU"},
+        "roofline": roof, "cpu_baseline": cpu, "finite": finite,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

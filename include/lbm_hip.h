@@ -1,0 +1,134 @@
+/*
+ * lbm_hip.h -- C ABI of the MI355X lattice-update library (liblbm_hip.so).
+ *
+ * The library replaces the reference's per-timestep operators (geirev/LBM, called from the
+ * time loop src/main.F90:290-329).  One context holds one domain: runtime grid dimensions,
+ * lattice, collision options, boundary codes, one HIP stream and all device memory.
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a non-zero LBM_E* code on failure; the message
+ *     is available from lbm_last_error().  (The reference's operators return nothing and
+ *     `stop` on fatal paths; a Fortran binding stops on a non-zero status.)
+ *   - Host arrays are borrowed for the duration of the call and use the reference's layouts:
+ *       populations f(nl,0:nx+1,0:ny+1,0:nz+1), cell fields rho(0:nx+1,0:ny+1,0:nz+1),
+ *       external_forcing(3,0:nx+1,0:ny+1,0:nz+1), uu(ny,nz,0:nrturb), all column major.
+ *     `realkind` is the byte size of the host reals (4 or 8), as the reference's REAL kind.
+ *   - Calls on one context are not thread safe; all device work is ordered on its stream.
+ *     Every call that returns host data synchronises that stream.
+ */
+#ifndef LBM_HIP_H
+#define LBM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBM_OK 0
+#define LBM_EINVAL 1   /* invalid argument */
+#define LBM_EHIP 2     /* HIP runtime error (message holds hipGetErrorString) */
+#define LBM_ESTATE 3   /* call out of order (e.g. step before a state was set) */
+#define LBM_EUNSTABLE 4 /* non-finite or non-positive density found */
+
+/* Storage of the populations in HBM; the collision always computes in FP32 (FP64 for F64). */
+#define LBM_STORE_F32 0
+#define LBM_STORE_F64 1
+#define LBM_STORE_F16S 2  /* (f - w_l) * 2^10 as IEEE half */
+#define LBM_STORE_BF16S 3 /* (f - w_l) as bfloat16 */
+
+typedef struct lbm_ctx lbm_ctx;
+
+/* Run-time description of one domain.  Field meanings follow the reference's infile.in
+ * (src/m_readinfile.F90:86-160) and mod_dimensions (src/mod_dimensions.F90:21-26). */
+typedef struct {
+    int32_t nx, ny, nz;        /* interior cells (>= 2 each)                                  */
+    int32_t q;                 /* 19 or 27 (the reference's -DD3Q19 switch)                   */
+    int32_t storage;           /* LBM_STORE_*                                                 */
+    int32_t ibgk;              /* 2 or 3: order of the Hermite equilibrium                    */
+    int32_t ihrr;              /* 1: regularized non-equilibrium (Malaspinas recursion)       */
+    int32_t ivreman;           /* 1: Vreman eddy viscosity                                    */
+    int32_t ibnd, jbnd, kbnd;  /* 0 periodic, 1 inflow/outflow (i only), 11/12/21/22 closed   */
+    int32_t iturb_pos;         /* inflow-turbulence slice (reference: 10)                     */
+    int32_t store_tau;         /* 1: keep the per-cell tau of the Vreman closure              */
+    int32_t block_x, block_y;  /* step-kernel block shape; 0 = default                        */
+    double kinevisc;           /* non-dimensional kinematic viscosity: tau0 = 3*kinevisc+0.5  */
+    double smagorinsky;        /* Vreman constant c = 2.5*smagorinsky^2                       */
+    double rho0;               /* reference density of the inflow condition (1.0)             */
+} lbm_desc;
+
+/* Context lifetime.  Replaces the reference's allocations (src/main.F90:150-177) and
+ * hermite_polynomials() (src/mod_D3Q27setup.F90:71-169). */
+int lbm_create(const lbm_desc* desc, lbm_ctx** out);
+int lbm_destroy(lbm_ctx* ctx);
+/* Message of the last failure on ctx (or of the last failed lbm_create when ctx is NULL). */
+const char* lbm_last_error(const lbm_ctx* ctx);
+
+/* State in: f in the restart layout including ghosts.  Mirrors the restart path
+ * readrestart -> macrovars -> fB = fA (src/main.F90:255-258, src/m_readrestart.F90:163-185):
+ * both device buffers receive f and the next step collides it in place. */
+int lbm_set_state(lbm_ctx* ctx, const void* f, int realkind);
+
+/* Equilibrium start from moments (src/main.F90:243-247): fequil3, fillghosts, fA = fB,
+ * boundarycond(fB,fA), boundarycond(fA,fB).  The first step then collides with these moments,
+ * as the reference's first postcoll uses inipert's rho,u,v,w.  it0 selects uvel/udir. */
+int lbm_init_equilibrium(lbm_ctx* ctx, const void* rho, const void* u, const void* v, const void* w,
+                         int realkind, int it0);
+
+/* State out: the reference's f1 after the last step (post-streaming populations and ghost
+ * layers), as saverestart/testing write it (src/m_saverestart.F90:125-136). */
+int lbm_get_state(lbm_ctx* ctx, void* f, int realkind);
+
+/* Moments of that state for every cell including ghosts: macrovars (src/m_macrovars.F90:4). */
+int lbm_get_macros(lbm_ctx* ctx, void* rho, void* u, void* v, void* w, int realkind);
+
+/* Per-cell relaxation time of the last step (needs store_tau and ivreman; interior cells,
+ * others hold tau0).  The reference's tau array (src/m_postcoll_kernel.F90:304). */
+int lbm_get_tau(lbm_ctx* ctx, void* tau, int realkind);
+
+/* Inputs.  Solid mask lblanking(0:nx+1,0:ny+1,0:nz+1) as bytes (non-zero = solid), NULL to
+ * remove; replaces solids(f,lblanking) (src/m_solids.F90:3). */
+int lbm_set_blanking(lbm_ctx* ctx, const uint8_t* mask);
+/* ablvisc(nz), added to the Vreman viscosity (src/m_postcoll_kernel.F90:304). */
+int lbm_set_ablvisc(lbm_ctx* ctx, const void* ablvisc, int realkind);
+/* Inflow: uvel = uvel_time(it)*uvel_shear(k), direction udir_time(it) in degrees
+ * (src/main.F90:313-314, src/m_inflow.F90).  shear may be NULL (all ones).  Steps outside
+ * [it_lo, it_hi] (or with series == NULL) use uini and udir. */
+int lbm_set_inflow(lbm_ctx* ctx, const void* shear, double uini, double udir, int realkind);
+int lbm_set_inflow_series(lbm_ctx* ctx, int it_lo, int it_hi, const void* uvel_time,
+                          const void* udir_time, int realkind);
+/* Inflow-turbulence fields uu,vv,ww(ny,nz,0:nrturb) and amplitude: the df of
+ * inflow_turbulence_forcing (src/m_inflow_turbulence_forcing.F90:5) added to slice iturb_pos
+ * after the collision (inflow_turbulence_apply, src/m_inflow_turbulence_apply.F90:4).
+ * uu == NULL switches it off. */
+int lbm_set_turbulence(lbm_ctx* ctx, const void* uu, const void* vv, const void* ww, int nrturb,
+                       double ampl, int realkind);
+/* external_forcing(3,0:nx+1,0:ny+1,0:nz+1) applied every step by forcings_apply
+ * (src/m_forcings_apply.F90:3); NULL switches it off. */
+int lbm_set_external_forcing(lbm_ctx* ctx, const void* force, int realkind);
+
+/* Time steps it0+1 .. it0+nsteps: the sequence of src/main.F90:296-329 (turbulence df,
+ * postcoll, turbulence apply, forcings_apply, solids, boundarycond, drift, swap, macrovars). */
+int lbm_step(lbm_ctx* ctx, int it0, int nsteps);
+
+/* Wait for the context's stream.  The stream itself (a hipStream_t) for callers that order
+ * their own work after the library's. */
+int lbm_synchronize(lbm_ctx* ctx);
+void* lbm_stream(lbm_ctx* ctx);
+
+/* Kernel timing with HIP events on the context's stream.  While on, every launch of the
+ * step kernel and every group of boundary kernels is bracketed by events; lbm_timing_read
+ * synchronises and returns the summed milliseconds and launch counts since the last read. */
+int lbm_set_timing(lbm_ctx* ctx, int on);
+int lbm_timing_read(lbm_ctx* ctx, double* step_ms, int* n_step, double* bc_ms, int* n_bc);
+
+/* Global mass and momentum of the interior state in FP64 (accumulated on the device). */
+int lbm_get_totals(lbm_ctx* ctx, double* mass, double* momentum3);
+
+/* Bytes of HBM the context holds, and the device layout pitch (elements per row). */
+int lbm_get_info(lbm_ctx* ctx, int64_t* device_bytes, int32_t* row_pitch, int64_t* plane_elems);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

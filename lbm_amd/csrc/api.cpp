@@ -1,0 +1,698 @@
+// C ABI of liblbm_hip.so (declared in include/lbm_hip.h): context, device memory, and the
+// per-step launch sequence that stands in for the reference's time loop body
+// (src/main.F90:290-329).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/lbm_hip.h"
+#include "impl.h"
+#include "layout.h"
+
+using namespace lbm;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t n = 0;
+};
+
+}  // namespace
+
+struct lbm_ctx {
+    lbm_desc d{};
+    Layout lay{};
+    const Ops* ops = nullptr;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    // population buffers
+    void* buf[2] = {nullptr, nullptr};
+    size_t buf_bytes = 0;
+    int cur = 0;
+    bool have_state = false;
+    bool pending = false;          // buf[cur] holds an uncollided state (next step collides in place)
+    bool ghost_raw[2] = {true, true};
+    bool use_ext = false;          // first collision uses the moments given at initialisation
+    // inputs
+    DevBuf ablvisc, shear, solid, turb[3], force[3], tau, ext[4], stage, partial;
+    int solid_wpr = 0;
+    bool has_solid = false, has_turb = false, has_force = false;
+    int nrturb = 0;
+    double ampl = 0.0;
+    double uini = 0.0, udir = 0.0;
+    int series_lo = 0, series_hi = -1;
+    std::vector<double> ut_series, ud_series;
+    double tau0 = 0.0, fac0 = 0.0;
+    // timing
+    bool timing = false;
+    std::vector<hipEvent_t> ev_step, ev_bc;   // pairs
+    size_t n_step_ev = 0, n_bc_ev = 0;
+    std::string err;
+};
+
+namespace {
+
+int fail(lbm_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
+int hipfail(lbm_ctx* c, hipError_t e, const char* where) {
+    return fail(c, LBM_EHIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(c, call)                                   \
+    do {                                                  \
+        hipError_t e_ = (call);                           \
+        if (e_ != hipSuccess) return hipfail((c), e_, #call); \
+    } while (0)
+
+int dev_alloc(lbm_ctx* c, DevBuf& b, size_t n) {
+    if (b.n >= n && b.p) return LBM_OK;
+    if (b.p) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipFree(b.p));
+        b.p = nullptr;
+        b.n = 0;
+    }
+    HIPCHK(c, hipMalloc(&b.p, n));
+    b.n = n;
+    return LBM_OK;
+}
+
+void dev_free(DevBuf& b) {
+    if (b.p) hipFree(b.p);
+    b.p = nullptr;
+    b.n = 0;
+}
+
+size_t ncell_padded(const lbm_ctx* c) {
+    return (size_t)(c->d.nx + 2) * (c->d.ny + 2) * (c->d.nz + 2);
+}
+
+int check_kind(lbm_ctx* c, int realkind) {
+    if (realkind != 4 && realkind != 8) return fail(c, LBM_EINVAL, "realkind must be 4 or 8");
+    return LBM_OK;
+}
+
+// Host reals of either kind -> device array of the compute type.
+int upload_reals(lbm_ctx* c, DevBuf& b, const void* host, size_t n, int realkind) {
+    const int rb = c->ops->real_bytes;
+    std::vector<unsigned char> tmp(n * rb);
+    for (size_t q = 0; q < n; ++q) {
+        double v = realkind == 8 ? static_cast<const double*>(host)[q] : static_cast<const float*>(host)[q];
+        if (rb == 8) reinterpret_cast<double*>(tmp.data())[q] = v;
+        else reinterpret_cast<float*>(tmp.data())[q] = (float)v;
+    }
+    int rc = dev_alloc(c, b, n * rb);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(b.p, tmp.data(), n * rb, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+// A value in the compute type's rounding.
+double round_real(const lbm_ctx* c, double v) {
+    return c->ops->real_bytes == 4 ? (double)(float)v : v;
+}
+
+void opts_of(int code, int* o1, int* oN) {
+    // src/m_boundarycond.F90:166-172
+    switch (code) {
+        case 11: *o1 = -1; *oN = -1; break;
+        case 12: *o1 = -1; *oN = +1; break;
+        case 21: *o1 = +1; *oN = -1; break;
+        case 22: *o1 = +1; *oN = +1; break;
+        default: *o1 = 0; *oN = 0;
+    }
+}
+
+// Ghost edges and corners in the reference's order (src/m_boundarycond.F90:270-310).
+EdgeList edge_list(const lbm_desc& d) {
+    EdgeList e{};
+    e.n = 0;
+    const int n[3] = {d.nx, d.ny, d.nz};
+    auto add_edge = [&](int d1, int s1, int d2, int s2, int m0, int m1, int m2) {
+        const int t = 3 - d1 - d2;
+        const int q = e.n++;
+        e.kind[q] = 0;
+        e.t[q] = t;
+        e.len[q] = n[t];
+        e.g[q][t] = 0;
+        e.g[q][d1] = s1 ? n[d1] + 1 : 0;
+        e.g[q][d2] = s2 ? n[d2] + 1 : 0;
+        e.out[q][0] = e.out[q][1] = e.out[q][2] = 0;
+        e.out[q][d1] = s1 ? 1 : -1;
+        e.out[q][d2] = s2 ? 1 : -1;
+        e.mir[q][0] = m0; e.mir[q][1] = m1; e.mir[q][2] = m2;
+    };
+    if (d.jbnd > 10 && d.kbnd > 10) {
+        // src/m_boundary_j_k_edges.F90; the (ny+1,nz+1) edge keeps cx (line 129)
+        const int opt = (d.jbnd == 22 && d.kbnd == 22) ? 1 : -1;
+        add_edge(1, 0, 2, 0, opt, -1, -1);
+        add_edge(1, 1, 2, 0, opt, -1, -1);
+        add_edge(1, 0, 2, 1, opt, -1, -1);
+        add_edge(1, 1, 2, 1, 1, -1, -1);
+    }
+    if (d.ibnd > 10 && d.jbnd > 10) {
+        const int opt = (d.ibnd == 22 && d.jbnd == 22) ? 1 : -1;   // src/m_boundary_i_j_edges.F90
+        add_edge(0, 0, 1, 0, -1, -1, opt);
+        add_edge(0, 1, 1, 0, -1, -1, opt);
+        add_edge(0, 0, 1, 1, -1, -1, opt);
+        add_edge(0, 1, 1, 1, -1, -1, opt);
+    }
+    if (d.ibnd > 10 && d.kbnd > 10) {
+        const int opt = (d.ibnd == 22 && d.kbnd == 22) ? 1 : -1;   // src/m_boundary_i_k_edges.F90
+        add_edge(0, 0, 2, 0, -1, opt, -1);
+        add_edge(0, 1, 2, 0, -1, opt, -1);
+        add_edge(0, 0, 2, 1, -1, opt, -1);
+        add_edge(0, 1, 2, 1, -1, opt, -1);
+    }
+    if (d.ibnd > 10 && d.jbnd > 10 && d.kbnd > 10) {
+        // src/m_boundary_closed_corners.F90:177-186 builds its table with reshape(list,[8,3]),
+        // which fills column major: corner m = (list[m], list[8+m], list[16+m]).  Entries whose
+        // indices are not all ghost indices of their own axis do nothing there; repeated
+        // corners are idempotent.  Keep the distinct real corners of that table.
+        const int nx = d.nx, ny = d.ny, nz = d.nz;
+        const int list[24] = {0, 0, 0, 0, 0, nz + 1, 0, ny + 1, 0, 0, ny + 1, nz + 1,
+                              nx + 1, 0, 0, nx + 1, 0, nz + 1, nx + 1, ny + 1, 0, nx + 1, ny + 1, nz + 1};
+        for (int m = 0; m < 8; ++m) {
+            const int ic = list[m], jc = list[8 + m], kc = list[16 + m];
+            const bool ok = (ic == 0 || ic == nx + 1) && (jc == 0 || jc == ny + 1) && (kc == 0 || kc == nz + 1);
+            if (!ok) continue;
+            bool dup = false;
+            for (int q = 0; q < e.n; ++q)
+                if (e.kind[q] == 1 && e.g[q][0] == ic && e.g[q][1] == jc && e.g[q][2] == kc) dup = true;
+            if (dup) continue;
+            const int q = e.n++;
+            e.kind[q] = 1;
+            e.t[q] = 0;
+            e.len[q] = 1;
+            e.g[q][0] = ic; e.g[q][1] = jc; e.g[q][2] = kc;
+            e.out[q][0] = ic == 0 ? -1 : 1;
+            e.out[q][1] = jc == 0 ? -1 : 1;
+            e.out[q][2] = kc == 0 ? -1 : 1;
+            e.mir[q][0] = e.mir[q][1] = e.mir[q][2] = -1;
+        }
+    }
+    return e;
+}
+
+double utime_at(const lbm_ctx* c, int it) {
+    if (!c->ut_series.empty() && it >= c->series_lo && it <= c->series_hi) return c->ut_series[it - c->series_lo];
+    return c->uini;
+}
+double udir_at(const lbm_ctx* c, int it) {
+    if (!c->ud_series.empty() && it >= c->series_lo && it <= c->series_hi) return c->ud_series[it - c->series_lo];
+    return c->udir;
+}
+
+hipEvent_t next_event(lbm_ctx* c, std::vector<hipEvent_t>& pool, size_t& used) {
+    if (used == pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        pool.push_back(e);
+    }
+    return pool[used++];
+}
+
+// boundarycond(f1, f2) for the non-periodic directions (src/m_boundarycond.F90:166-365).
+int boundary(lbm_ctx* c, void* f1, const void* f2, int it) {
+    const lbm_desc& d = c->d;
+    const Layout& L = c->lay;
+    hipEvent_t e0 = nullptr;
+    if (c->timing) {
+        e0 = next_event(c, c->ev_bc, c->n_bc_ev);
+        if (e0) hipEventRecord(e0, c->stream);
+    }
+    int o1, oN;
+    opts_of(d.ibnd, &o1, &oN);
+    if (o1) HIPCHK(c, c->ops->closed(L, f1, f2, 0, o1, oN, c->stream));
+    opts_of(d.jbnd, &o1, &oN);
+    if (o1) HIPCHK(c, c->ops->closed(L, f1, f2, 1, o1, oN, c->stream));
+    opts_of(d.kbnd, &o1, &oN);
+    if (o1) HIPCHK(c, c->ops->closed(L, f1, f2, 2, o1, oN, c->stream));
+    const EdgeList e = edge_list(d);
+    if (e.n) HIPCHK(c, c->ops->edges(L, f1, f2, e, c->stream));
+    if (d.ibnd == 1) {
+        // cos(udir*pi/180.0) in the reference's REAL precision (src/m_boundary_i_inflow_kernel.F90:71-73)
+        const double ut = round_real(c, utime_at(c, it));
+        double cs, sn;
+        if (c->ops->real_bytes == 4) {
+            const float ang = (float)udir_at(c, it) * 3.1415927410125732f / 180.0f;
+            cs = cosf(ang);
+            sn = sinf(ang);
+        } else {
+            const double ang = udir_at(c, it) * 3.1415927410125732 / 180.0;
+            cs = cos(ang);
+            sn = sin(ang);
+        }
+        HIPCHK(c, c->ops->inflow(L, f1, c->shear.p, ut, cs, sn, d.rho0, c->stream));
+    }
+    if (c->timing && e0) {
+        hipEvent_t e1 = next_event(c, c->ev_bc, c->n_bc_ev);
+        if (e1) hipEventRecord(e1, c->stream);
+    }
+    return LBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* lbm_last_error(const lbm_ctx* ctx) {
+    return ctx ? ctx->err.c_str() : g_create_error.c_str();
+}
+
+int lbm_create(const lbm_desc* desc, lbm_ctx** out) {
+    if (!desc || !out) return fail(nullptr, LBM_EINVAL, "null argument");
+    *out = nullptr;
+    const lbm_desc& d = *desc;
+    if (d.nx < 2 || d.ny < 2 || d.nz < 2) return fail(nullptr, LBM_EINVAL, "nx, ny, nz must be >= 2");
+    if ((long long)(d.nx + 2) * (d.ny + 2) * (d.nz + 2) > (1LL << 30))
+        return fail(nullptr, LBM_EINVAL, "grid too large for 32-bit cell offsets");
+    if (d.q != 19 && d.q != 27) return fail(nullptr, LBM_EINVAL, "q must be 19 or 27");
+    if (d.ibgk != 2 && d.ibgk != 3) return fail(nullptr, LBM_EINVAL, "ibgk must be 2 or 3");
+    auto okb = [](int b) { return b == 0 || b == 11 || b == 12 || b == 21 || b == 22; };
+    if (!(okb(d.ibnd) || d.ibnd == 1) || !okb(d.jbnd) || !okb(d.kbnd))
+        return fail(nullptr, LBM_EINVAL, "boundary codes: 0, 11, 12, 21, 22 (and 1 for ibnd)");
+    if (d.ibnd == 1 && (d.iturb_pos < 1 || d.iturb_pos > d.nx) && d.iturb_pos != 0)
+        return fail(nullptr, LBM_EINVAL, "iturb_pos outside 1..nx");
+    const Ops* ops = nullptr;
+    switch (d.storage) {
+        case LBM_STORE_F32: ops = ops_f32(d.q); break;
+        case LBM_STORE_F64: ops = ops_f64(d.q); break;
+        case LBM_STORE_F16S: ops = ops_f16s(d.q); break;
+        case LBM_STORE_BF16S: ops = ops_bf16s(d.q); break;
+        default: return fail(nullptr, LBM_EINVAL, "unknown storage kind");
+    }
+    int bx = d.block_x, by = d.block_y;
+    if (bx == 0) {
+        const int rows = (d.nx + 63) / 64;
+        bx = rows * 64 <= 512 ? rows * 64 : 256;
+    }
+    if (by == 0) by = bx >= 256 ? 1 : 256 / bx;
+    if (bx % 64 != 0 || bx * by > 1024 || by < 1) return fail(nullptr, LBM_EINVAL, "block_x must be a multiple of 64, block_x*block_y <= 1024");
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev == 0) return fail(nullptr, LBM_EHIP, std::string("no HIP device: ") + hipGetErrorString(e));
+
+    lbm_ctx* c = new lbm_ctx();
+    c->d = d;
+    c->d.block_x = bx;
+    c->d.block_y = by;
+    if (c->d.iturb_pos == 0) c->d.iturb_pos = 10;
+    c->ops = ops;
+    hipGetDevice(&c->device);
+    c->lay = make_layout(d.nx, d.ny, d.nz, ops->raw_bytes, d.ibnd == 0, d.jbnd == 0, d.kbnd == 0);
+    c->tau0 = round_real(c, round_real(c, 3.0 * round_real(c, d.kinevisc)) + 0.5);
+    c->fac0 = round_real(c, 1.0 - round_real(c, 1.0 / c->tau0));
+    auto bail = [&](hipError_t err, const char* where) {
+        std::string m = std::string(where) + ": " + hipGetErrorString(err);
+        lbm_destroy(c);
+        return fail(nullptr, LBM_EHIP, m);
+    };
+    if ((e = hipStreamCreate(&c->stream)) != hipSuccess) return bail(e, "hipStreamCreate");
+    c->buf_bytes = (size_t)c->lay.plane * d.q * ops->raw_bytes;
+    for (int b = 0; b < 2; ++b)
+        if ((e = hipMalloc(&c->buf[b], c->buf_bytes)) != hipSuccess) return bail(e, "hipMalloc(populations)");
+    for (int b = 0; b < 2; ++b)
+        if ((e = hipMemsetAsync(c->buf[b], 0, c->buf_bytes, c->stream)) != hipSuccess) return bail(e, "hipMemset");
+    if (d.store_tau && d.ivreman) {
+        const size_t tb = (size_t)c->lay.plane * ops->real_bytes;
+        if ((e = hipMalloc(&c->tau.p, tb)) != hipSuccess) return bail(e, "hipMalloc(tau)");
+        c->tau.n = tb;
+        if ((e = hipMemsetAsync(c->tau.p, 0, tb, c->stream)) != hipSuccess) return bail(e, "hipMemset(tau)");
+    }
+    // defaults: ablvisc = 0, uvel_shear = 1
+    std::vector<double> zeros(d.nz, 0.0), ones(d.nz, 1.0);
+    if (upload_reals(c, c->ablvisc, zeros.data(), d.nz, 8) || upload_reals(c, c->shear, ones.data(), d.nz, 8)) {
+        std::string m = c->err;
+        lbm_destroy(c);
+        return fail(nullptr, LBM_EHIP, m);
+    }
+    if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return bail(e, "hipStreamSynchronize");
+    *out = c;
+    return LBM_OK;
+}
+
+int lbm_destroy(lbm_ctx* c) {
+    if (!c) return LBM_OK;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (int b = 0; b < 2; ++b)
+        if (c->buf[b]) hipFree(c->buf[b]);
+    DevBuf* all[] = {&c->ablvisc, &c->shear, &c->solid, &c->turb[0], &c->turb[1], &c->turb[2], &c->force[0],
+                     &c->force[1], &c->force[2], &c->tau, &c->ext[0], &c->ext[1], &c->ext[2], &c->ext[3],
+                     &c->stage, &c->partial};
+    for (DevBuf* b : all) dev_free(*b);
+    for (hipEvent_t ev : c->ev_step) hipEventDestroy(ev);
+    for (hipEvent_t ev : c->ev_bc) hipEventDestroy(ev);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return LBM_OK;
+}
+
+int lbm_set_state(lbm_ctx* c, const void* f, int realkind) {
+    if (!c || !f) return fail(c, LBM_EINVAL, "null argument");
+    if (int rc = check_kind(c, realkind)) return rc;
+    const size_t n = ncell_padded(c) * c->d.q;
+    if (int rc = dev_alloc(c, c->stage, n * realkind)) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->stage.p, f, n * realkind, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, c->ops->aos_in(c->lay, c->stage.p, realkind, c->buf[0], c->buf[1], c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->cur = 0;
+    c->have_state = true;
+    c->pending = true;
+    c->use_ext = false;
+    c->ghost_raw[0] = c->ghost_raw[1] = true;
+    return LBM_OK;
+}
+
+int lbm_init_equilibrium(lbm_ctx* c, const void* rho, const void* u, const void* v, const void* w, int realkind,
+                         int it0) {
+    if (!c || !rho || !u || !v || !w) return fail(c, LBM_EINVAL, "null argument");
+    if (int rc = check_kind(c, realkind)) return rc;
+    const size_t nc = ncell_padded(c);
+    const size_t cb = nc * realkind;
+    if (int rc = dev_alloc(c, c->stage, 4 * cb)) return rc;
+    const size_t mb = (size_t)c->lay.plane * c->ops->real_bytes;
+    for (int q = 0; q < 4; ++q)
+        if (int rc = dev_alloc(c, c->ext[q], mb)) return rc;
+    char* st = static_cast<char*>(c->stage.p);
+    HIPCHK(c, hipMemcpyAsync(st, rho, cb, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(st + cb, u, cb, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(st + 2 * cb, v, cb, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(st + 3 * cb, w, cb, hipMemcpyHostToDevice, c->stream));
+    // fB = feq with clamped ghosts; fA = fB; boundarycond(fB,fA); boundarycond(fA,fB)
+    HIPCHK(c, c->ops->feq_init(c->lay, c->d.ibgk == 3, st, st + cb, st + 2 * cb, st + 3 * cb, realkind, c->buf[1],
+                               c->ext[0].p, c->ext[1].p, c->ext[2].p, c->ext[3].p, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->buf[0], c->buf[1], c->buf_bytes, hipMemcpyDeviceToDevice, c->stream));
+    if (int rc = boundary(c, c->buf[1], c->buf[0], it0)) return rc;
+    if (int rc = boundary(c, c->buf[0], c->buf[1], it0)) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->n_bc_ev = 0;
+    c->cur = 0;
+    c->have_state = true;
+    c->pending = true;
+    c->use_ext = true;
+    c->ghost_raw[0] = c->ghost_raw[1] = false;
+    return LBM_OK;
+}
+
+int lbm_step(lbm_ctx* c, int it0, int nsteps) {
+    if (!c) return fail(c, LBM_EINVAL, "null context");
+    if (!c->have_state) return fail(c, LBM_ESTATE, "no state: call lbm_set_state or lbm_init_equilibrium first");
+    if (nsteps < 0) return fail(c, LBM_EINVAL, "nsteps < 0");
+    const lbm_desc& d = c->d;
+    StepHost h{};
+    h.fac0 = c->fac0;
+    h.kinevisc = d.kinevisc;
+    h.vconst = round_real(c, 2.5 * round_real(c, round_real(c, d.smagorinsky) * round_real(c, d.smagorinsky)));
+    h.ablvisc = c->ablvisc.p;
+    h.solid = c->has_solid ? static_cast<const uint32_t*>(c->solid.p) : nullptr;
+    h.solid_wpr = c->solid_wpr;
+    h.ip = d.iturb_pos;
+    h.ampl = c->ampl;
+    for (int q = 0; q < 3; ++q) h.F[q] = c->has_force ? c->force[q].p : nullptr;
+    h.tau_out = (d.store_tau && d.ivreman) ? c->tau.p : nullptr;
+    h.bgk3 = d.ibgk == 3;
+    h.hrr = d.ihrr == 1;
+    h.vre = d.ivreman == 1;
+    h.block_x = d.block_x;
+    h.block_y = d.block_y;
+    const size_t slice = (size_t)d.ny * d.nz * c->ops->real_bytes;
+    for (int it = it0 + 1; it <= it0 + nsteps; ++it) {
+        if (c->pending) {
+            // collide the uploaded state in place: read the copy in the other buffer
+            h.src = c->buf[c->cur ^ 1];
+            h.dst = c->buf[c->cur];
+            h.stream = 0;
+        } else {
+            h.src = c->buf[c->cur];
+            h.dst = c->buf[c->cur ^ 1];
+            h.stream = 1;
+        }
+        for (int q = 0; q < 4; ++q) h.ext[q] = c->use_ext ? c->ext[q].p : nullptr;
+        if (c->has_turb) {
+            int lit = it % c->nrturb;           // src/m_inflow_turbulence_forcing.F90:45-46
+            if (lit == 0) lit = c->nrturb;
+            for (int q = 0; q < 3; ++q) h.turb[q] = static_cast<const char*>(c->turb[q].p) + (size_t)lit * slice;
+        } else {
+            h.turb[0] = h.turb[1] = h.turb[2] = nullptr;
+        }
+        hipEvent_t e0 = nullptr;
+        if (c->timing && (e0 = next_event(c, c->ev_step, c->n_step_ev))) hipEventRecord(e0, c->stream);
+        HIPCHK(c, c->ops->step(c->lay, h, c->stream));
+        if (c->timing && e0) {
+            hipEvent_t e1 = next_event(c, c->ev_step, c->n_step_ev);
+            if (e1) hipEventRecord(e1, c->stream);
+        }
+        if (!c->pending) c->cur ^= 1;
+        c->pending = false;
+        c->use_ext = false;
+        if (int rc = boundary(c, c->buf[c->cur], c->buf[c->cur ^ 1], it)) return rc;
+        c->ghost_raw[c->cur] = false;
+    }
+    return LBM_OK;
+}
+
+int lbm_get_state(lbm_ctx* c, void* f, int realkind) {
+    if (!c || !f) return fail(c, LBM_EINVAL, "null argument");
+    if (!c->have_state) return fail(c, LBM_ESTATE, "no state");
+    if (int rc = check_kind(c, realkind)) return rc;
+    const size_t n = ncell_padded(c) * c->d.q;
+    if (int rc = dev_alloc(c, c->stage, n * realkind)) return rc;
+    const void* cur = c->buf[c->cur];
+    const int g = c->pending ? c->cur : c->cur ^ 1;
+    HIPCHK(c, c->ops->state_out(c->lay, cur, c->buf[g], c->pending ? 0 : 1, c->ghost_raw[g] ? 1 : 0, c->stage.p,
+                                nullptr, nullptr, nullptr, nullptr, realkind, c->stream));
+    HIPCHK(c, hipMemcpyAsync(f, c->stage.p, n * realkind, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+int lbm_get_macros(lbm_ctx* c, void* rho, void* u, void* v, void* w, int realkind) {
+    if (!c || !rho || !u || !v || !w) return fail(c, LBM_EINVAL, "null argument");
+    if (!c->have_state) return fail(c, LBM_ESTATE, "no state");
+    if (int rc = check_kind(c, realkind)) return rc;
+    const size_t nc = ncell_padded(c);
+    const size_t cb = nc * realkind;
+    if (int rc = dev_alloc(c, c->stage, 4 * cb)) return rc;
+    char* st = static_cast<char*>(c->stage.p);
+    const int g = c->pending ? c->cur : c->cur ^ 1;
+    HIPCHK(c, c->ops->state_out(c->lay, c->buf[c->cur], c->buf[g], c->pending ? 0 : 1, c->ghost_raw[g] ? 1 : 0,
+                                nullptr, st, st + cb, st + 2 * cb, st + 3 * cb, realkind, c->stream));
+    HIPCHK(c, hipMemcpyAsync(rho, st, cb, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(u, st + cb, cb, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(v, st + 2 * cb, cb, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(w, st + 3 * cb, cb, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+int lbm_get_tau(lbm_ctx* c, void* tau, int realkind) {
+    if (!c || !tau) return fail(c, LBM_EINVAL, "null argument");
+    if (int rc = check_kind(c, realkind)) return rc;
+    const Layout& L = c->lay;
+    const size_t nc = ncell_padded(c);
+    std::vector<double> out(nc, c->tau0);
+    if (c->d.store_tau && c->d.ivreman && c->tau.p) {
+        const int rb = c->ops->real_bytes;
+        std::vector<unsigned char> dev((size_t)L.plane * rb);
+        HIPCHK(c, hipMemcpyAsync(dev.data(), c->tau.p, dev.size(), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (int k = 1; k <= L.nz; ++k)
+            for (int j = 1; j <= L.ny; ++j)
+                for (int i = 1; i <= L.nx; ++i) {
+                    const size_t o = (size_t)L.cell(i, j, k);
+                    const double v = rb == 8 ? reinterpret_cast<double*>(dev.data())[o] : reinterpret_cast<float*>(dev.data())[o];
+                    out[((size_t)k * (L.ny + 2) + j) * (L.nx + 2) + i] = v;
+                }
+    }
+    for (size_t q = 0; q < nc; ++q) {
+        if (realkind == 8) static_cast<double*>(tau)[q] = out[q];
+        else static_cast<float*>(tau)[q] = (float)out[q];
+    }
+    return LBM_OK;
+}
+
+int lbm_set_blanking(lbm_ctx* c, const uint8_t* mask) {
+    if (!c) return fail(c, LBM_EINVAL, "null context");
+    if (!mask) {
+        c->has_solid = false;
+        return LBM_OK;
+    }
+    const lbm_desc& d = c->d;
+    const int wpr = (d.nx + 31) / 32;
+    std::vector<uint32_t> bits((size_t)wpr * d.ny * d.nz, 0u);
+    bool any = false;
+    for (int k = 1; k <= d.nz; ++k)
+        for (int j = 1; j <= d.ny; ++j)
+            for (int i = 1; i <= d.nx; ++i)
+                if (mask[((size_t)k * (d.ny + 2) + j) * (d.nx + 2) + i]) {
+                    bits[((size_t)(k - 1) * d.ny + (j - 1)) * wpr + (i - 1) / 32] |= 1u << ((i - 1) % 32);
+                    any = true;
+                }
+    if (int rc = dev_alloc(c, c->solid, bits.size() * 4)) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->solid.p, bits.data(), bits.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->solid_wpr = wpr;
+    c->has_solid = any;   // the reference's lsolids (src/main.F90:310)
+    return LBM_OK;
+}
+
+int lbm_set_ablvisc(lbm_ctx* c, const void* abl, int realkind) {
+    if (!c || !abl) return fail(c, LBM_EINVAL, "null argument");
+    if (int rc = check_kind(c, realkind)) return rc;
+    return upload_reals(c, c->ablvisc, abl, c->d.nz, realkind);
+}
+
+int lbm_set_inflow(lbm_ctx* c, const void* shear, double uini, double udir, int realkind) {
+    if (!c) return fail(c, LBM_EINVAL, "null context");
+    if (int rc = check_kind(c, realkind)) return rc;
+    c->uini = uini;
+    c->udir = udir;
+    if (shear) return upload_reals(c, c->shear, shear, c->d.nz, realkind);
+    std::vector<double> ones(c->d.nz, 1.0);
+    return upload_reals(c, c->shear, ones.data(), c->d.nz, 8);
+}
+
+int lbm_set_inflow_series(lbm_ctx* c, int it_lo, int it_hi, const void* ut, const void* ud, int realkind) {
+    if (!c) return fail(c, LBM_EINVAL, "null context");
+    if (int rc = check_kind(c, realkind)) return rc;
+    c->ut_series.clear();
+    c->ud_series.clear();
+    if (!ut || !ud || it_hi < it_lo) return LBM_OK;
+    const int n = it_hi - it_lo + 1;
+    c->series_lo = it_lo;
+    c->series_hi = it_hi;
+    for (int q = 0; q < n; ++q) {
+        c->ut_series.push_back(realkind == 8 ? static_cast<const double*>(ut)[q] : static_cast<const float*>(ut)[q]);
+        c->ud_series.push_back(realkind == 8 ? static_cast<const double*>(ud)[q] : static_cast<const float*>(ud)[q]);
+    }
+    return LBM_OK;
+}
+
+int lbm_set_turbulence(lbm_ctx* c, const void* uu, const void* vv, const void* ww, int nrturb, double ampl,
+                       int realkind) {
+    if (!c) return fail(c, LBM_EINVAL, "null context");
+    if (!uu) {
+        c->has_turb = false;
+        return LBM_OK;
+    }
+    if (!vv || !ww || nrturb < 1) return fail(c, LBM_EINVAL, "turbulence fields and nrturb >= 1 required");
+    if (int rc = check_kind(c, realkind)) return rc;
+    if (c->d.iturb_pos < 1 || c->d.iturb_pos > c->d.nx) return fail(c, LBM_EINVAL, "iturb_pos outside the grid");
+    const size_t n = (size_t)c->d.ny * c->d.nz * (nrturb + 1);
+    const void* src[3] = {uu, vv, ww};
+    for (int q = 0; q < 3; ++q)
+        if (int rc = upload_reals(c, c->turb[q], src[q], n, realkind)) return rc;
+    c->nrturb = nrturb;
+    c->ampl = round_real(c, ampl);
+    c->has_turb = true;
+    return LBM_OK;
+}
+
+int lbm_set_external_forcing(lbm_ctx* c, const void* force, int realkind) {
+    if (!c) return fail(c, LBM_EINVAL, "null context");
+    if (!force) {
+        c->has_force = false;
+        return LBM_OK;
+    }
+    if (int rc = check_kind(c, realkind)) return rc;
+    const Layout& L = c->lay;
+    const size_t nc = ncell_padded(c);
+    const int rb = c->ops->real_bytes;
+    for (int q = 0; q < 3; ++q) {
+        std::vector<unsigned char> plane((size_t)L.plane * rb, 0);
+        for (int k = 0; k <= L.nz + 1; ++k)
+            for (int j = 0; j <= L.ny + 1; ++j)
+                for (int i = 0; i <= L.nx + 1; ++i) {
+                    const size_t hc = ((size_t)k * (L.ny + 2) + j) * (L.nx + 2) + i;
+                    const double v = realkind == 8 ? static_cast<const double*>(force)[3 * hc + q]
+                                                   : static_cast<const float*>(force)[3 * hc + q];
+                    const size_t o = (size_t)L.cell(i, j, k);
+                    if (rb == 8) reinterpret_cast<double*>(plane.data())[o] = v;
+                    else reinterpret_cast<float*>(plane.data())[o] = (float)v;
+                }
+        if (int rc = dev_alloc(c, c->force[q], plane.size())) return rc;
+        HIPCHK(c, hipMemcpyAsync(c->force[q].p, plane.data(), plane.size(), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    (void)nc;
+    c->has_force = true;
+    return LBM_OK;
+}
+
+int lbm_synchronize(lbm_ctx* c) {
+    if (!c) return fail(c, LBM_EINVAL, "null context");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+void* lbm_stream(lbm_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int lbm_set_timing(lbm_ctx* c, int on) {
+    if (!c) return fail(c, LBM_EINVAL, "null context");
+    c->timing = on != 0;
+    c->n_step_ev = c->n_bc_ev = 0;
+    return LBM_OK;
+}
+
+int lbm_timing_read(lbm_ctx* c, double* step_ms, int* n_step, double* bc_ms, int* n_bc) {
+    if (!c) return fail(c, LBM_EINVAL, "null context");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double s = 0, b = 0;
+    for (size_t q = 0; q + 1 < c->n_step_ev; q += 2) {
+        float ms = 0;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev_step[q], c->ev_step[q + 1]));
+        s += ms;
+    }
+    for (size_t q = 0; q + 1 < c->n_bc_ev; q += 2) {
+        float ms = 0;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev_bc[q], c->ev_bc[q + 1]));
+        b += ms;
+    }
+    if (step_ms) *step_ms = s;
+    if (n_step) *n_step = (int)(c->n_step_ev / 2);
+    if (bc_ms) *bc_ms = b;
+    if (n_bc) *n_bc = (int)(c->n_bc_ev / 2);
+    c->n_step_ev = c->n_bc_ev = 0;
+    return LBM_OK;
+}
+
+int lbm_get_totals(lbm_ctx* c, double* mass, double* mom) {
+    if (!c) return fail(c, LBM_EINVAL, "null context");
+    if (!c->have_state) return fail(c, LBM_ESTATE, "no state");
+    const int nb = 1024;
+    if (int rc = dev_alloc(c, c->partial, (size_t)nb * 4 * sizeof(double))) return rc;
+    HIPCHK(c, c->ops->totals(c->lay, c->buf[c->cur], c->pending ? 0 : 1, static_cast<double*>(c->partial.p), nb,
+                             c->stream));
+    std::vector<double> h((size_t)nb * 4);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->partial.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double t[4] = {0, 0, 0, 0};
+    for (int b = 0; b < nb; ++b)
+        for (int q = 0; q < 4; ++q) t[q] += h[(size_t)4 * b + q];
+    if (mass) *mass = t[0];
+    if (mom) { mom[0] = t[1]; mom[1] = t[2]; mom[2] = t[3]; }
+    return LBM_OK;
+}
+
+int lbm_get_info(lbm_ctx* c, int64_t* bytes, int32_t* pitch, int64_t* plane) {
+    if (!c) return fail(c, LBM_EINVAL, "null context");
+    if (bytes) *bytes = (int64_t)(2 * c->buf_bytes);
+    if (pitch) *pitch = c->lay.pitch;
+    if (plane) *plane = c->lay.plane;
+    return LBM_OK;
+}
+
+}  // extern "C"

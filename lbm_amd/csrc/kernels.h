@@ -1,0 +1,483 @@
+// Device kernels of the lattice update.
+//
+// One time step (the reference's src/main.F90:290-329) is
+//   1. step_kernel      : pull-stream + moments + equilibrium + regularization + Vreman tau +
+//                         relaxation + inflow-turbulence df + body-force df + solid swap, one
+//                         pass over the interior, reading buffer P(n-1) and writing P(n);
+//   2. boundary kernels : the non-periodic ghost passes of boundarycond on the buffer just
+//                         written, in the reference's pass order (closed i, j, k faces; edges and
+//                         corners; inflow/outflow).
+// The buffers hold post-collision populations (the reference's f1 after postcoll/solids) in
+// the interior and the reference's persistent ghost layers.  The reference's post-streaming
+// state f1 = drift(P(n)) is formed on the fly by the next step, and by state_out_kernel when the
+// caller asks for it.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "codec.h"
+#include "lattice.h"
+#include "layout.h"
+#include "impl.h"
+
+namespace lbm {
+
+template <typename R>
+struct StepArgs {
+    Layout lay;
+    const void* src;       // buffer holding P(n-1) (or the uploaded state when stream == 0)
+    void* dst;             // buffer receiving P(n)
+    int stream;            // 1: pull-stream from src; 0: collide the state in place
+    R fac0;                // 1 - 1/tau0 when Vreman is off (tau0 = 3*kinevisc + 0.5)
+    R kinevisc, vconst, eps, tiny;
+    const R* ablvisc;      // [nz]
+    const R* ext_rho;      // moments to use instead of the computed ones (first step after
+    const R* ext_u;        //   an equilibrium initialisation, src/main.F90:214,243,299);
+    const R* ext_v;        //   cell layout, nullptr otherwise
+    const R* ext_w;
+    const uint32_t* solid; // one bit per interior cell, rows of solid_wpr words; nullptr: none
+    int solid_wpr;
+    int ip;                // inflow-turbulence slice (reference: 10)
+    const R* tu;           // turbulence slice (ny*nz, j fastest) for this step; nullptr: off
+    const R* tv;
+    const R* tw;
+    R ampl;
+    const R* Fx;           // external forcing (cell layout); nullptr: off
+    const R* Fy;
+    const R* Fz;
+    R* tau_out;            // cell layout; nullptr: tau not stored
+};
+
+// Signed sum  sx*a + sy*b + sz*c  for compile-time signs in {-1,0,1}: adds only.
+template <typename R>
+__device__ __forceinline__ R sdot(int sx, int sy, int sz, R a, R b, R c) {
+    R s = 0;
+    bool first = true;
+    if (sx != 0) { s = sx > 0 ? a : -a; first = false; }
+    if (sy != 0) { s = first ? (sy > 0 ? b : -b) : (sy > 0 ? s + b : s - b); first = false; }
+    if (sz != 0) { s = first ? (sz > 0 ? c : -c) : (sz > 0 ? s + c : s - c); }
+    return s;
+}
+
+// Hermite equilibrium of direction l (src/m_postcoll_kernel.F90:136-183 in closed form):
+//   feq = w rho (1 - 1.5 u.u + cu (3 + 4.5 cu [+ 4.5 (cu^2 - u.u)]))   with cu = c.u
+// The bracket is the reference's full H3:A3 contraction, 4.5 rho cu (cu^2 - u.u), applied for
+// l > 0 when the third order is on.
+template <int Q, bool ORDER3, typename R>
+__device__ __forceinline__ R feq_dir(int l, R rho, R ux, R uy, R uz, R usq) {
+    using Lt = Lattice<Q>;
+    const R cu = sdot<R>(Lt::cx(l), Lt::cy(l), Lt::cz(l), ux, uy, uz);
+    R poly = R(3) + R(4.5) * cu;
+    if (ORDER3 && l > 0) poly += R(4.5) * (cu * cu - usq);
+    return (weight<Q, R>(l) * rho) * ((R(1) - R(1.5) * usq) + cu * poly);
+}
+
+template <int Q, class C, bool BGK3, bool HRR, bool VRE>
+__global__ void __launch_bounds__(1024) step_kernel(const StepArgs<typename C::real> a) {
+    using R = typename C::real;
+    using raw = typename C::raw;
+    using Lt = Lattice<Q>;
+    const Layout& L = a.lay;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > L.nx || j > L.ny || k > L.nz) return;
+
+    // Source offsets: population l comes from (i - cx, j - cy, k - cz), periodic directions
+    // folded onto the interior.  Index d = c + 1.
+    const int s = a.stream;
+    int ox[3], oy[3], oz[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const int c = d - 1;
+        ox[d] = L.xoff + Layout::wrap(i - s * c, L.nx, L.per_i);
+        oy[d] = Layout::wrap(j - s * c, L.ny, L.per_j) * L.pitch;
+        oz[d] = Layout::wrap(k - s * c, L.nz, L.per_k) * L.ystride;
+    }
+    const raw* __restrict__ src = static_cast<const raw*>(a.src);
+    raw rv[Q];
+#pragma unroll
+    for (int l = 0; l < Q; ++l)
+        rv[l] = src[(size_t)l * (size_t)L.plane + (size_t)(oz[Lt::cz(l) + 1] + oy[Lt::cy(l) + 1] + ox[Lt::cx(l) + 1])];
+
+    // Moments in the reference's order (src/m_macrovars_kernel.F90:41-49), plus the second
+    // moments of the weight-shifted populations for the non-equilibrium tensor.
+    R fv[Q];
+    R rho = 0, jx = 0, jy = 0, jz = 0;
+    R dr = 0, pxx = 0, pyy = 0, pzz = 0, pxy = 0, pxz = 0, pyz = 0;
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+        const R f = C::template decode<Q>(rv[l], l);
+        fv[l] = f;
+        rho += f;
+        if (Lt::cx(l) > 0) jx += f; else if (Lt::cx(l) < 0) jx -= f;
+        if (Lt::cy(l) > 0) jy += f; else if (Lt::cy(l) < 0) jy -= f;
+        if (Lt::cz(l) > 0) jz += f; else if (Lt::cz(l) < 0) jz -= f;
+        if (HRR || VRE) {
+            const R d = C::template shifted<Q>(rv[l], l);
+            dr += d;
+            if (Lt::cx(l) != 0) pxx += d;
+            if (Lt::cy(l) != 0) pyy += d;
+            if (Lt::cz(l) != 0) pzz += d;
+            if (Lt::cx(l) * Lt::cy(l) > 0) pxy += d; else if (Lt::cx(l) * Lt::cy(l) < 0) pxy -= d;
+            if (Lt::cx(l) * Lt::cz(l) > 0) pxz += d; else if (Lt::cx(l) * Lt::cz(l) < 0) pxz -= d;
+            if (Lt::cy(l) * Lt::cz(l) > 0) pyz += d; else if (Lt::cy(l) * Lt::cz(l) < 0) pyz -= d;
+        }
+    }
+    const R invrho = R(1) / rho;
+    R rh = rho, ux = jx * invrho, uy = jy * invrho, uz = jz * invrho;
+    const int oc = L.cell(i, j, k);
+    if (a.ext_rho) { rh = a.ext_rho[oc]; ux = a.ext_u[oc]; uy = a.ext_v[oc]; uz = a.ext_w[oc]; }
+    const R usq = ux * ux + uy * uy + uz * uz;
+
+    // Non-equilibrium second-order tensor a2 = sum_l H2_l (f_l - feq_l)
+    //   = Pi' - rho u u - (sum_l (f_l - w_l)) I/3   (Pi' = sum_l c c (f_l - w_l)),
+    // exact for D3Q19/D3Q27 because sum_l c c feq_l = rho u u + rho I/3.  For the regularized
+    // populations the same tensor is the Vreman alpha (src/m_postcoll_kernel.F90:255-268), since
+    // sum_l w_l H2_l H2_l : a2 * 4.5 = a2 on these lattices.
+    R axx = 0, ayy = 0, azz = 0, axy = 0, axz = 0, ayz = 0;
+    if (HRR || VRE) {
+        const R third = R(1) / R(3);
+        const R rux = rh * ux, ruy = rh * uy, ruz = rh * uz;
+        axx = pxx - rux * ux - dr * third;
+        ayy = pyy - ruy * uy - dr * third;
+        azz = pzz - ruz * uz - dr * third;
+        axy = pxy - rux * uy;
+        axz = pxz - rux * uz;
+        ayz = pyz - ruy * uz;
+    }
+
+    R fac = a.fac0;
+    R tau = 0;
+    if (VRE) {
+        // Vreman (2004) eddy viscosity as src/m_postcoll_kernel.F90:270-304 forms it.
+        R amag = axx * axx + ayy * ayy + azz * azz + R(2) * (axy * axy + axz * axz + ayz * ayz);
+        amag = amag > a.tiny ? amag : a.tiny;
+        const R b11 = a.eps + (axx * axx + axy * axy + axz * axz);
+        const R b22 = a.eps + (axy * axy + ayy * ayy + ayz * ayz);
+        const R b33 = a.eps + (axz * axz + ayz * ayz + azz * azz);
+        const R b12 = a.eps + (axx * axy + axy * ayy + axz * ayz);
+        const R b13 = a.eps + (axx * axz + axy * ayz + axz * azz);
+        const R b23 = a.eps + (axy * axz + ayy * ayz + ayz * azz);
+        const R bb = b11 * b22 - b12 * b12 + b11 * b33 - b13 * b13 + b22 * b33 - b23 * b23;
+        const R t = bb / amag;
+        const R eddy = t > a.eps ? a.vconst * sqrt(t) : R(0);
+        tau = R(3) * (a.kinevisc + eddy + a.ablvisc[k - 1]) + R(0.5);
+        fac = R(1) - R(1) / tau;
+    }
+
+    // Regularized non-equilibrium (src/m_postcoll_kernel.F90:189-244) in closed form:
+    //   fneq_l = 4.5 w_l [ cAc (1 + 3 cu) - T (1/3 + cu) - 2 c.g ],  T = tr a2,  g = a2 u
+    const R T = axx + ayy + azz;
+    const R gx = axx * ux + axy * uy + axz * uz;
+    const R gy = axy * ux + ayy * uy + ayz * uz;
+    const R gz = axz * ux + ayz * uy + azz * uz;
+
+    R out[Q];
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+        const int cx = Lt::cx(l), cy = Lt::cy(l), cz = Lt::cz(l);
+        const R feq = feq_dir<Q, BGK3, R>(l, rh, ux, uy, uz, usq);
+        R neq;
+        if (HRR) {
+            const R cu = sdot<R>(cx, cy, cz, ux, uy, uz);
+            R cac = R(0);
+            if (cx != 0) cac += axx;
+            if (cy != 0) cac += ayy;
+            if (cz != 0) cac += azz;
+            const R off = sdot<R>(cx * cy, cx * cz, cy * cz, axy, axz, ayz);
+            cac += R(2) * off;
+            const R cg = sdot<R>(cx, cy, cz, gx, gy, gz);
+            neq = (R(4.5) * weight<Q, R>(l)) * (cac * (R(1) + R(3) * cu) - T * (R(1) / R(3) + cu) - R(2) * cg);
+        } else {
+            neq = fv[l] - feq;
+        }
+        out[l] = feq + fac * neq;
+    }
+
+    // Inflow-turbulence forcing on slice ip (src/m_inflow_turbulence_forcing.F90:57-128 and
+    // src/m_inflow_turbulence_apply.F90:21-26): f += feq(rho, u + ampl*uu) - feq(rho, u).
+    if (a.tu != nullptr && i == a.ip) {
+        const int sl = (k - 1) * L.ny + (j - 1);
+        const R vx = ux + a.ampl * a.tu[sl], vy = uy + a.ampl * a.tv[sl], vz = uz + a.ampl * a.tw[sl];
+        const R vsq = vx * vx + vy * vy + vz * vz;
+#pragma unroll
+        for (int l = 0; l < Q; ++l)
+            out[l] = out[l] + (feq_dir<Q, BGK3, R>(l, rh, vx, vy, vz, vsq) - feq_dir<Q, BGK3, R>(l, rh, ux, uy, uz, usq));
+    }
+
+    // External body force, Kupershtokh exact difference with the 2nd-order equilibrium and
+    // du = -F/rho (src/m_forcings_apply_kernel.F90:142-271); a cell with rho <= 0 is skipped.
+    if (a.Fx != nullptr && rh > R(0)) {
+        const R vx = ux + (-a.Fx[oc] / rh), vy = uy + (-a.Fy[oc] / rh), vz = uz + (-a.Fz[oc] / rh);
+        const R vsq = vx * vx + vy * vy + vz * vz;
+#pragma unroll
+        for (int l = 0; l < Q; ++l)
+            out[l] = out[l] + (feq_dir<Q, false, R>(l, rh, vx, vy, vz, vsq) - feq_dir<Q, false, R>(l, rh, ux, uy, uz, usq));
+    }
+
+    // Solid cells: swap the opposite pairs (2,3),(4,5),... (src/m_solids.F90:19-32).
+    if (a.solid != nullptr) {
+        const uint32_t word = a.solid[((k - 1) * L.ny + (j - 1)) * a.solid_wpr + ((i - 1) >> 5)];
+        if ((word >> ((i - 1) & 31)) & 1u) {
+#pragma unroll
+            for (int l = 1; l + 1 < Q; l += 2) {
+                const R t = out[l];
+                out[l] = out[l + 1];
+                out[l + 1] = t;
+            }
+        }
+    }
+
+    raw* __restrict__ dst = static_cast<raw*>(a.dst);
+#pragma unroll
+    for (int l = 0; l < Q; ++l)
+        dst[(size_t)l * (size_t)L.plane + (size_t)oc] = C::template encode<Q>(out[l], l);
+    if (VRE && a.tau_out != nullptr) a.tau_out[oc] = tau;
+}
+
+// ------------------------------------------------------------------------------------------
+// Boundary kernels (non-periodic ghost layers).  f1 is the buffer the step just wrote, f2 the
+// other one; reads of ghost cells in periodic directions wrap onto the interior.
+// ------------------------------------------------------------------------------------------
+template <int Q, class C>
+__device__ __forceinline__ typename C::real ld(const Layout& L, const void* f, int l, int off) {
+    return C::template decode<Q>(static_cast<const typename C::raw*>(f)[(size_t)l * (size_t)L.plane + (size_t)off], l);
+}
+template <int Q, class C>
+__device__ __forceinline__ void st(const Layout& L, void* f, int l, int off, typename C::real v) {
+    static_cast<typename C::raw*>(f)[(size_t)l * (size_t)L.plane + (size_t)off] = C::template encode<Q>(v, l);
+}
+// Raw copy (no decode/encode round trip) between two cells of possibly different buffers.
+template <int Q, class C>
+__device__ __forceinline__ void cp(const Layout& L, void* fdst, int ldst, int odst, const void* fsrc, int lsrc, int osrc) {
+    using raw = typename C::raw;
+    (void)ldst;
+    static_cast<raw*>(fdst)[(size_t)ldst * (size_t)L.plane + (size_t)odst] =
+        static_cast<const raw*>(fsrc)[(size_t)lsrc * (size_t)L.plane + (size_t)osrc];
+}
+
+// Reflected direction for a wall normal to axis d: normal component negated, tangential
+// components multiplied by opt (-1 no-slip, +1 free-slip).
+template <int Q>
+__host__ __device__ constexpr int mirror(int l, int d, int opt) {
+    using Lt = Lattice<Q>;
+    return Lt::find(d == 0 ? -Lt::cx(l) : opt * Lt::cx(l), d == 1 ? -Lt::cy(l) : opt * Lt::cy(l),
+                    d == 2 ? -Lt::cz(l) : opt * Lt::cz(l));
+}
+
+// Closed faces normal to axis DIR, both sides in one launch (blockIdx.z = side):
+//   src/m_boundary_i_closed_kernel.F90:44-58, _j_:103-114, _k_:157-168.
+template <int Q, class C, int DIR>
+__global__ void bc_closed_kernel(Layout L, void* f1, const void* f2, int opt_lo, int opt_hi) {
+    using Lt = Lattice<Q>;
+    const int ta = blockIdx.x * blockDim.x + threadIdx.x + 1;   // first tangential index
+    const int tb = blockIdx.y * blockDim.y + threadIdx.y + 1;   // second tangential index
+    const int side = blockIdx.z;
+    const int n = DIR == 0 ? L.nx : DIR == 1 ? L.ny : L.nz;
+    const int na = DIR == 0 ? L.ny : L.nx;
+    const int nb = DIR == 2 ? L.ny : L.nz;
+    if (ta > na || tb > nb) return;
+    const int plane = side ? n : 1, ghost = side ? n + 1 : 0, opt = side ? opt_hi : opt_lo;
+    int g[3];
+    if (DIR == 0) { g[0] = ghost; g[1] = ta; g[2] = tb; }
+    else if (DIR == 1) { g[0] = ta; g[1] = ghost; g[2] = tb; }
+    else { g[0] = ta; g[1] = tb; g[2] = ghost; }
+    const int og = L.cell(g[0], g[1], g[2]);
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+        const int cn = Lt::c(l, DIR);
+        if (cn == 0) continue;
+        if ((side == 0 && cn > 0) || (side == 1 && cn < 0)) continue;
+        const int m = opt < 0 ? mirror<Q>(l, DIR, -1) : mirror<Q>(l, DIR, +1);
+        cp<Q, C>(L, f1, m, og, f2, l, og);
+        const int si = DIR == 0 ? plane : g[0] - Lt::cx(l);
+        const int sj = DIR == 1 ? plane : g[1] - Lt::cy(l);
+        const int sk = DIR == 2 ? plane : g[2] - Lt::cz(l);
+        cp<Q, C>(L, f1, l, og, f1, l, L.cell_w(si, sj, sk));
+    }
+}
+
+// Ghost edges and corners: both passes of src/m_boundary_{j_k,i_j,i_k}_edges.F90 and
+// src/m_boundary_closed_corners.F90.  blockIdx.y selects the entry of the list.  Per cell:
+//   pass 1: f1(m, g) = f2(l, g) for each outgoing l, m = mirrored l;
+//   pass 2: f1(l, g) = f1(l, g - c_l along the closed axes).
+// The two passes write disjoint directions and read only the other buffer and interior cells,
+// so one thread performs both, and different entries never touch the same cell.
+template <int Q, class C>
+__global__ void bc_edges_kernel(Layout L, void* f1, const void* f2, const EdgeList e) {
+    using Lt = Lattice<Q>;
+    const int s = blockIdx.y;
+    const int a = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    if (a > e.len[s]) return;
+    int g[3] = {e.g[s][0], e.g[s][1], e.g[s][2]};
+    if (e.kind[s] == 0) g[e.t[s]] = a;
+    const int og = L.cell(g[0], g[1], g[2]);
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+        const int c[3] = {Lt::cx(l), Lt::cy(l), Lt::cz(l)};
+        bool outgoing = true;
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            if (e.out[s][d] != 0 && c[d] * e.out[s][d] <= 0) outgoing = false;
+        if (!outgoing) continue;
+        const int m = Lt::find(e.mir[s][0] * c[0], e.mir[s][1] * c[1], e.mir[s][2] * c[2]);
+        if (m >= 0) cp<Q, C>(L, f1, m, og, f2, l, og);
+        int src[3] = {g[0], g[1], g[2]};
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            if (e.out[s][d] != 0) src[d] = g[d] - c[d];
+        cp<Q, C>(L, f1, l, og, f1, l, L.cell(src[0], src[1], src[2]));
+    }
+}
+
+// Inflow at i = 0 and zero-gradient outflow at i = nx+1 (src/m_boundary_i_inflow_kernel.F90:61-111).
+template <int Q, class C>
+__global__ void bc_inflow_kernel(Layout L, void* f1, const typename C::real* shear, typename C::real utime,
+                                 typename C::real cosd, typename C::real sind, typename C::real rho0) {
+    using R = typename C::real;
+    using Lt = Lattice<Q>;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int k = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (j > L.ny || k > L.nz) return;
+    const R uu = utime * shear[k - 1];        // uvel(k) = uvel_time(it)*uvel_shear(k), taper = 1
+    const R cs2 = R(1) / R(3);
+    const int o1 = L.cell(1, j, k), o0 = L.cell(0, j, k), on = L.cell(L.nx, j, k), oe = L.cell(L.nx + 1, j, k);
+    R fg[Q];
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+        const R wl = weight<Q, R>(l);
+        const R cxl = R(Lt::cx(l)), cyl = R(Lt::cy(l));
+        fg[l] = ld<Q, C>(L, f1, l, o1) - R(2) * wl * rho0 * (cxl * uu * cosd + cyl * uu * sind) / cs2;
+    }
+#pragma unroll
+    for (int l = 0; l < Q; ++l)
+        st<Q, C>(L, f1, l, o0, Lt::cx(l) <= 0 ? fg[l] : fg[Lt::opposite(l)]);
+#pragma unroll
+    for (int l = 0; l < Q; ++l)
+        cp<Q, C>(L, f1, l, oe, f1, l, on);
+}
+
+// ------------------------------------------------------------------------------------------
+// State transfer between the reference's AoS layout f(nl,0:nx+1,0:ny+1,0:nz+1) (H = float or
+// double, as the caller's REAL kind) and the device buffers.
+// ------------------------------------------------------------------------------------------
+template <int Q, class C, typename H>
+__global__ void aos_to_soa_kernel(Layout L, const H* __restrict__ aos, void* dstA, void* dstB) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    const int k = blockIdx.z;
+    if (i > L.nx + 1) return;
+    const size_t base = ((size_t)k * (L.ny + 2) * (L.nx + 2) + (size_t)j * (L.nx + 2) + i) * Q;
+    const int oc = L.cell(i, j, k);
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+        const typename C::real v = (typename C::real)aos[base + l];
+        st<Q, C>(L, dstA, l, oc, v);
+        if (dstB) st<Q, C>(L, dstB, l, oc, v);
+    }
+}
+
+// The reference's f1 (post-streaming state including ghosts) and/or its moments
+// (src/m_macrovars_kernel.F90:41-49, all cells including ghosts).
+//   interior: stream_interior ? cur(l, x - c) : cur(l, x)
+//   ghosts  : ghost_raw ? gsrc(l, x) as stored : gsrc(l, x with periodic axes wrapped)
+template <int Q, class C, typename H>
+__global__ void state_out_kernel(Layout L, const void* cur, const void* gsrc, int stream_interior, int ghost_raw,
+                                 H* __restrict__ aos, H* rho, H* u, H* v, H* w) {
+    using R = typename C::real;
+    using Lt = Lattice<Q>;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    const int k = blockIdx.z;
+    if (i > L.nx + 1) return;
+    const bool interior = i >= 1 && i <= L.nx && j >= 1 && j <= L.ny && k >= 1 && k <= L.nz;
+    const size_t cidx = (size_t)k * (L.ny + 2) * (L.nx + 2) + (size_t)j * (L.nx + 2) + i;
+    R tr = 0, tu = 0, tv = 0, tw = 0;
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+        R f;
+        if (interior) {
+            const int s = stream_interior;
+            f = ld<Q, C>(L, cur, l, L.cell_w(i - s * Lt::cx(l), j - s * Lt::cy(l), k - s * Lt::cz(l)));
+        } else {
+            f = ld<Q, C>(L, gsrc, l, ghost_raw ? L.cell(i, j, k) : L.cell_w(i, j, k));
+        }
+        if (aos) aos[cidx * Q + l] = (H)f;
+        tr += f;
+        if (Lt::cx(l) > 0) tu += f; else if (Lt::cx(l) < 0) tu -= f;
+        if (Lt::cy(l) > 0) tv += f; else if (Lt::cy(l) < 0) tv -= f;
+        if (Lt::cz(l) > 0) tw += f; else if (Lt::cz(l) < 0) tw -= f;
+    }
+    if (rho) {
+        const R inv = R(1) / tr;
+        rho[cidx] = (H)tr;
+        u[cidx] = (H)(tu * inv);
+        v[cidx] = (H)(tv * inv);
+        w[cidx] = (H)(tw * inv);
+    }
+}
+
+// Equilibrium initialisation (src/m_fequil3_kernel.F90 then src/m_fillghosts.F90): every cell,
+// ghosts included, gets feq of its clamped interior cell.  The moments (H, reference cell
+// layout) are also kept in the device cell layout for the first collision.
+template <int Q, class C, bool BGK3, typename H>
+__global__ void feq_init_kernel(Layout L, const H* rho, const H* u, const H* v, const H* w, void* dst,
+                                typename C::real* mr, typename C::real* mu, typename C::real* mv,
+                                typename C::real* mw) {
+    using R = typename C::real;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    const int k = blockIdx.z;
+    if (i > L.nx + 1) return;
+    const int i1 = i < 1 ? 1 : (i > L.nx ? L.nx : i);
+    const int j1 = j < 1 ? 1 : (j > L.ny ? L.ny : j);
+    const int k1 = k < 1 ? 1 : (k > L.nz ? L.nz : k);
+    const size_t c = (size_t)k1 * (L.ny + 2) * (L.nx + 2) + (size_t)j1 * (L.nx + 2) + i1;
+    const R r = (R)rho[c], ux = (R)u[c], uy = (R)v[c], uz = (R)w[c];
+    const R usq = ux * ux + uy * uy + uz * uz;
+    const int oc = L.cell(i, j, k);
+#pragma unroll
+    for (int l = 0; l < Q; ++l)
+        st<Q, C>(L, dst, l, oc, feq_dir<Q, BGK3, R>(l, r, ux, uy, uz, usq));
+    if (i == i1 && j == j1 && k == k1) {
+        mr[oc] = r; mu[oc] = ux; mv[oc] = uy; mw[oc] = uz;
+    }
+}
+
+// Global mass and momentum of the state (interior cells) in FP64: per-block partial sums,
+// finished on the host.  partial[4*block + {0,1,2,3}] = (mass, mx, my, mz).
+template <int Q, class C>
+__global__ void __launch_bounds__(256) totals_kernel(Layout L, const void* cur, int stream_interior, double* partial) {
+    using Lt = Lattice<Q>;
+    __shared__ double sh[4][256];
+    const long long ncell = (long long)L.nx * L.ny * L.nz;
+    double m = 0, mx = 0, my = 0, mz = 0;
+    const int s = stream_interior;
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(c % L.nx) + 1;
+        const int j = (int)((c / L.nx) % L.ny) + 1;
+        const int k = (int)(c / ((long long)L.nx * L.ny)) + 1;
+#pragma unroll
+        for (int l = 0; l < Q; ++l) {
+            const double f = (double)ld<Q, C>(L, cur, l, L.cell_w(i - s * Lt::cx(l), j - s * Lt::cy(l), k - s * Lt::cz(l)));
+            m += f;
+            mx += Lt::cx(l) * f;
+            my += Lt::cy(l) * f;
+            mz += Lt::cz(l) * f;
+        }
+    }
+    const int t = threadIdx.x;
+    sh[0][t] = m; sh[1][t] = mx; sh[2][t] = my; sh[3][t] = mz;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (t < w)
+            for (int q = 0; q < 4; ++q) sh[q][t] += sh[q][t + w];
+        __syncthreads();
+    }
+    if (t == 0)
+        for (int q = 0; q < 4; ++q) partial[4 * blockIdx.x + q] = sh[q][0];
+}
+
+}  // namespace lbm

@@ -1,0 +1,167 @@
+// Host launch wrappers for one (lattice, codec) pair; included once per codec translation unit.
+#pragma once
+
+#include <cfloat>
+#include <cmath>
+
+#include "impl.h"
+#include "kernels.h"
+
+namespace lbm {
+
+template <typename R>
+struct Limits;
+template <>
+struct Limits<float> {
+    static float tiny() { return FLT_MIN; }
+    static float eps() { return sqrtf(FLT_MIN); }   // sqrt(tiny(1.0)), src/m_postcoll.F90:47
+};
+template <>
+struct Limits<double> {
+    static double tiny() { return DBL_MIN; }
+    static double eps() { return sqrt(DBL_MIN); }
+};
+
+inline dim3 grid_cells(const Layout& L, int bx) { return dim3((L.nx + 2 + bx - 1) / bx, L.ny + 2, L.nz + 2); }
+
+template <int Q, class C>
+struct Launch {
+    using R = typename C::real;
+
+    template <bool B3, bool H, bool V>
+    static hipError_t step_t(const Layout& L, const StepArgs<R>& a, dim3 grid, dim3 block, hipStream_t s) {
+        hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V>), grid, block, 0, s, a);
+        return hipGetLastError();
+    }
+
+    static hipError_t step(const Layout& L, const StepHost& h, hipStream_t s) {
+        StepArgs<R> a;
+        a.lay = L;
+        a.src = h.src;
+        a.dst = h.dst;
+        a.stream = h.stream;
+        a.fac0 = (R)h.fac0;
+        a.kinevisc = (R)h.kinevisc;
+        a.vconst = (R)h.vconst;
+        a.eps = Limits<R>::eps();
+        a.tiny = Limits<R>::tiny();
+        a.ablvisc = static_cast<const R*>(h.ablvisc);
+        a.ext_rho = static_cast<const R*>(h.ext[0]);
+        a.ext_u = static_cast<const R*>(h.ext[1]);
+        a.ext_v = static_cast<const R*>(h.ext[2]);
+        a.ext_w = static_cast<const R*>(h.ext[3]);
+        a.solid = h.solid;
+        a.solid_wpr = h.solid_wpr;
+        a.ip = h.ip;
+        a.tu = static_cast<const R*>(h.turb[0]);
+        a.tv = static_cast<const R*>(h.turb[1]);
+        a.tw = static_cast<const R*>(h.turb[2]);
+        a.ampl = (R)h.ampl;
+        a.Fx = static_cast<const R*>(h.F[0]);
+        a.Fy = static_cast<const R*>(h.F[1]);
+        a.Fz = static_cast<const R*>(h.F[2]);
+        a.tau_out = static_cast<R*>(h.tau_out);
+        const dim3 block(h.block_x, h.block_y, 1);
+        const dim3 grid((L.nx + h.block_x - 1) / h.block_x, (L.ny + h.block_y - 1) / h.block_y, L.nz);
+        const int key = (h.bgk3 ? 4 : 0) | (h.hrr ? 2 : 0) | (h.vre ? 1 : 0);
+        switch (key) {
+            case 0: return step_t<false, false, false>(L, a, grid, block, s);
+            case 1: return step_t<false, false, true>(L, a, grid, block, s);
+            case 2: return step_t<false, true, false>(L, a, grid, block, s);
+            case 3: return step_t<false, true, true>(L, a, grid, block, s);
+            case 4: return step_t<true, false, false>(L, a, grid, block, s);
+            case 5: return step_t<true, false, true>(L, a, grid, block, s);
+            case 6: return step_t<true, true, false>(L, a, grid, block, s);
+            default: return step_t<true, true, true>(L, a, grid, block, s);
+        }
+    }
+
+    static hipError_t closed(const Layout& L, void* f1, const void* f2, int dir, int opt_lo, int opt_hi, hipStream_t s) {
+        const int na = dir == 0 ? L.ny : L.nx;
+        const int nb = dir == 2 ? L.ny : L.nz;
+        const dim3 block(64, 4, 1);
+        const dim3 grid((na + 63) / 64, (nb + 3) / 4, 2);
+        if (dir == 0) hipLaunchKernelGGL((bc_closed_kernel<Q, C, 0>), grid, block, 0, s, L, f1, f2, opt_lo, opt_hi);
+        else if (dir == 1) hipLaunchKernelGGL((bc_closed_kernel<Q, C, 1>), grid, block, 0, s, L, f1, f2, opt_lo, opt_hi);
+        else hipLaunchKernelGGL((bc_closed_kernel<Q, C, 2>), grid, block, 0, s, L, f1, f2, opt_lo, opt_hi);
+        return hipGetLastError();
+    }
+
+    static hipError_t edges(const Layout& L, void* f1, const void* f2, const EdgeList& e, hipStream_t s) {
+        if (e.n == 0) return hipSuccess;
+        int mx = 1;
+        for (int q = 0; q < e.n; ++q) mx = e.len[q] > mx ? e.len[q] : mx;
+        const dim3 block(64, 1, 1);
+        const dim3 grid((mx + 63) / 64, e.n, 1);
+        hipLaunchKernelGGL((bc_edges_kernel<Q, C>), grid, block, 0, s, L, f1, f2, e);
+        return hipGetLastError();
+    }
+
+    static hipError_t inflow(const Layout& L, void* f1, const void* shear, double utime, double cosd, double sind,
+                             double rho0, hipStream_t s) {
+        const dim3 block(64, 4, 1);
+        const dim3 grid((L.ny + 63) / 64, (L.nz + 3) / 4, 1);
+        hipLaunchKernelGGL((bc_inflow_kernel<Q, C>), grid, block, 0, s, L, f1, static_cast<const R*>(shear), (R)utime,
+                           (R)cosd, (R)sind, (R)rho0);
+        return hipGetLastError();
+    }
+
+    static hipError_t aos_in(const Layout& L, const void* aos, int realkind, void* dA, void* dB, hipStream_t s) {
+        const dim3 block(128, 1, 1);
+        const dim3 grid = grid_cells(L, 128);
+        if (realkind == 8)
+            hipLaunchKernelGGL((aos_to_soa_kernel<Q, C, double>), grid, block, 0, s, L, static_cast<const double*>(aos), dA, dB);
+        else
+            hipLaunchKernelGGL((aos_to_soa_kernel<Q, C, float>), grid, block, 0, s, L, static_cast<const float*>(aos), dA, dB);
+        return hipGetLastError();
+    }
+
+    static hipError_t state_out(const Layout& L, const void* cur, const void* gsrc, int si, int raw, void* aos,
+                                void* rho, void* u, void* v, void* w, int realkind, hipStream_t s) {
+        const dim3 block(128, 1, 1);
+        const dim3 grid = grid_cells(L, 128);
+        if (realkind == 8)
+            hipLaunchKernelGGL((state_out_kernel<Q, C, double>), grid, block, 0, s, L, cur, gsrc, si, raw,
+                               static_cast<double*>(aos), static_cast<double*>(rho), static_cast<double*>(u),
+                               static_cast<double*>(v), static_cast<double*>(w));
+        else
+            hipLaunchKernelGGL((state_out_kernel<Q, C, float>), grid, block, 0, s, L, cur, gsrc, si, raw,
+                               static_cast<float*>(aos), static_cast<float*>(rho), static_cast<float*>(u),
+                               static_cast<float*>(v), static_cast<float*>(w));
+        return hipGetLastError();
+    }
+
+    template <typename H>
+    static hipError_t feq_init_h(const Layout& L, int bgk3, const void* r, const void* u, const void* v, const void* w,
+                                 void* dst, void* mr, void* mu, void* mv, void* mw, hipStream_t s) {
+        const dim3 block(128, 1, 1);
+        const dim3 grid = grid_cells(L, 128);
+        const H *hr = static_cast<const H*>(r), *hu = static_cast<const H*>(u), *hv = static_cast<const H*>(v),
+                *hw = static_cast<const H*>(w);
+        R *pr = static_cast<R*>(mr), *pu = static_cast<R*>(mu), *pv = static_cast<R*>(mv), *pw = static_cast<R*>(mw);
+        if (bgk3)
+            hipLaunchKernelGGL((feq_init_kernel<Q, C, true, H>), grid, block, 0, s, L, hr, hu, hv, hw, dst, pr, pu, pv, pw);
+        else
+            hipLaunchKernelGGL((feq_init_kernel<Q, C, false, H>), grid, block, 0, s, L, hr, hu, hv, hw, dst, pr, pu, pv, pw);
+        return hipGetLastError();
+    }
+
+    static hipError_t feq_init(const Layout& L, int bgk3, const void* r, const void* u, const void* v, const void* w,
+                               int realkind, void* dst, void* mr, void* mu, void* mv, void* mw, hipStream_t s) {
+        if (realkind == 8) return feq_init_h<double>(L, bgk3, r, u, v, w, dst, mr, mu, mv, mw, s);
+        return feq_init_h<float>(L, bgk3, r, u, v, w, dst, mr, mu, mv, mw, s);
+    }
+
+    static hipError_t totals(const Layout& L, const void* cur, int si, double* partial, int nblocks, hipStream_t s) {
+        hipLaunchKernelGGL((totals_kernel<Q, C>), dim3(nblocks), dim3(256), 0, s, L, cur, si, partial);
+        return hipGetLastError();
+    }
+
+    static const Ops* ops() {
+        static const Ops o = {&step, &closed, &edges, &inflow, &aos_in, &state_out, &feq_init, &totals,
+                              (int)sizeof(R), (int)sizeof(typename C::raw)};
+        return &o;
+    }
+};
+
+}  // namespace lbm

@@ -1,0 +1,240 @@
+"""Python front end of liblbm_hip.so (include/lbm_hip.h) through ctypes.
+
+``Domain`` mirrors the reference driver's use of its per-step operators (src/main.F90):
+    set_state        readrestart -> macrovars -> fB = fA      (src/main.F90:255-258)
+    init_equilibrium fequil3, fillghosts, fA = fB, 2x boundarycond (src/main.F90:243-247)
+    step             the time-loop body                        (src/main.F90:290-329)
+    get_state        f1 as saverestart/testing write it        (src/m_saverestart.F90:125-136)
+    macrovars        rho,u,v,w of f1 incl. ghosts              (src/m_macrovars.F90:4)
+Arrays use the reference's layouts (C-contiguous numpy views of the Fortran arrays):
+    f (nz+2, ny+2, nx+2, q), cell fields (nz+2, ny+2, nx+2), forcing (nz+2, ny+2, nx+2, 3),
+    turbulence (nrturb+1, nz, ny).
+The library must be built (python -m lbm_amd.build); there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblbm_hip.so")
+
+STORAGE = {"f32": 0, "f64": 1, "f16s": 2, "bf16s": 3}
+
+EXPORTS = ["lbm_create", "lbm_destroy", "lbm_last_error", "lbm_set_state", "lbm_init_equilibrium",
+           "lbm_get_state", "lbm_get_macros", "lbm_get_tau", "lbm_set_blanking", "lbm_set_ablvisc",
+           "lbm_set_inflow", "lbm_set_inflow_series", "lbm_set_turbulence", "lbm_set_external_forcing",
+           "lbm_step", "lbm_synchronize", "lbm_stream", "lbm_set_timing", "lbm_timing_read",
+           "lbm_get_totals", "lbm_get_info"]
+
+
+class Desc(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("q", C.c_int32),
+                ("storage", C.c_int32), ("ibgk", C.c_int32), ("ihrr", C.c_int32), ("ivreman", C.c_int32),
+                ("ibnd", C.c_int32), ("jbnd", C.c_int32), ("kbnd", C.c_int32), ("iturb_pos", C.c_int32),
+                ("store_tau", C.c_int32), ("block_x", C.c_int32), ("block_y", C.c_int32),
+                ("kinevisc", C.c_double), ("smagorinsky", C.c_double), ("rho0", C.c_double)]
+
+
+class LBMError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load liblbm_hip.so and declare its signatures.  Raises if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise LBMError(f"{path} not found: build it with `python -m lbm_amd.build`")
+    lib = C.CDLL(path)
+    vp, i32, dbl = C.c_void_p, C.c_int, C.c_double
+    sig = {
+        "lbm_create": ([C.POINTER(Desc), C.POINTER(vp)], i32),
+        "lbm_destroy": ([vp], i32),
+        "lbm_last_error": ([vp], C.c_char_p),
+        "lbm_set_state": ([vp, vp, i32], i32),
+        "lbm_init_equilibrium": ([vp, vp, vp, vp, vp, i32, i32], i32),
+        "lbm_get_state": ([vp, vp, i32], i32),
+        "lbm_get_macros": ([vp, vp, vp, vp, vp, i32], i32),
+        "lbm_get_tau": ([vp, vp, i32], i32),
+        "lbm_set_blanking": ([vp, vp], i32),
+        "lbm_set_ablvisc": ([vp, vp, i32], i32),
+        "lbm_set_inflow": ([vp, vp, dbl, dbl, i32], i32),
+        "lbm_set_inflow_series": ([vp, i32, i32, vp, vp, i32], i32),
+        "lbm_set_turbulence": ([vp, vp, vp, vp, i32, dbl, i32], i32),
+        "lbm_set_external_forcing": ([vp, vp, i32], i32),
+        "lbm_step": ([vp, i32, i32], i32),
+        "lbm_synchronize": ([vp], i32),
+        "lbm_stream": ([vp], vp),
+        "lbm_set_timing": ([vp, i32], i32),
+        "lbm_timing_read": ([vp, C.POINTER(dbl), C.POINTER(i32), C.POINTER(dbl), C.POINTER(i32)], i32),
+        "lbm_get_totals": ([vp, C.POINTER(dbl), C.POINTER(dbl)], i32),
+        "lbm_get_info": ([vp, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_int64)], i32),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Domain:
+    """One lattice-update context on the current HIP device."""
+
+    def __init__(self, q, dims, storage="f32", ibgk=2, ihrr=0, ivreman=0, smagorinsky=0.0,
+                 kinevisc=1.0 / 30.0, ibnd=0, jbnd=0, kbnd=0, rho0=1.0, iturb_pos=10, store_tau=False,
+                 block=(0, 0)):
+        self.lib = load_library()
+        nx, ny, nz = dims
+        self.q, self.dims, self.storage = q, (nx, ny, nz), storage
+        self.real = np.float64 if storage == "f64" else np.float32
+        d = Desc(nx, ny, nz, q, STORAGE[storage], ibgk, ihrr, ivreman, ibnd, jbnd, kbnd, iturb_pos,
+                 int(store_tau), block[0], block[1], float(kinevisc), float(smagorinsky), float(rho0))
+        h = C.c_void_p()
+        rc = self.lib.lbm_create(C.byref(d), C.byref(h))
+        if rc != 0:
+            raise LBMError(f"lbm_create failed ({rc}): {self.lib.lbm_last_error(None).decode()}")
+        self.h = h
+        self.desc = d
+
+    # -- plumbing -----------------------------------------------------------------------
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise LBMError(f"{what} failed ({rc}): {self.lib.lbm_last_error(self.h).decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.lbm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def shape_f(self):
+        nx, ny, nz = self.dims
+        return (nz + 2, ny + 2, nx + 2, self.q)
+
+    @property
+    def shape_cell(self):
+        nx, ny, nz = self.dims
+        return (nz + 2, ny + 2, nx + 2)
+
+    def _arr(self, a, shape, dtype=None):
+        a = np.ascontiguousarray(a, dtype=dtype or (a.dtype if a.dtype in (np.float32, np.float64) else np.float64))
+        if tuple(a.shape) != tuple(shape):
+            raise ValueError(f"expected shape {shape}, got {a.shape}")
+        return a
+
+    # -- state --------------------------------------------------------------------------
+    def set_state(self, f):
+        f = self._arr(f, self.shape_f)
+        self._chk(self.lib.lbm_set_state(self.h, _p(f), f.itemsize), "lbm_set_state")
+
+    def init_equilibrium(self, rho, u, v, w, it0=0):
+        arrs = [self._arr(a, self.shape_cell, np.float64) for a in (rho, u, v, w)]
+        self._chk(self.lib.lbm_init_equilibrium(self.h, *[_p(a) for a in arrs], 8, it0), "lbm_init_equilibrium")
+
+    def get_state(self, dtype=None):
+        out = np.empty(self.shape_f, dtype=dtype or self.real)
+        self._chk(self.lib.lbm_get_state(self.h, _p(out), out.itemsize), "lbm_get_state")
+        return out
+
+    def macrovars(self, dtype=None):
+        dt = dtype or self.real
+        r, u, v, w = (np.empty(self.shape_cell, dt) for _ in range(4))
+        self._chk(self.lib.lbm_get_macros(self.h, _p(r), _p(u), _p(v), _p(w), r.itemsize), "lbm_get_macros")
+        return r, u, v, w
+
+    def tau(self):
+        t = np.empty(self.shape_cell, self.real)
+        self._chk(self.lib.lbm_get_tau(self.h, _p(t), t.itemsize), "lbm_get_tau")
+        return t
+
+    def totals(self):
+        m = C.c_double()
+        mom = (C.c_double * 3)()
+        self._chk(self.lib.lbm_get_totals(self.h, C.byref(m), mom), "lbm_get_totals")
+        return m.value, np.array(mom[:])
+
+    def info(self):
+        b, p, pl = C.c_int64(), C.c_int32(), C.c_int64()
+        self._chk(self.lib.lbm_get_info(self.h, C.byref(b), C.byref(p), C.byref(pl)), "lbm_get_info")
+        return dict(device_bytes=b.value, row_pitch=p.value, plane=pl.value)
+
+    # -- inputs -------------------------------------------------------------------------
+    def set_blanking(self, mask):
+        if mask is None:
+            self._chk(self.lib.lbm_set_blanking(self.h, None), "lbm_set_blanking")
+            return
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        if m.shape != self.shape_cell:
+            raise ValueError("mask shape")
+        self._chk(self.lib.lbm_set_blanking(self.h, _p(m)), "lbm_set_blanking")
+
+    def set_ablvisc(self, abl):
+        a = np.ascontiguousarray(abl, dtype=np.float64)
+        self._chk(self.lib.lbm_set_ablvisc(self.h, _p(a), 8), "lbm_set_ablvisc")
+
+    def set_inflow(self, uini, udir=0.0, shear=None):
+        s = None if shear is None else np.ascontiguousarray(shear, dtype=np.float64)
+        self._chk(self.lib.lbm_set_inflow(self.h, _p(s), float(uini), float(udir), 8), "lbm_set_inflow")
+
+    def set_inflow_series(self, it_lo, uvel_time, udir_time):
+        ut = np.ascontiguousarray(uvel_time, dtype=np.float64)
+        ud = np.ascontiguousarray(udir_time, dtype=np.float64)
+        self._chk(self.lib.lbm_set_inflow_series(self.h, it_lo, it_lo + len(ut) - 1, _p(ut), _p(ud), 8),
+                  "lbm_set_inflow_series")
+
+    def set_turbulence(self, uu, vv, ww, nrturb, ampl):
+        if uu is None:
+            self._chk(self.lib.lbm_set_turbulence(self.h, None, None, None, 0, 0.0, 8), "lbm_set_turbulence")
+            return
+        dt = np.float64 if self.real == np.float64 else np.float32
+        a = [np.ascontiguousarray(x, dtype=dt) for x in (uu, vv, ww)]
+        self._chk(self.lib.lbm_set_turbulence(self.h, *[_p(x) for x in a], int(nrturb), float(ampl),
+                                              a[0].itemsize), "lbm_set_turbulence")
+
+    def set_external_forcing(self, force):
+        if force is None:
+            self._chk(self.lib.lbm_set_external_forcing(self.h, None, 8), "lbm_set_external_forcing")
+            return
+        F = self._arr(force, self.shape_cell + (3,))
+        self._chk(self.lib.lbm_set_external_forcing(self.h, _p(F), F.itemsize), "lbm_set_external_forcing")
+
+    # -- time stepping -------------------------------------------------------------------
+    def step(self, it0, nsteps=1):
+        self._chk(self.lib.lbm_step(self.h, int(it0), int(nsteps)), "lbm_step")
+
+    def synchronize(self):
+        self._chk(self.lib.lbm_synchronize(self.h), "lbm_synchronize")
+
+    def set_timing(self, on=True):
+        self._chk(self.lib.lbm_set_timing(self.h, int(on)), "lbm_set_timing")
+
+    def timing_read(self):
+        s, b = C.c_double(), C.c_double()
+        ns, nb = C.c_int(), C.c_int()
+        self._chk(self.lib.lbm_timing_read(self.h, C.byref(s), C.byref(ns), C.byref(b), C.byref(nb)),
+                  "lbm_timing_read")
+        return dict(step_ms=s.value, n_step=ns.value, bc_ms=b.value, n_bc=nb.value)

@@ -96,6 +96,25 @@ class OracleSim:
                               _ptr(self.v), _ptr(self.w))
         self.tau[...] = self.dt(3.0) * self.kinevisc + self.dt(0.5)
 
+    def init_equilibrium(self, rho, u, v, w, it0=0, uini=0.0, udir=0.0, uvel_shear=None):
+        """nt0 = 0 path (src/main.F90:243-247): fequil3(fB), fillghosts(fB), fA = fB,
+        boundarycond(fB,fA), boundarycond(fA,fB); the first postcoll uses these moments."""
+        dt = self.dt
+        for dst, a in ((self.rho, rho), (self.u, u), (self.v, v), (self.w, w)):
+            dst[...] = np.asarray(a, dt)
+        self.fB[...] = 0
+        self.lib.or_fequil3(C.byref(self.p), _ptr(self.fB), _ptr(self.rho), _ptr(self.u), _ptr(self.v), _ptr(self.w))
+        self.lib.or_fillghosts(C.byref(self.p), _ptr(self.fB))
+        self.fA[...] = self.fB
+        nz = self.dims[2]
+        shear = np.ones(nz, dt) if uvel_shear is None else np.asarray(uvel_shear, dt)
+        uvel = np.ascontiguousarray(dt(uini) * shear, dt)
+        real = C.c_float if self.prec == "f32" else C.c_double
+        self.lib.or_boundarycond(C.byref(self.p), _ptr(self.fB), _ptr(self.fA), _ptr(uvel), real(udir))
+        self.lib.or_boundarycond(C.byref(self.p), _ptr(self.fA), _ptr(self.fB), _ptr(uvel), real(udir))
+        self.cur = 0
+        self.tau[...] = self.dt(3.0) * self.kinevisc + self.dt(0.5)
+
     def macros(self):
         return self.rho.copy(), self.u.copy(), self.v.copy(), self.w.copy()
 
