@@ -47,6 +47,9 @@ def build(verbose: bool = False) -> str:
             print(out)
     if failed:
         raise RuntimeError("hipcc failed for " + ", ".join(failed))
+    # calibration copy kernel (plain copy of a known size), a separate executable
+    subprocess.run([HIPCC] + FLAGS + ["-o", os.path.join(BUILD, "copy_bw"), os.path.join(CSRC, "copy_bw.hip")],
+                   check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs
     subprocess.run(cmd, check=True)

@@ -294,9 +294,10 @@ int lbm_create(const lbm_desc* desc, lbm_ctx** out) {
         case LBM_STORE_BF16S: ops = ops_bf16s(d.q); break;
         default: return fail(nullptr, LBM_EINVAL, "unknown storage kind");
     }
+    const Layout lay = make_layout(d.nx, d.ny, d.nz, ops->raw_bytes, d.ibnd == 0, d.jbnd == 0, d.kbnd == 0);
     int bx = d.block_x, by = d.block_y;
     if (bx == 0) {
-        const int rows = (d.nx + 63) / 64;
+        const int rows = (lay.n(lay.ax0) + 63) / 64;
         bx = rows * 64 <= 512 ? rows * 64 : 256;
     }
     if (by == 0) by = bx >= 256 ? 1 : 256 / bx;
@@ -312,7 +313,7 @@ int lbm_create(const lbm_desc* desc, lbm_ctx** out) {
     if (c->d.iturb_pos == 0) c->d.iturb_pos = 10;
     c->ops = ops;
     hipGetDevice(&c->device);
-    c->lay = make_layout(d.nx, d.ny, d.nz, ops->raw_bytes, d.ibnd == 0, d.jbnd == 0, d.kbnd == 0);
+    c->lay = lay;
     c->tau0 = round_real(c, round_real(c, 3.0 * round_real(c, d.kinevisc)) + 0.5);
     c->fac0 = round_real(c, 1.0 - round_real(c, 1.0 / c->tau0));
     auto bail = [&](hipError_t err, const char* where) {
@@ -531,16 +532,22 @@ int lbm_set_blanking(lbm_ctx* c, const uint8_t* mask) {
         return LBM_OK;
     }
     const lbm_desc& d = c->d;
-    const int wpr = (d.nx + 31) / 32;
-    std::vector<uint32_t> bits((size_t)wpr * d.ny * d.nz, 0u);
+    const Layout& L = c->lay;
+    // one bit per interior cell, rows of bits along the contiguous axis ax0
+    const int n0 = L.n(L.ax0), n1 = L.n(L.ax1), n2 = L.n(L.ax2);
+    const int wpr = (n0 + 31) / 32;
+    std::vector<uint32_t> bits((size_t)wpr * n1 * n2, 0u);
     bool any = false;
-    for (int k = 1; k <= d.nz; ++k)
-        for (int j = 1; j <= d.ny; ++j)
-            for (int i = 1; i <= d.nx; ++i)
+    for (int cc = 1; cc <= n2; ++cc)
+        for (int cb = 1; cb <= n1; ++cb)
+            for (int ca = 1; ca <= n0; ++ca) {
+                int i, j, k;
+                L.ijk(ca, cb, cc, i, j, k);
                 if (mask[((size_t)k * (d.ny + 2) + j) * (d.nx + 2) + i]) {
-                    bits[((size_t)(k - 1) * d.ny + (j - 1)) * wpr + (i - 1) / 32] |= 1u << ((i - 1) % 32);
+                    bits[((size_t)(cc - 1) * n1 + (cb - 1)) * wpr + (ca - 1) / 32] |= 1u << ((ca - 1) % 32);
                     any = true;
                 }
+            }
     if (int rc = dev_alloc(c, c->solid, bits.size() * 4)) return rc;
     HIPCHK(c, hipMemcpyAsync(c->solid.p, bits.data(), bits.size() * 4, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -36,7 +36,8 @@ struct StepArgs {
     const R* ext_u;        //   an equilibrium initialisation, src/main.F90:214,243,299);
     const R* ext_v;        //   cell layout, nullptr otherwise
     const R* ext_w;
-    const uint32_t* solid; // one bit per interior cell, rows of solid_wpr words; nullptr: none
+    const uint32_t* solid; // one bit per interior cell in rows along the contiguous axis,
+                           //   solid_wpr words per row; nullptr: no solids
     int solid_wpr;
     int ip;                // inflow-turbulence slice (reference: 10)
     const R* tu;           // turbulence slice (ny*nz, j fastest) for this step; nullptr: off
@@ -79,10 +80,13 @@ __global__ void __launch_bounds__(1024) step_kernel(const StepArgs<typename C::r
     using raw = typename C::raw;
     using Lt = Lattice<Q>;
     const Layout& L = a.lay;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
-    if (i > L.nx || j > L.ny || k > L.nz) return;
+    // Threads run along the storage axes: x along the contiguous axis ax0, y along ax1, z along ax2.
+    const int ca = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int cb = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int cc = blockIdx.z + 1;
+    if (ca > L.n(L.ax0) || cb > L.n(L.ax1) || cc > L.n(L.ax2)) return;
+    int i, j, k;
+    L.ijk(ca, cb, cc, i, j, k);
 
     // Source offsets: population l comes from (i - cx, j - cy, k - cz), periodic directions
     // folded onto the interior.  Index d = c + 1.
@@ -91,9 +95,9 @@ __global__ void __launch_bounds__(1024) step_kernel(const StepArgs<typename C::r
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         const int c = d - 1;
-        ox[d] = L.xoff + Layout::wrap(i - s * c, L.nx, L.per_i);
-        oy[d] = Layout::wrap(j - s * c, L.ny, L.per_j) * L.pitch;
-        oz[d] = Layout::wrap(k - s * c, L.nz, L.per_k) * L.ystride;
+        ox[d] = L.base + Layout::wrap(i - s * c, L.nx, L.per_i) * L.si;
+        oy[d] = Layout::wrap(j - s * c, L.ny, L.per_j) * L.sj;
+        oz[d] = Layout::wrap(k - s * c, L.nz, L.per_k) * L.sk;
     }
     const raw* __restrict__ src = static_cast<const raw*>(a.src);
     raw rv[Q];
@@ -219,8 +223,9 @@ __global__ void __launch_bounds__(1024) step_kernel(const StepArgs<typename C::r
 
     // Solid cells: swap the opposite pairs (2,3),(4,5),... (src/m_solids.F90:19-32).
     if (a.solid != nullptr) {
-        const uint32_t word = a.solid[((k - 1) * L.ny + (j - 1)) * a.solid_wpr + ((i - 1) >> 5)];
-        if ((word >> ((i - 1) & 31)) & 1u) {
+        // rows of bits along the contiguous axis
+        const uint32_t word = a.solid[((cc - 1) * L.n(L.ax1) + (cb - 1)) * a.solid_wpr + ((ca - 1) >> 5)];
+        if ((word >> ((ca - 1) & 31)) & 1u) {
 #pragma unroll
             for (int l = 1; l + 1 < Q; l += 2) {
                 const R t = out[l];
@@ -267,35 +272,54 @@ __host__ __device__ constexpr int mirror(int l, int d, int opt) {
                     d == 2 ? -Lt::cz(l) : opt * Lt::cz(l));
 }
 
-// Closed faces normal to axis DIR, both sides in one launch (blockIdx.z = side):
-//   src/m_boundary_i_closed_kernel.F90:44-58, _j_:103-114, _k_:157-168.
+// Closed faces normal to axis DIR (src/m_boundary_i_closed_kernel.F90:44-58, _j_:103-114,
+// _k_:157-168).  One thread per tangential position handles both faces: it first loads every
+// source (the other buffer's ghost cell and the wall-plane cells), then writes the ghosts.  The
+// sources are never written by this kernel (plane n >= 2), so loading first keeps the
+// reference's result and lets the loads overlap.  Threads run along the tangential axis TA
+// (the one stored first), then TB.
 template <int Q, class C, int DIR>
-__global__ void bc_closed_kernel(Layout L, void* f1, const void* f2, int opt_lo, int opt_hi) {
+__global__ void bc_closed_kernel(Layout L, void* f1, const void* f2, int opt_lo, int opt_hi, int TA, int TB) {
     using Lt = Lattice<Q>;
-    const int ta = blockIdx.x * blockDim.x + threadIdx.x + 1;   // first tangential index
-    const int tb = blockIdx.y * blockDim.y + threadIdx.y + 1;   // second tangential index
-    const int side = blockIdx.z;
-    const int n = DIR == 0 ? L.nx : DIR == 1 ? L.ny : L.nz;
-    const int na = DIR == 0 ? L.ny : L.nx;
-    const int nb = DIR == 2 ? L.ny : L.nz;
-    if (ta > na || tb > nb) return;
-    const int plane = side ? n : 1, ghost = side ? n + 1 : 0, opt = side ? opt_hi : opt_lo;
-    int g[3];
-    if (DIR == 0) { g[0] = ghost; g[1] = ta; g[2] = tb; }
-    else if (DIR == 1) { g[0] = ta; g[1] = ghost; g[2] = tb; }
-    else { g[0] = ta; g[1] = tb; g[2] = ghost; }
-    const int og = L.cell(g[0], g[1], g[2]);
+    using raw = typename C::raw;
+    const int ta = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int tb = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (ta > L.n(TA) || tb > L.n(TB)) return;
+    const int n = L.n(DIR);
+    raw* __restrict__ p1 = static_cast<raw*>(f1);
+    const raw* __restrict__ p2 = static_cast<const raw*>(f2);
+    int og[2];
+    raw v2[Q], v1[Q];
 #pragma unroll
-    for (int l = 0; l < Q; ++l) {
-        const int cn = Lt::c(l, DIR);
-        if (cn == 0) continue;
-        if ((side == 0 && cn > 0) || (side == 1 && cn < 0)) continue;
-        const int m = opt < 0 ? mirror<Q>(l, DIR, -1) : mirror<Q>(l, DIR, +1);
-        cp<Q, C>(L, f1, m, og, f2, l, og);
-        const int si = DIR == 0 ? plane : g[0] - Lt::cx(l);
-        const int sj = DIR == 1 ? plane : g[1] - Lt::cy(l);
-        const int sk = DIR == 2 ? plane : g[2] - Lt::cz(l);
-        cp<Q, C>(L, f1, l, og, f1, l, L.cell_w(si, sj, sk));
+    for (int side = 0; side < 2; ++side) {
+        int g[3];
+        g[DIR] = side ? n + 1 : 0;
+        g[TA] = ta;
+        g[TB] = tb;
+        og[side] = L.cell(g[0], g[1], g[2]);
+        const int plane = side ? n : 1;
+#pragma unroll
+        for (int l = 0; l < Q; ++l) {
+            const int cn = Lt::c(l, DIR);
+            if (!(side == 0 ? cn < 0 : cn > 0)) continue;
+            v2[l] = p2[(size_t)l * L.plane + og[side]];
+            const int si = DIR == 0 ? plane : g[0] - Lt::cx(l);
+            const int sj = DIR == 1 ? plane : g[1] - Lt::cy(l);
+            const int sk = DIR == 2 ? plane : g[2] - Lt::cz(l);
+            v1[l] = p1[(size_t)l * L.plane + L.cell_w(si, sj, sk)];
+        }
+    }
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+        const int opt = side ? opt_hi : opt_lo;
+#pragma unroll
+        for (int l = 0; l < Q; ++l) {
+            const int cn = Lt::c(l, DIR);
+            if (!(side == 0 ? cn < 0 : cn > 0)) continue;
+            const int m = opt < 0 ? mirror<Q>(l, DIR, -1) : mirror<Q>(l, DIR, +1);
+            p1[(size_t)m * L.plane + og[side]] = v2[l];
+            p1[(size_t)l * L.plane + og[side]] = v1[l];
+        }
     }
 }
 
@@ -314,6 +338,11 @@ __global__ void bc_edges_kernel(Layout L, void* f1, const void* f2, const EdgeLi
     int g[3] = {e.g[s][0], e.g[s][1], e.g[s][2]};
     if (e.kind[s] == 0) g[e.t[s]] = a;
     const int og = L.cell(g[0], g[1], g[2]);
+    using raw = typename C::raw;
+    raw* __restrict__ p1 = static_cast<raw*>(f1);
+    const raw* __restrict__ p2 = static_cast<const raw*>(f2);
+    raw v2[Q], v1[Q];
+    bool outg[Q];
 #pragma unroll
     for (int l = 0; l < Q; ++l) {
         const int c[3] = {Lt::cx(l), Lt::cy(l), Lt::cz(l)};
@@ -321,42 +350,59 @@ __global__ void bc_edges_kernel(Layout L, void* f1, const void* f2, const EdgeLi
 #pragma unroll
         for (int d = 0; d < 3; ++d)
             if (e.out[s][d] != 0 && c[d] * e.out[s][d] <= 0) outgoing = false;
+        outg[l] = outgoing;
         if (!outgoing) continue;
-        const int m = Lt::find(e.mir[s][0] * c[0], e.mir[s][1] * c[1], e.mir[s][2] * c[2]);
-        if (m >= 0) cp<Q, C>(L, f1, m, og, f2, l, og);
+        v2[l] = p2[(size_t)l * L.plane + og];
         int src[3] = {g[0], g[1], g[2]};
 #pragma unroll
         for (int d = 0; d < 3; ++d)
             if (e.out[s][d] != 0) src[d] = g[d] - c[d];
-        cp<Q, C>(L, f1, l, og, f1, l, L.cell(src[0], src[1], src[2]));
+        v1[l] = p1[(size_t)l * L.plane + L.cell(src[0], src[1], src[2])];
+    }
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+        if (!outg[l]) continue;
+        const int m = Lt::find(e.mir[s][0] * Lt::cx(l), e.mir[s][1] * Lt::cy(l), e.mir[s][2] * Lt::cz(l));
+        if (m >= 0) p1[(size_t)m * L.plane + og] = v2[l];
+        p1[(size_t)l * L.plane + og] = v1[l];
     }
 }
 
 // Inflow at i = 0 and zero-gradient outflow at i = nx+1 (src/m_boundary_i_inflow_kernel.F90:61-111).
+// Threads run along j (jfast = 1) or k, whichever is stored first; all loads precede the stores.
 template <int Q, class C>
 __global__ void bc_inflow_kernel(Layout L, void* f1, const typename C::real* shear, typename C::real utime,
-                                 typename C::real cosd, typename C::real sind, typename C::real rho0) {
+                                 typename C::real cosd, typename C::real sind, typename C::real rho0, int jfast) {
     using R = typename C::real;
+    using raw = typename C::raw;
     using Lt = Lattice<Q>;
-    const int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int k = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int j = jfast ? x : y;
+    const int k = jfast ? y : x;
     if (j > L.ny || k > L.nz) return;
+    raw* __restrict__ p = static_cast<raw*>(f1);
     const R uu = utime * shear[k - 1];        // uvel(k) = uvel_time(it)*uvel_shear(k), taper = 1
     const R cs2 = R(1) / R(3);
     const int o1 = L.cell(1, j, k), o0 = L.cell(0, j, k), on = L.cell(L.nx, j, k), oe = L.cell(L.nx + 1, j, k);
     R fg[Q];
+    raw fo[Q];
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+        fg[l] = C::template decode<Q>(p[(size_t)l * L.plane + o1], l);
+        fo[l] = p[(size_t)l * L.plane + on];
+    }
 #pragma unroll
     for (int l = 0; l < Q; ++l) {
         const R wl = weight<Q, R>(l);
         const R cxl = R(Lt::cx(l)), cyl = R(Lt::cy(l));
-        fg[l] = ld<Q, C>(L, f1, l, o1) - R(2) * wl * rho0 * (cxl * uu * cosd + cyl * uu * sind) / cs2;
+        fg[l] = fg[l] - R(2) * wl * rho0 * (cxl * uu * cosd + cyl * uu * sind) / cs2;
     }
 #pragma unroll
-    for (int l = 0; l < Q; ++l)
-        st<Q, C>(L, f1, l, o0, Lt::cx(l) <= 0 ? fg[l] : fg[Lt::opposite(l)]);
-#pragma unroll
-    for (int l = 0; l < Q; ++l)
-        cp<Q, C>(L, f1, l, oe, f1, l, on);
+    for (int l = 0; l < Q; ++l) {
+        p[(size_t)l * L.plane + o0] = C::template encode<Q>(Lt::cx(l) <= 0 ? fg[l] : fg[Lt::opposite(l)], l);
+        p[(size_t)l * L.plane + oe] = fo[l];
+    }
 }
 
 // ------------------------------------------------------------------------------------------

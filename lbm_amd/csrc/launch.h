@@ -62,7 +62,7 @@ struct Launch {
         a.Fz = static_cast<const R*>(h.F[2]);
         a.tau_out = static_cast<R*>(h.tau_out);
         const dim3 block(h.block_x, h.block_y, 1);
-        const dim3 grid((L.nx + h.block_x - 1) / h.block_x, (L.ny + h.block_y - 1) / h.block_y, L.nz);
+        const dim3 grid((L.n(L.ax0) + h.block_x - 1) / h.block_x, (L.n(L.ax1) + h.block_y - 1) / h.block_y, L.n(L.ax2));
         const int key = (h.bgk3 ? 4 : 0) | (h.hrr ? 2 : 0) | (h.vre ? 1 : 0);
         switch (key) {
             case 0: return step_t<false, false, false>(L, a, grid, block, s);
@@ -77,13 +77,16 @@ struct Launch {
     }
 
     static hipError_t closed(const Layout& L, void* f1, const void* f2, int dir, int opt_lo, int opt_hi, hipStream_t s) {
-        const int na = dir == 0 ? L.ny : L.nx;
-        const int nb = dir == 2 ? L.ny : L.nz;
+        // tangential axes in storage order: threads run along the first
+        const int order[3] = {L.ax0, L.ax1, L.ax2};
+        int t[2], q = 0;
+        for (int a = 0; a < 3; ++a)
+            if (order[a] != dir) t[q++] = order[a];
         const dim3 block(64, 4, 1);
-        const dim3 grid((na + 63) / 64, (nb + 3) / 4, 2);
-        if (dir == 0) hipLaunchKernelGGL((bc_closed_kernel<Q, C, 0>), grid, block, 0, s, L, f1, f2, opt_lo, opt_hi);
-        else if (dir == 1) hipLaunchKernelGGL((bc_closed_kernel<Q, C, 1>), grid, block, 0, s, L, f1, f2, opt_lo, opt_hi);
-        else hipLaunchKernelGGL((bc_closed_kernel<Q, C, 2>), grid, block, 0, s, L, f1, f2, opt_lo, opt_hi);
+        const dim3 grid((L.n(t[0]) + 63) / 64, (L.n(t[1]) + 3) / 4, 1);
+        if (dir == 0) hipLaunchKernelGGL((bc_closed_kernel<Q, C, 0>), grid, block, 0, s, L, f1, f2, opt_lo, opt_hi, t[0], t[1]);
+        else if (dir == 1) hipLaunchKernelGGL((bc_closed_kernel<Q, C, 1>), grid, block, 0, s, L, f1, f2, opt_lo, opt_hi, t[0], t[1]);
+        else hipLaunchKernelGGL((bc_closed_kernel<Q, C, 2>), grid, block, 0, s, L, f1, f2, opt_lo, opt_hi, t[0], t[1]);
         return hipGetLastError();
     }
 
@@ -99,10 +102,13 @@ struct Launch {
 
     static hipError_t inflow(const Layout& L, void* f1, const void* shear, double utime, double cosd, double sind,
                              double rho0, hipStream_t s) {
+        // threads along j or k, whichever is stored first
+        const int jfast = (L.ax0 == 1) || (L.ax0 == 0 && L.ax1 == 1);
+        const int nx_ = jfast ? L.ny : L.nz, ny_ = jfast ? L.nz : L.ny;
         const dim3 block(64, 4, 1);
-        const dim3 grid((L.ny + 63) / 64, (L.nz + 3) / 4, 1);
+        const dim3 grid((nx_ + 63) / 64, (ny_ + 3) / 4, 1);
         hipLaunchKernelGGL((bc_inflow_kernel<Q, C>), grid, block, 0, s, L, f1, static_cast<const R*>(shear), (R)utime,
-                           (R)cosd, (R)sind, (R)rho0);
+                           (R)cosd, (R)sind, (R)rho0, jfast);
         return hipGetLastError();
     }
 

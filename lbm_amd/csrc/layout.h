@@ -1,10 +1,15 @@
 // Device layout of one population buffer (structure of arrays).
 //
-// A buffer holds Q planes; plane l is the padded grid (0:nx+1, 0:ny+1, 0:nz+1) of population l
-// with i contiguous.  Rows are padded so that interior cell i = 1 starts on a 128-byte
-// boundary: storage column of i is xoff + i with xoff = 128/sizeof(raw) - 1, and the row pitch is
-// a multiple of 128 bytes.  A 64-lane wave along i therefore reads and writes whole cache lines
-// for the c_x = 0 populations and straddles one extra line for c_x = +-1.
+// A buffer holds Q planes; plane l is the padded grid (0:nx+1, 0:ny+1, 0:nz+1) of population l.
+// One axis is stored contiguously (the "fast" axis); rows along it are padded so that interior
+// index 1 starts on a 128-byte boundary (storage column of index x is xoff + x with
+// xoff = 128/sizeof(raw) - 1) and the row pitch is a multiple of 128 bytes.  The two other axes
+// follow in natural order.  A 64-lane wave along the fast axis reads and writes whole cache
+// lines for populations with no velocity along that axis.
+//
+// The fast axis is chosen per domain (make_layout): a periodic axis if there is one (i first),
+// so that the ghost planes the boundary kernels touch (closed faces, the inflow/outflow planes
+// of i) are rows along the fast axis and their accesses coalesce.
 //
 // Ghost layers in a periodic direction are not maintained: every reader wraps that index onto
 // the interior instead (ghost 0 -> n, n+1 -> 1), which is what the reference's periodic passes
@@ -20,13 +25,15 @@ namespace lbm {
 
 struct Layout {
     int nx, ny, nz;
-    int xoff;          // storage column of ghost i = 0 is xoff, of cell i is xoff + i
-    int pitch;         // elements per (j,k) row
-    int ystride;       // elements per k slab = pitch*(ny+2)
-    long long plane;   // elements per population plane (ystride*(nz+2), rounded to 256 B)
     int per_i, per_j, per_k;
+    int ax0, ax1, ax2;      // storage order: ax0 contiguous, ax2 slowest (0 = i, 1 = j, 2 = k)
+    int si, sj, sk;         // element strides of i, j, k
+    int base;               // storage offset of cell (0,0,0)
+    int pitch;              // elements per row along ax0
+    long long plane;        // elements per population plane (rounded to 256 B)
 
-    __host__ __device__ inline int cell(int i, int j, int k) const { return k * ystride + j * pitch + xoff + i; }
+    __host__ __device__ inline int n(int a) const { return a == 0 ? nx : a == 1 ? ny : nz; }
+    __host__ __device__ inline int cell(int i, int j, int k) const { return base + i * si + j * sj + k * sk; }
     __host__ __device__ static inline int wrap(int x, int n, int per) {
         return per ? (x == 0 ? n : (x == n + 1 ? 1 : x)) : x;
     }
@@ -34,19 +41,35 @@ struct Layout {
     __host__ __device__ inline int cell_w(int i, int j, int k) const {
         return cell(wrap(i, nx, per_i), wrap(j, ny, per_j), wrap(k, nz, per_k));
     }
+    // Map coordinates given in storage-axis order (a along ax0, b along ax1, c along ax2) to i,j,k.
+    __host__ __device__ inline void ijk(int a, int b, int c, int& i, int& j, int& k) const {
+        i = ax0 == 0 ? a : (ax1 == 0 ? b : c);
+        j = ax0 == 1 ? a : (ax1 == 1 ? b : c);
+        k = ax0 == 2 ? a : (ax1 == 2 ? b : c);
+    }
 };
 
-inline Layout make_layout(int nx, int ny, int nz, int raw_bytes, int per_i, int per_j, int per_k) {
+// fast < 0: choose (first periodic axis, else i).
+inline Layout make_layout(int nx, int ny, int nz, int raw_bytes, int per_i, int per_j, int per_k, int fast = -1) {
     Layout L;
     L.nx = nx; L.ny = ny; L.nz = nz;
-    int a = 128 / raw_bytes;                 // elements per 128 B
-    L.xoff = a - 1;
-    L.pitch = ((a + nx + 1) + a - 1) / a * a;
-    L.ystride = L.pitch * (ny + 2);
-    long long pl = (long long)L.ystride * (nz + 2);
+    L.per_i = per_i; L.per_j = per_j; L.per_k = per_k;
+    if (fast < 0) fast = per_i ? 0 : per_j ? 1 : per_k ? 2 : 0;
+    L.ax0 = fast;
+    L.ax1 = fast == 0 ? 1 : 0;
+    L.ax2 = fast == 2 ? 1 : 2;
+    const int a = 128 / raw_bytes;           // elements per 128 B
+    const int n0 = L.n(L.ax0), n1 = L.n(L.ax1), n2 = L.n(L.ax2);
+    L.pitch = ((a + n0 + 1) + a - 1) / a * a;
+    int st[3];
+    st[L.ax0] = 1;
+    st[L.ax1] = L.pitch;
+    st[L.ax2] = L.pitch * (n1 + 2);
+    L.si = st[0]; L.sj = st[1]; L.sk = st[2];
+    L.base = a - 1;
+    long long pl = (long long)L.pitch * (n1 + 2) * (n2 + 2);
     long long ea = 256 / raw_bytes;
     L.plane = (pl + ea - 1) / ea * ea;
-    L.per_i = per_i; L.per_j = per_j; L.per_k = per_k;
     return L;
 }
 
