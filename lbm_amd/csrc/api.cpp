@@ -41,7 +41,8 @@ struct lbm_ctx {
     bool ghost_raw[2] = {true, true};
     bool use_ext = false;          // first collision uses the moments given at initialisation
     // inputs
-    DevBuf ablvisc, shear, solid, turb[3], force[3], tau, ext[4], stage, partial;
+    DevBuf ablvisc, shear, solid, turb[3], force[3], tau, ext[4], stage, partial, snap;
+    SnapList snap_k{}, snap_j{};
     int solid_wpr = 0;
     bool has_solid = false, has_turb = false, has_force = false;
     int nrturb = 0;
@@ -206,6 +207,31 @@ EdgeList edge_list(const lbm_desc& d) {
     return e;
 }
 
+// Ghost cells read by a closed-face pass while holding the first periodic pass's copy of a cell
+// that an earlier closed pass of the same call has rewritten (pass order i, j, k;
+// src/m_boundarycond.F90:69-255).  For the k faces: tangential (i, j) ghosts where one of i, j is
+// periodic and the other closed; for the j faces: (i, k) ghosts with k periodic and i closed.
+void snap_lists(const lbm_desc& d, SnapList& sk, SnapList& sj) {
+    sk = SnapList{};
+    sj = SnapList{};
+    const bool ci = d.ibnd > 10, cj = d.jbnd > 10, ck = d.kbnd > 10;
+    const bool pi = d.ibnd == 0, pj = d.jbnd == 0, pk = d.kbnd == 0;
+    if (ck && ((pi && cj) || (ci && pj)))
+        for (int a : {0, d.nx + 1})
+            for (int b : {0, d.ny + 1})
+                for (int c : {1, d.nz}) {
+                    sk.g[sk.n][0] = a; sk.g[sk.n][1] = b; sk.g[sk.n][2] = c;
+                    ++sk.n;
+                }
+    if (cj && pk && ci)
+        for (int a : {0, d.nx + 1})
+            for (int b : {1, d.ny})
+                for (int c : {0, d.nz + 1}) {
+                    sj.g[sj.n][0] = a; sj.g[sj.n][1] = b; sj.g[sj.n][2] = c;
+                    ++sj.n;
+                }
+}
+
 double utime_at(const lbm_ctx* c, int it) {
     if (!c->ut_series.empty() && it >= c->series_lo && it <= c->series_hi) return c->ut_series[it - c->series_lo];
     return c->uini;
@@ -233,13 +259,18 @@ int boundary(lbm_ctx* c, void* f1, const void* f2, int it) {
         e0 = next_event(c, c->ev_bc, c->n_bc_ev);
         if (e0) hipEventRecord(e0, c->stream);
     }
+    const SnapList none{};
+    char* snap = static_cast<char*>(c->snap.p);
+    const size_t half = (size_t)8 * d.q * c->ops->raw_bytes;
+    if (c->snap_k.n) HIPCHK(c, c->ops->snapshot(L, f1, snap, c->snap_k, c->stream));
+    if (c->snap_j.n) HIPCHK(c, c->ops->snapshot(L, f1, snap + half, c->snap_j, c->stream));
     int o1, oN;
     opts_of(d.ibnd, &o1, &oN);
-    if (o1) HIPCHK(c, c->ops->closed(L, f1, f2, 0, o1, oN, c->stream));
+    if (o1) HIPCHK(c, c->ops->closed(L, f1, f2, 0, o1, oN, nullptr, none, c->stream));
     opts_of(d.jbnd, &o1, &oN);
-    if (o1) HIPCHK(c, c->ops->closed(L, f1, f2, 1, o1, oN, c->stream));
+    if (o1) HIPCHK(c, c->ops->closed(L, f1, f2, 1, o1, oN, snap + half, c->snap_j, c->stream));
     opts_of(d.kbnd, &o1, &oN);
-    if (o1) HIPCHK(c, c->ops->closed(L, f1, f2, 2, o1, oN, c->stream));
+    if (o1) HIPCHK(c, c->ops->closed(L, f1, f2, 2, o1, oN, snap, c->snap_k, c->stream));
     const EdgeList e = edge_list(d);
     if (e.n) HIPCHK(c, c->ops->edges(L, f1, f2, e, c->stream));
     if (d.ibnd == 1) {
@@ -327,6 +358,12 @@ int lbm_create(const lbm_desc* desc, lbm_ctx** out) {
         if ((e = hipMalloc(&c->buf[b], c->buf_bytes)) != hipSuccess) return bail(e, "hipMalloc(populations)");
     for (int b = 0; b < 2; ++b)
         if ((e = hipMemsetAsync(c->buf[b], 0, c->buf_bytes, c->stream)) != hipSuccess) return bail(e, "hipMemset");
+    snap_lists(d, c->snap_k, c->snap_j);
+    if (c->snap_k.n || c->snap_j.n) {
+        const size_t sb = (size_t)16 * d.q * ops->raw_bytes;
+        if ((e = hipMalloc(&c->snap.p, sb)) != hipSuccess) return bail(e, "hipMalloc(snapshot)");
+        c->snap.n = sb;
+    }
     if (d.store_tau && d.ivreman) {
         const size_t tb = (size_t)c->lay.plane * ops->real_bytes;
         if ((e = hipMalloc(&c->tau.p, tb)) != hipSuccess) return bail(e, "hipMalloc(tau)");
@@ -352,7 +389,7 @@ int lbm_destroy(lbm_ctx* c) {
         if (c->buf[b]) hipFree(c->buf[b]);
     DevBuf* all[] = {&c->ablvisc, &c->shear, &c->solid, &c->turb[0], &c->turb[1], &c->turb[2], &c->force[0],
                      &c->force[1], &c->force[2], &c->tau, &c->ext[0], &c->ext[1], &c->ext[2], &c->ext[3],
-                     &c->stage, &c->partial};
+                     &c->stage, &c->partial, &c->snap};
     for (DevBuf* b : all) dev_free(*b);
     for (hipEvent_t ev : c->ev_step) hipEventDestroy(ev);
     for (hipEvent_t ev : c->ev_bc) hipEventDestroy(ev);

@@ -1,14 +1,23 @@
 // Plain device-to-device copy of a known size: the calibration kernel for HBM byte counters
 // (FETCH_SIZE / WRITE_SIZE) and the achievable-bandwidth reference next to the lattice kernels.
-//   copy_bw [MiB] [iterations]   prints one JSON line
+//   copy_bw [MiB] [iterations] [blocks]   prints one JSON line
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
 #include <cstdlib>
 
+// Each thread moves four 16-byte vectors per iteration, loads issued before stores.
 __global__ void __launch_bounds__(256) copy_kernel(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {
-    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
-        dst[t] = src[t];
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += 4 * stride) {
+        float4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (t + u * stride < n) v[u] = src[t + u * stride];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (t + u * stride < n) dst[t + u * stride] = v[u];
+    }
 }
 
 #define CHK(x)                                                                      \
@@ -32,7 +41,7 @@ int main(int argc, char** argv) {
     CHK(hipMemset(b, 0, bytes));
     hipDeviceProp_t prop;
     CHK(hipGetDeviceProperties(&prop, 0));
-    const int blocks = prop.multiProcessorCount * 8;
+    const int blocks = argc > 3 ? atoi(argv[3]) : prop.multiProcessorCount * 16;
     hipEvent_t e0, e1;
     CHK(hipEventCreate(&e0));
     CHK(hipEventCreate(&e1));

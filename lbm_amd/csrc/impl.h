@@ -20,6 +20,15 @@ struct EdgeList {
     int mir[20][3];
 };
 
+// Ghost cells that a closed-face pass reads while they hold a copy made by the reference's first
+// periodic pass (src/m_boundarycond.F90:69-159) of a cell that an earlier closed pass of the same
+// call has since rewritten.  g holds the ghost coordinates as read; the snapshot of the copied
+// cell is taken before the closed passes.
+struct SnapList {
+    int n;
+    int g[8][3];
+};
+
 // Everything the host passes to one step-kernel launch, in double; converted to the compute
 // type by the wrapper.
 struct StepHost {
@@ -42,7 +51,9 @@ struct StepHost {
 
 struct Ops {
     hipError_t (*step)(const Layout&, const StepHost&, hipStream_t);
-    hipError_t (*closed)(const Layout&, void* f1, const void* f2, int dir, int opt_lo, int opt_hi, hipStream_t);
+    hipError_t (*closed)(const Layout&, void* f1, const void* f2, int dir, int opt_lo, int opt_hi, const void* snap,
+                         const SnapList&, hipStream_t);
+    hipError_t (*snapshot)(const Layout&, const void* f1, void* snap, const SnapList&, hipStream_t);
     hipError_t (*edges)(const Layout&, void* f1, const void* f2, const EdgeList&, hipStream_t);
     hipError_t (*inflow)(const Layout&, void* f1, const void* shear, double utime, double cosd, double sind,
                          double rho0, hipStream_t);

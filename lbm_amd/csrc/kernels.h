@@ -279,7 +279,8 @@ __host__ __device__ constexpr int mirror(int l, int d, int opt) {
 // reference's result and lets the loads overlap.  Threads run along the tangential axis TA
 // (the one stored first), then TB.
 template <int Q, class C, int DIR>
-__global__ void bc_closed_kernel(Layout L, void* f1, const void* f2, int opt_lo, int opt_hi, int TA, int TB) {
+__global__ void bc_closed_kernel(Layout L, void* f1, const void* f2, int opt_lo, int opt_hi, int TA, int TB,
+                                 const void* snapv, const SnapList sl) {
     using Lt = Lattice<Q>;
     using raw = typename C::raw;
     const int ta = blockIdx.x * blockDim.x + threadIdx.x + 1;
@@ -307,6 +308,11 @@ __global__ void bc_closed_kernel(Layout L, void* f1, const void* f2, int opt_lo,
             const int sj = DIR == 1 ? plane : g[1] - Lt::cy(l);
             const int sk = DIR == 2 ? plane : g[2] - Lt::cz(l);
             v1[l] = p1[(size_t)l * L.plane + L.cell_w(si, sj, sk)];
+            if (sl.n > 0) {
+                for (int e = 0; e < sl.n; ++e)
+                    if (sl.g[e][0] == si && sl.g[e][1] == sj && sl.g[e][2] == sk)
+                        v1[l] = static_cast<const raw*>(snapv)[e * Q + l];
+            }
         }
     }
 #pragma unroll
@@ -321,6 +327,17 @@ __global__ void bc_closed_kernel(Layout L, void* f1, const void* f2, int opt_lo,
             p1[(size_t)l * L.plane + og[side]] = v1[l];
         }
     }
+}
+
+// Copy of the cells listed in sl (periodic axes wrapped) taken before the closed passes.
+template <int Q, class C>
+__global__ void bc_snapshot_kernel(Layout L, const void* f1, void* snapv, const SnapList sl) {
+    using raw = typename C::raw;
+    const int t = threadIdx.x;
+    if (t >= sl.n * Q) return;
+    const int e = t / Q, l = t % Q;
+    static_cast<raw*>(snapv)[t] =
+        static_cast<const raw*>(f1)[(size_t)l * L.plane + L.cell_w(sl.g[e][0], sl.g[e][1], sl.g[e][2])];
 }
 
 // Ghost edges and corners: both passes of src/m_boundary_{j_k,i_j,i_k}_edges.F90 and

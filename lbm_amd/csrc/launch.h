@@ -76,7 +76,14 @@ struct Launch {
         }
     }
 
-    static hipError_t closed(const Layout& L, void* f1, const void* f2, int dir, int opt_lo, int opt_hi, hipStream_t s) {
+    static hipError_t snapshot(const Layout& L, const void* f1, void* snap, const SnapList& sl, hipStream_t s) {
+        if (sl.n == 0) return hipSuccess;
+        hipLaunchKernelGGL((bc_snapshot_kernel<Q, C>), dim3(1), dim3(256), 0, s, L, f1, snap, sl);
+        return hipGetLastError();
+    }
+
+    static hipError_t closed(const Layout& L, void* f1, const void* f2, int dir, int opt_lo, int opt_hi,
+                             const void* snap, const SnapList& sl, hipStream_t s) {
         // tangential axes in storage order: threads run along the first
         const int order[3] = {L.ax0, L.ax1, L.ax2};
         int t[2], q = 0;
@@ -84,9 +91,9 @@ struct Launch {
             if (order[a] != dir) t[q++] = order[a];
         const dim3 block(64, 4, 1);
         const dim3 grid((L.n(t[0]) + 63) / 64, (L.n(t[1]) + 3) / 4, 1);
-        if (dir == 0) hipLaunchKernelGGL((bc_closed_kernel<Q, C, 0>), grid, block, 0, s, L, f1, f2, opt_lo, opt_hi, t[0], t[1]);
-        else if (dir == 1) hipLaunchKernelGGL((bc_closed_kernel<Q, C, 1>), grid, block, 0, s, L, f1, f2, opt_lo, opt_hi, t[0], t[1]);
-        else hipLaunchKernelGGL((bc_closed_kernel<Q, C, 2>), grid, block, 0, s, L, f1, f2, opt_lo, opt_hi, t[0], t[1]);
+        if (dir == 0) hipLaunchKernelGGL((bc_closed_kernel<Q, C, 0>), grid, block, 0, s, L, f1, f2, opt_lo, opt_hi, t[0], t[1], snap, sl);
+        else if (dir == 1) hipLaunchKernelGGL((bc_closed_kernel<Q, C, 1>), grid, block, 0, s, L, f1, f2, opt_lo, opt_hi, t[0], t[1], snap, sl);
+        else hipLaunchKernelGGL((bc_closed_kernel<Q, C, 2>), grid, block, 0, s, L, f1, f2, opt_lo, opt_hi, t[0], t[1], snap, sl);
         return hipGetLastError();
     }
 
@@ -164,7 +171,7 @@ struct Launch {
     }
 
     static const Ops* ops() {
-        static const Ops o = {&step, &closed, &edges, &inflow, &aos_in, &state_out, &feq_init, &totals,
+        static const Ops o = {&step, &closed, &snapshot, &edges, &inflow, &aos_in, &state_out, &feq_init, &totals,
                               (int)sizeof(R), (int)sizeof(typename C::raw)};
         return &o;
     }
