@@ -84,7 +84,7 @@ def cpu_baseline(w, f0, turb, sample_steps, threads):
             return None
     nx, ny, nz = w["dims"]
     nr = sample_steps + 5
-    tf = tuple(a[: nr + 1] for a in turb) + (np.zeros_like(turb[0][: nr + 1]),)
+    tf = tuple(a[: nr + 1] for a in turb)
     wd = tempfile.mkdtemp(prefix="lbm_cpu_")
     try:
         _, out = oracle.run_reference(exe, f0, nt0=1, nt1=1 + sample_steps, turb_fields=tf, workdir=wd,

@@ -5,6 +5,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -51,6 +52,7 @@ struct lbm_ctx {
     int series_lo = 0, series_hi = -1;
     std::vector<double> ut_series, ud_series;
     double tau0 = 0.0, fac0 = 0.0;
+    int variant = 0;               // step-kernel variant (LBM_STEP_VARIANT), see impl.h
     // timing
     bool timing = false;
     std::vector<hipEvent_t> ev_step, ev_bc;   // pairs
@@ -343,6 +345,7 @@ int lbm_create(const lbm_desc* desc, lbm_ctx** out) {
     c->d.block_y = by;
     if (c->d.iturb_pos == 0) c->d.iturb_pos = 10;
     c->ops = ops;
+    if (const char* v = getenv("LBM_STEP_VARIANT")) c->variant = atoi(v) & 3;
     hipGetDevice(&c->device);
     c->lay = lay;
     c->tau0 = round_real(c, round_real(c, 3.0 * round_real(c, d.kinevisc)) + 0.5);
@@ -466,6 +469,7 @@ int lbm_step(lbm_ctx* c, int it0, int nsteps) {
     h.vre = d.ivreman == 1;
     h.block_x = d.block_x;
     h.block_y = d.block_y;
+    h.variant = c->variant;
     const size_t slice = (size_t)d.ny * d.nz * c->ops->real_bytes;
     for (int it = it0 + 1; it <= it0 + nsteps; ++it) {
         if (c->pending) {

@@ -49,28 +49,40 @@ struct CodecF64 {
 
 constexpr float kHalfScale = 1024.0f;
 
+// 16-bit codecs keep the raw bits in an unsigned short so that every codec moves plain scalars.
 struct CodecF16S {
-    using raw = __half;
+    using raw = unsigned short;
     using real = float;
     static constexpr int kind = STORE_F16S;
     template <int Q>
-    __device__ static inline real shifted(raw r, int) { return __half2float(r) * (1.0f / kHalfScale); }
+    __device__ static inline real shifted(raw r, int) { return __half2float(__ushort_as_half(r)) * (1.0f / kHalfScale); }
     template <int Q>
     __device__ static inline real decode(raw r, int l) { return weight<Q, real>(l) + shifted<Q>(r, l); }
     template <int Q>
-    __device__ static inline raw encode(real v, int l) { return __float2half_rn((v - weight<Q, real>(l)) * kHalfScale); }
+    __device__ static inline raw encode(real v, int l) {
+        return __half_as_ushort(__float2half_rn((v - weight<Q, real>(l)) * kHalfScale));
+    }
 };
 
+// bfloat16 = the upper half of an IEEE float; encode rounds to nearest even.
+__device__ inline float bf16_bits_to_float(unsigned short r) { return __uint_as_float((unsigned)r << 16); }
+__device__ inline unsigned short float_to_bf16_bits(float x) {
+    unsigned u = __float_as_uint(x);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
 struct CodecBF16S {
-    using raw = __hip_bfloat16;
+    using raw = unsigned short;
     using real = float;
     static constexpr int kind = STORE_BF16S;
     template <int Q>
-    __device__ static inline real shifted(raw r, int) { return __bfloat162float(r); }
+    __device__ static inline real shifted(raw r, int) { return bf16_bits_to_float(r); }
     template <int Q>
     __device__ static inline real decode(raw r, int l) { return weight<Q, real>(l) + shifted<Q>(r, l); }
     template <int Q>
-    __device__ static inline raw encode(real v, int l) { return __float2bfloat16(v - weight<Q, real>(l)); }
+    __device__ static inline raw encode(real v, int l) { return float_to_bf16_bits(v - weight<Q, real>(l)); }
 };
 
 }  // namespace lbm

@@ -74,7 +74,8 @@ __device__ __forceinline__ R feq_dir(int l, R rho, R ux, R uy, R uz, R usq) {
     return (weight<Q, R>(l) * rho) * ((R(1) - R(1.5) * usq) + cu * poly);
 }
 
-template <int Q, class C, bool BGK3, bool HRR, bool VRE>
+// VAR bit 0: non-temporal stores, bit 1: non-temporal loads.
+template <int Q, class C, bool BGK3, bool HRR, bool VRE, int VAR>
 __global__ void __launch_bounds__(1024) step_kernel(const StepArgs<typename C::real> a) {
     using R = typename C::real;
     using raw = typename C::raw;
@@ -102,8 +103,11 @@ __global__ void __launch_bounds__(1024) step_kernel(const StepArgs<typename C::r
     const raw* __restrict__ src = static_cast<const raw*>(a.src);
     raw rv[Q];
 #pragma unroll
-    for (int l = 0; l < Q; ++l)
-        rv[l] = src[(size_t)l * (size_t)L.plane + (size_t)(oz[Lt::cz(l) + 1] + oy[Lt::cy(l) + 1] + ox[Lt::cx(l) + 1])];
+    for (int l = 0; l < Q; ++l) {
+        const raw* p = src + (size_t)l * (size_t)L.plane + (size_t)(oz[Lt::cz(l) + 1] + oy[Lt::cy(l) + 1] + ox[Lt::cx(l) + 1]);
+        if (VAR & 2) rv[l] = __builtin_nontemporal_load(p);
+        else rv[l] = *p;
+    }
 
     // Moments in the reference's order (src/m_macrovars_kernel.F90:41-49), plus the second
     // moments of the weight-shifted populations for the non-equilibrium tensor.
@@ -237,8 +241,11 @@ __global__ void __launch_bounds__(1024) step_kernel(const StepArgs<typename C::r
 
     raw* __restrict__ dst = static_cast<raw*>(a.dst);
 #pragma unroll
-    for (int l = 0; l < Q; ++l)
-        dst[(size_t)l * (size_t)L.plane + (size_t)oc] = C::template encode<Q>(out[l], l);
+    for (int l = 0; l < Q; ++l) {
+        raw* p = dst + (size_t)l * (size_t)L.plane + (size_t)oc;
+        if (VAR & 1) __builtin_nontemporal_store(C::template encode<Q>(out[l], l), p);
+        else *p = C::template encode<Q>(out[l], l);
+    }
     if (VRE && a.tau_out != nullptr) a.tau_out[oc] = tau;
 }
 

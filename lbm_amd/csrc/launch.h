@@ -29,8 +29,13 @@ struct Launch {
     using R = typename C::real;
 
     template <bool B3, bool H, bool V>
-    static hipError_t step_t(const Layout& L, const StepArgs<R>& a, dim3 grid, dim3 block, hipStream_t s) {
-        hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V>), grid, block, 0, s, a);
+    static hipError_t step_t(const Layout& L, const StepArgs<R>& a, dim3 grid, dim3 block, int var, hipStream_t s) {
+        switch (var & 3) {
+            case 0: hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 0>), grid, block, 0, s, a); break;
+            case 1: hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 1>), grid, block, 0, s, a); break;
+            case 2: hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 2>), grid, block, 0, s, a); break;
+            default: hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 3>), grid, block, 0, s, a); break;
+        }
         return hipGetLastError();
     }
 
@@ -65,14 +70,14 @@ struct Launch {
         const dim3 grid((L.n(L.ax0) + h.block_x - 1) / h.block_x, (L.n(L.ax1) + h.block_y - 1) / h.block_y, L.n(L.ax2));
         const int key = (h.bgk3 ? 4 : 0) | (h.hrr ? 2 : 0) | (h.vre ? 1 : 0);
         switch (key) {
-            case 0: return step_t<false, false, false>(L, a, grid, block, s);
-            case 1: return step_t<false, false, true>(L, a, grid, block, s);
-            case 2: return step_t<false, true, false>(L, a, grid, block, s);
-            case 3: return step_t<false, true, true>(L, a, grid, block, s);
-            case 4: return step_t<true, false, false>(L, a, grid, block, s);
-            case 5: return step_t<true, false, true>(L, a, grid, block, s);
-            case 6: return step_t<true, true, false>(L, a, grid, block, s);
-            default: return step_t<true, true, true>(L, a, grid, block, s);
+            case 0: return step_t<false, false, false>(L, a, grid, block, h.variant, s);
+            case 1: return step_t<false, false, true>(L, a, grid, block, h.variant, s);
+            case 2: return step_t<false, true, false>(L, a, grid, block, h.variant, s);
+            case 3: return step_t<false, true, true>(L, a, grid, block, h.variant, s);
+            case 4: return step_t<true, false, false>(L, a, grid, block, h.variant, s);
+            case 5: return step_t<true, false, true>(L, a, grid, block, h.variant, s);
+            case 6: return step_t<true, true, false>(L, a, grid, block, h.variant, s);
+            default: return step_t<true, true, true>(L, a, grid, block, h.variant, s);
         }
     }
 
