@@ -87,7 +87,7 @@ def cpu_baseline(w, f0, turb, sample_steps, threads):
     tf = tuple(a[: nr + 1] for a in turb)
     wd = tempfile.mkdtemp(prefix="lbm_cpu_")
     try:
-        _, out = oracle.run_reference(exe, f0, nt0=1, nt1=1 + sample_steps, turb_fields=tf, workdir=wd,
+        _, out = oracle.run_reference(exe, f0, nt0=1, nt1=1 + sample_steps, turb_fields=tf, workdir=wd, read_output=False,
                                       threads=threads, ibgk=3, ihrr=1, ivreman=1, smagorinsky=float(np.float32(SMAG)),
                                       ibnd=1, jbnd=0, kbnd=12, uini=w["uini"], udir=0.0, visc=w["kinevisc"],
                                       ampl=w["ampl"], nrturb=nr, experiment="cylinder", ltiming=True,

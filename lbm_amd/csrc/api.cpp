@@ -52,7 +52,8 @@ struct lbm_ctx {
     int series_lo = 0, series_hi = -1;
     std::vector<double> ut_series, ud_series;
     double tau0 = 0.0, fac0 = 0.0;
-    int variant = 0;               // step-kernel variant (LBM_STEP_VARIANT), see impl.h
+    int variant = 1;               // step-kernel variant (LBM_STEP_VARIANT), see impl.h; non-temporal
+                                   // stores measured fastest on MI355X (5.32 vs 5.11 TB/s)
     // timing
     bool timing = false;
     std::vector<hipEvent_t> ev_step, ev_bc;   // pairs

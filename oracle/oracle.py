@@ -178,7 +178,8 @@ def infile_text(*, ibgk=2, ihrr=0, ivreman=0, smagorinsky=0.0, nt0=1, nt1=2, ibn
     return "\n".join(rows) + "\n"
 
 
-def run_reference(exe, f_in, *, nt0, nt1, turb_fields=None, workdir=None, threads=1, timeout=3600, **opts):
+def run_reference(exe, f_in, *, nt0, nt1, turb_fields=None, workdir=None, threads=1, timeout=3600,
+                  read_output=True, **opts):
     """Run a reference executable (built by oracle/build_ref.py) from restart state ``f_in``
     at step nt0 to nt1.  Returns (f_out, stdout).  f_out is the restart record written at
     nt1 (post-streaming f1 with ghosts, src/main.F90:398)."""
@@ -201,6 +202,8 @@ def run_reference(exe, f_in, *, nt0, nt1, turb_fields=None, workdir=None, thread
                            capture_output=True, text=True, timeout=timeout)
         if p.returncode != 0 or "infile.in problem" in p.stdout:
             raise RuntimeError(f"reference run failed ({p.returncode}):\n{p.stdout[-3000:]}\n{p.stderr[-3000:]}")
+        if not read_output:
+            return None, p.stdout
         f_out = fio.read_restart(os.path.join(wd, "restart", fio.restart_name(nt1)))
         return f_out, p.stdout
     finally:
