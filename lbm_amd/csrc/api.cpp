@@ -346,7 +346,11 @@ int lbm_create(const lbm_desc* desc, lbm_ctx** out) {
     c->d.block_y = by;
     if (c->d.iturb_pos == 0) c->d.iturb_pos = 10;
     c->ops = ops;
-    if (const char* v = getenv("LBM_STEP_VARIANT")) c->variant = atoi(v) & 3;
+    // 16-bit storage moves two cells per lane (aligned 32-bit words) when the contiguous axis
+    // has an even length; LBM_STEP_VARIANT overrides (bit 0 non-temporal stores, bit 2 pairs).
+    if ((d.storage == LBM_STORE_F16S || d.storage == LBM_STORE_BF16S) && lay.n(lay.ax0) % 2 == 0) c->variant |= 4;
+    if (const char* v = getenv("LBM_STEP_VARIANT")) c->variant = atoi(v) & 5;
+    if ((c->variant & 4) && (lay.n(lay.ax0) % 2 != 0 || ops->raw_bytes > 4)) c->variant &= ~4;
     hipGetDevice(&c->device);
     c->lay = lay;
     c->tau0 = round_real(c, round_real(c, 3.0 * round_real(c, d.kinevisc)) + 0.5);

@@ -46,7 +46,7 @@ struct StepHost {
     const void* F[3];
     void* tau_out;
     int bgk3, hrr, vre;
-    int variant;      // step-kernel variant: bit 0 non-temporal stores, bit 1 non-temporal loads
+    int variant;      // step-kernel variant: bit 0 non-temporal stores, bit 2 two cells per lane
     int block_x, block_y;
 };
 

@@ -74,41 +74,16 @@ __device__ __forceinline__ R feq_dir(int l, R rho, R ux, R uy, R uz, R usq) {
     return (weight<Q, R>(l) * rho) * ((R(1) - R(1.5) * usq) + cu * poly);
 }
 
-// VAR bit 0: non-temporal stores, bit 1: non-temporal loads.
-template <int Q, class C, bool BGK3, bool HRR, bool VRE, int VAR>
-__global__ void __launch_bounds__(1024) step_kernel(const StepArgs<typename C::real> a) {
+// The per-cell part of a step: moments, collision, turbulence and body-force increments, solid
+// swap.  rv holds the streamed populations of cell (i,j,k) (storage coordinates ca,cb,cc, device
+// cell offset oc); out receives its post-collision populations, tau the Vreman relaxation time.
+template <int Q, class C, bool BGK3, bool HRR, bool VRE>
+__device__ __forceinline__ void collide_cell(const StepArgs<typename C::real>& a, const typename C::raw (&rv)[Q], int i,
+                                             int j, int k, int ca, int cb, int cc, int oc,
+                                             typename C::real (&out)[Q], typename C::real& tau_out) {
     using R = typename C::real;
-    using raw = typename C::raw;
     using Lt = Lattice<Q>;
     const Layout& L = a.lay;
-    // Threads run along the storage axes: x along the contiguous axis ax0, y along ax1, z along ax2.
-    const int ca = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int cb = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const int cc = blockIdx.z + 1;
-    if (ca > L.n(L.ax0) || cb > L.n(L.ax1) || cc > L.n(L.ax2)) return;
-    int i, j, k;
-    L.ijk(ca, cb, cc, i, j, k);
-
-    // Source offsets: population l comes from (i - cx, j - cy, k - cz), periodic directions
-    // folded onto the interior.  Index d = c + 1.
-    const int s = a.stream;
-    int ox[3], oy[3], oz[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        const int c = d - 1;
-        ox[d] = L.base + Layout::wrap(i - s * c, L.nx, L.per_i) * L.si;
-        oy[d] = Layout::wrap(j - s * c, L.ny, L.per_j) * L.sj;
-        oz[d] = Layout::wrap(k - s * c, L.nz, L.per_k) * L.sk;
-    }
-    const raw* __restrict__ src = static_cast<const raw*>(a.src);
-    raw rv[Q];
-#pragma unroll
-    for (int l = 0; l < Q; ++l) {
-        const raw* p = src + (size_t)l * (size_t)L.plane + (size_t)(oz[Lt::cz(l) + 1] + oy[Lt::cy(l) + 1] + ox[Lt::cx(l) + 1]);
-        if (VAR & 2) rv[l] = __builtin_nontemporal_load(p);
-        else rv[l] = *p;
-    }
-
     // Moments in the reference's order (src/m_macrovars_kernel.F90:41-49), plus the second
     // moments of the weight-shifted populations for the non-equilibrium tensor.
     R fv[Q];
@@ -135,7 +110,6 @@ __global__ void __launch_bounds__(1024) step_kernel(const StepArgs<typename C::r
     }
     const R invrho = R(1) / rho;
     R rh = rho, ux = jx * invrho, uy = jy * invrho, uz = jz * invrho;
-    const int oc = L.cell(i, j, k);
     if (a.ext_rho) { rh = a.ext_rho[oc]; ux = a.ext_u[oc]; uy = a.ext_v[oc]; uz = a.ext_w[oc]; }
     const R usq = ux * ux + uy * uy + uz * uz;
 
@@ -182,7 +156,6 @@ __global__ void __launch_bounds__(1024) step_kernel(const StepArgs<typename C::r
     const R gy = axy * ux + ayy * uy + ayz * uz;
     const R gz = axz * ux + ayz * uy + azz * uz;
 
-    R out[Q];
 #pragma unroll
     for (int l = 0; l < Q; ++l) {
         const int cx = Lt::cx(l), cy = Lt::cy(l), cz = Lt::cz(l);
@@ -239,6 +212,49 @@ __global__ void __launch_bounds__(1024) step_kernel(const StepArgs<typename C::r
         }
     }
 
+    tau_out = tau;
+}
+
+// VAR bit 0: non-temporal stores, bit 1: non-temporal loads (measured slower; kept for tuning).
+template <int Q, class C, bool BGK3, bool HRR, bool VRE, int VAR>
+__global__ void __launch_bounds__(1024) step_kernel(const StepArgs<typename C::real> a) {
+    using R = typename C::real;
+    using raw = typename C::raw;
+    using Lt = Lattice<Q>;
+    const Layout& L = a.lay;
+    // Threads run along the storage axes: x along the contiguous axis ax0, y along ax1, z along ax2.
+    const int ca = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int cb = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int cc = blockIdx.z + 1;
+    if (ca > L.n(L.ax0) || cb > L.n(L.ax1) || cc > L.n(L.ax2)) return;
+    int i, j, k;
+    L.ijk(ca, cb, cc, i, j, k);
+
+    // Source offsets: population l comes from (i - cx, j - cy, k - cz), periodic directions
+    // folded onto the interior.  Index d = c + 1.
+    const int s = a.stream;
+    int ox[3], oy[3], oz[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const int c = d - 1;
+        ox[d] = L.base + Layout::wrap(i - s * c, L.nx, L.per_i) * L.si;
+        oy[d] = Layout::wrap(j - s * c, L.ny, L.per_j) * L.sj;
+        oz[d] = Layout::wrap(k - s * c, L.nz, L.per_k) * L.sk;
+    }
+    const raw* __restrict__ src = static_cast<const raw*>(a.src);
+    raw rv[Q];
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+        const raw* p = src + (size_t)l * (size_t)L.plane + (size_t)(oz[Lt::cz(l) + 1] + oy[Lt::cy(l) + 1] + ox[Lt::cx(l) + 1]);
+        if (VAR & 2) rv[l] = __builtin_nontemporal_load(p);
+        else rv[l] = *p;
+    }
+
+    R out[Q];
+    R tau = 0;
+    const int oc = L.cell(i, j, k);
+    collide_cell<Q, C, BGK3, HRR, VRE>(a, rv, i, j, k, ca, cb, cc, oc, out, tau);
+
     raw* __restrict__ dst = static_cast<raw*>(a.dst);
 #pragma unroll
     for (int l = 0; l < Q; ++l) {
@@ -247,6 +263,100 @@ __global__ void __launch_bounds__(1024) step_kernel(const StepArgs<typename C::r
         else *p = C::template encode<Q>(out[l], l);
     }
     if (VRE && a.tau_out != nullptr) a.tau_out[oc] = tau;
+}
+
+// Two raw values packed in one aligned word: 32 bits for 16-bit storage, 64 bits for FP32.
+template <typename raw>
+struct PairWord;
+template <>
+struct PairWord<unsigned short> {
+    using W = unsigned int;
+    __device__ static inline unsigned short half(W w, int hi) { return (unsigned short)(hi ? (w >> 16) : (w & 0xffffu)); }
+    __device__ static inline W pack(unsigned short lo, unsigned short hi) { return (W)lo | ((W)hi << 16); }
+};
+template <>
+struct PairWord<float> {
+    using W = unsigned long long;
+    __device__ static inline float half(W w, int hi) { return __uint_as_float((unsigned)(hi ? (w >> 32) : (w & 0xffffffffull))); }
+    __device__ static inline W pack(float lo, float hi) {
+        return (W)__float_as_uint(lo) | ((W)__float_as_uint(hi) << 32);
+    }
+};
+
+// Two adjacent cells along the contiguous axis per lane.  Every access is an aligned word of two
+// raw values, so a 64-lane wave moves 2*64*sizeof(raw) bytes per population (256 B for 16-bit
+// storage, where one value per lane would move only one 128-B line per instruction).  A
+// population shifted along the contiguous axis takes its two values from two aligned words.
+// Needs an even number of cells along the contiguous axis.
+template <int Q, class C, bool BGK3, bool HRR, bool VRE, int VAR>
+__global__ void __launch_bounds__(1024) step_kernel_pair(const StepArgs<typename C::real> a) {
+    using R = typename C::real;
+    using raw = typename C::raw;
+    using PW = PairWord<raw>;
+    using W = typename PW::W;
+    using Lt = Lattice<Q>;
+    const Layout& L = a.lay;
+    const int ca = 2 * (blockIdx.x * blockDim.x + threadIdx.x) + 1;
+    const int cb = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int cc = blockIdx.z + 1;
+    const int n0 = L.n(L.ax0);
+    if (ca > n0 || cb > L.n(L.ax1) || cc > L.n(L.ax2)) return;
+    int i, j, k, i2, j2, k2;
+    L.ijk(ca, cb, cc, i, j, k);
+    L.ijk(ca + 1, cb, cc, i2, j2, k2);
+    const int s = a.stream;
+    // Offsets of the first cell along i, j, k (index d = c + 1), and of the second cell's
+    // position along the contiguous axis.
+    int o1[3][3], o2f[3];
+    const int xc[3] = {i, j, k}, n[3] = {L.nx, L.ny, L.nz}, per[3] = {L.per_i, L.per_j, L.per_k};
+    const int st[3] = {L.si, L.sj, L.sk};
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax)
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            o1[ax][d] = Layout::wrap(xc[ax] - s * (d - 1), n[ax], per[ax]) * st[ax];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) o2f[d] = Layout::wrap(ca + 1 - s * (d - 1), n0, per[L.ax0]);
+    const raw* __restrict__ src = static_cast<const raw*>(a.src);
+    raw rA[Q], rB[Q];
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+        const int c[3] = {Lt::cx(l), Lt::cy(l), Lt::cz(l)};
+        const int cf = L.ax0 == 0 ? c[0] : L.ax0 == 1 ? c[1] : c[2];
+        // slow-axis part of the offset (the contiguous axis has stride 1)
+        const int offA = o1[0][c[0] + 1] + o1[1][c[1] + 1] + o1[2][c[2] + 1];
+        const size_t pl = (size_t)l * (size_t)L.plane + (size_t)L.base;
+        const long long eA = (long long)pl + offA;
+        if (cf == 0 || s == 0) {
+            const W w = *reinterpret_cast<const W*>(src + eA);
+            rA[l] = PW::half(w, 0);
+            rB[l] = PW::half(w, 1);
+        } else {
+            const int fA = L.ax0 == 0 ? o1[0][cf + 1] : L.ax0 == 1 ? o1[1][cf + 1] : o1[2][cf + 1];
+            const long long eB = eA - fA + o2f[cf + 1];
+            const W wa = *reinterpret_cast<const W*>(src + (eA & ~1LL));
+            const W wb = *reinterpret_cast<const W*>(src + (eB & ~1LL));
+            rA[l] = PW::half(wa, (int)(eA & 1));
+            rB[l] = PW::half(wb, (int)(eB & 1));
+        }
+    }
+    const int oc = L.cell(i, j, k);
+    R outA[Q], outB[Q];
+    R tauA = 0, tauB = 0;
+    collide_cell<Q, C, BGK3, HRR, VRE>(a, rA, i, j, k, ca, cb, cc, oc, outA, tauA);
+    collide_cell<Q, C, BGK3, HRR, VRE>(a, rB, i2, j2, k2, ca + 1, cb, cc, oc + 1, outB, tauB);
+    raw* __restrict__ dst = static_cast<raw*>(a.dst);
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+        W* p = reinterpret_cast<W*>(dst + (size_t)l * (size_t)L.plane + (size_t)oc);
+        const W w = PW::pack(C::template encode<Q>(outA[l], l), C::template encode<Q>(outB[l], l));
+        if (VAR & 1) __builtin_nontemporal_store(w, p);
+        else *p = w;
+    }
+    if (VRE && a.tau_out != nullptr) {
+        a.tau_out[oc] = tauA;
+        a.tau_out[oc + 1] = tauB;
+    }
 }
 
 // ------------------------------------------------------------------------------------------

@@ -30,12 +30,22 @@ struct Launch {
 
     template <bool B3, bool H, bool V>
     static hipError_t step_t(const Layout& L, const StepArgs<R>& a, dim3 grid, dim3 block, int var, hipStream_t s) {
-        switch (var & 3) {
-            case 0: hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 0>), grid, block, 0, s, a); break;
-            case 1: hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 1>), grid, block, 0, s, a); break;
-            case 2: hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 2>), grid, block, 0, s, a); break;
-            default: hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 3>), grid, block, 0, s, a); break;
+        const bool nt = var & 1;
+        if constexpr (sizeof(typename C::raw) <= 4) {
+            if (var & 4) {
+                // two cells per lane along the contiguous axis
+                const int half = L.n(L.ax0) / 2;
+                int bx = (half + 63) / 64 * 64;
+                if (bx > 512) bx = 256;
+                const int by = bx >= 256 ? 1 : 256 / bx;
+                const dim3 g((half + bx - 1) / bx, (L.n(L.ax1) + by - 1) / by, L.n(L.ax2));
+                if (nt) hipLaunchKernelGGL((step_kernel_pair<Q, C, B3, H, V, 1>), g, dim3(bx, by, 1), 0, s, a);
+                else hipLaunchKernelGGL((step_kernel_pair<Q, C, B3, H, V, 0>), g, dim3(bx, by, 1), 0, s, a);
+                return hipGetLastError();
+            }
         }
+        if (nt) hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 1>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 0>), grid, block, 0, s, a);
         return hipGetLastError();
     }
 
