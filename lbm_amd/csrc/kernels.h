@@ -288,19 +288,24 @@ struct PairWord<float> {
 // storage, where one value per lane would move only one 128-B line per instruction).  A
 // population shifted along the contiguous axis takes its two values from two aligned words.
 // Needs an even number of cells along the contiguous axis.
+// Threads are numbered over (pair, row) of one ax2 slab so that no lane idles when a row holds
+// a non-multiple of 64 pairs; blocks of at most 256 threads leave the compiler room to keep both
+// cells in registers.
 template <int Q, class C, bool BGK3, bool HRR, bool VRE, int VAR>
-__global__ void __launch_bounds__(1024) step_kernel_pair(const StepArgs<typename C::real> a) {
+__global__ void __launch_bounds__(256) step_kernel_pair(const StepArgs<typename C::real> a) {
     using R = typename C::real;
     using raw = typename C::raw;
     using PW = PairWord<raw>;
     using W = typename PW::W;
     using Lt = Lattice<Q>;
     const Layout& L = a.lay;
-    const int ca = 2 * (blockIdx.x * blockDim.x + threadIdx.x) + 1;
-    const int cb = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const int cc = blockIdx.z + 1;
     const int n0 = L.n(L.ax0);
-    if (ca > n0 || cb > L.n(L.ax1) || cc > L.n(L.ax2)) return;
+    const int npair = n0 >> 1;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npair * L.n(L.ax1)) return;
+    const int ca = 2 * (t % npair) + 1;
+    const int cb = t / npair + 1;
+    const int cc = blockIdx.y + 1;
     int i, j, k, i2, j2, k2;
     L.ijk(ca, cb, cc, i, j, k);
     L.ijk(ca + 1, cb, cc, i2, j2, k2);

@@ -34,13 +34,10 @@ struct Launch {
         if constexpr (sizeof(typename C::raw) <= 4) {
             if (var & 4) {
                 // two cells per lane along the contiguous axis
-                const int half = L.n(L.ax0) / 2;
-                int bx = (half + 63) / 64 * 64;
-                if (bx > 512) bx = 256;
-                const int by = bx >= 256 ? 1 : 256 / bx;
-                const dim3 g((half + bx - 1) / bx, (L.n(L.ax1) + by - 1) / by, L.n(L.ax2));
-                if (nt) hipLaunchKernelGGL((step_kernel_pair<Q, C, B3, H, V, 1>), g, dim3(bx, by, 1), 0, s, a);
-                else hipLaunchKernelGGL((step_kernel_pair<Q, C, B3, H, V, 0>), g, dim3(bx, by, 1), 0, s, a);
+                const long long nt_slab = (long long)(L.n(L.ax0) / 2) * L.n(L.ax1);
+                const dim3 g((unsigned)((nt_slab + 255) / 256), L.n(L.ax2), 1);
+                if (nt) hipLaunchKernelGGL((step_kernel_pair<Q, C, B3, H, V, 1>), g, dim3(256, 1, 1), 0, s, a);
+                else hipLaunchKernelGGL((step_kernel_pair<Q, C, B3, H, V, 0>), g, dim3(256, 1, 1), 0, s, a);
                 return hipGetLastError();
             }
         }
