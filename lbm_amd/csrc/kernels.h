@@ -291,7 +291,7 @@ struct PairWord<float> {
 // Threads are numbered over (pair, row) of one ax2 slab so that no lane idles when a row holds
 // a non-multiple of 64 pairs; blocks of at most 256 threads leave the compiler room to keep both
 // cells in registers.
-template <int Q, class C, bool BGK3, bool HRR, bool VRE, int VAR>
+template <int Q, class C, bool BGK3, bool HRR, bool VRE, int VAR, int AX0>
 __global__ void __launch_bounds__(256) step_kernel_pair(const StepArgs<typename C::real> a) {
     using R = typename C::real;
     using raw = typename C::raw;
@@ -321,23 +321,26 @@ __global__ void __launch_bounds__(256) step_kernel_pair(const StepArgs<typename 
         for (int d = 0; d < 3; ++d)
             o1[ax][d] = Layout::wrap(xc[ax] - s * (d - 1), n[ax], per[ax]) * st[ax];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) o2f[d] = Layout::wrap(ca + 1 - s * (d - 1), n0, per[L.ax0]);
+    for (int d = 0; d < 3; ++d) o2f[d] = Layout::wrap(ca + 1 - s * (d - 1), n0, per[AX0]);
     const raw* __restrict__ src = static_cast<const raw*>(a.src);
     raw rA[Q], rB[Q];
 #pragma unroll
     for (int l = 0; l < Q; ++l) {
         const int c[3] = {Lt::cx(l), Lt::cy(l), Lt::cz(l)};
-        const int cf = L.ax0 == 0 ? c[0] : L.ax0 == 1 ? c[1] : c[2];
+        const int cf = c[AX0];   // velocity component along the contiguous axis (compile time)
         // slow-axis part of the offset (the contiguous axis has stride 1)
         const int offA = o1[0][c[0] + 1] + o1[1][c[1] + 1] + o1[2][c[2] + 1];
         const size_t pl = (size_t)l * (size_t)L.plane + (size_t)L.base;
         const long long eA = (long long)pl + offA;
-        if (cf == 0 || s == 0) {
+        // No run-time branch here: a branch per population would split the loads into basic
+        // blocks and serialise their latencies.  The two-word form is also right for s == 0
+        // (both words are then the cell pair's own word).
+        if (cf == 0) {
             const W w = *reinterpret_cast<const W*>(src + eA);
             rA[l] = PW::half(w, 0);
             rB[l] = PW::half(w, 1);
         } else {
-            const int fA = L.ax0 == 0 ? o1[0][cf + 1] : L.ax0 == 1 ? o1[1][cf + 1] : o1[2][cf + 1];
+            const int fA = o1[AX0][cf + 1];
             const long long eB = eA - fA + o2f[cf + 1];
             const W wa = *reinterpret_cast<const W*>(src + (eA & ~1LL));
             const W wb = *reinterpret_cast<const W*>(src + (eB & ~1LL));

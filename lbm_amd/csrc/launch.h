@@ -36,8 +36,14 @@ struct Launch {
                 // two cells per lane along the contiguous axis
                 const long long nt_slab = (long long)(L.n(L.ax0) / 2) * L.n(L.ax1);
                 const dim3 g((unsigned)((nt_slab + 255) / 256), L.n(L.ax2), 1);
-                if (nt) hipLaunchKernelGGL((step_kernel_pair<Q, C, B3, H, V, 1>), g, dim3(256, 1, 1), 0, s, a);
-                else hipLaunchKernelGGL((step_kernel_pair<Q, C, B3, H, V, 0>), g, dim3(256, 1, 1), 0, s, a);
+                // the contiguous axis is a template parameter: velocity components along it
+                // must be compile-time constants to keep the loads free of branches
+                switch (L.ax0) {
+                    case 0: hipLaunchKernelGGL((step_kernel_pair<Q, C, B3, H, V, 1, 0>), g, dim3(256, 1, 1), 0, s, a); break;
+                    case 1: hipLaunchKernelGGL((step_kernel_pair<Q, C, B3, H, V, 1, 1>), g, dim3(256, 1, 1), 0, s, a); break;
+                    default: hipLaunchKernelGGL((step_kernel_pair<Q, C, B3, H, V, 1, 2>), g, dim3(256, 1, 1), 0, s, a); break;
+                }
+                (void)nt;   // the pair kernel always stores non-temporally
                 return hipGetLastError();
             }
         }
