@@ -74,9 +74,25 @@ __device__ __forceinline__ R feq_dir(int l, R rho, R ux, R uy, R uz, R usq) {
     return (weight<Q, R>(l) * rho) * ((R(1) - R(1.5) * usq) + cu * poly);
 }
 
+// Even and odd parts of the equilibrium of direction c (and of -c): feq(c) = E + O,
+// feq(-c) = E - O, with cu = c.u (src/m_postcoll_kernel.F90:136-183 in closed form):
+//   E = w rho (1 - 1.5 u.u + 4.5 cu^2),  O = w rho cu (3 [+ 4.5 (cu^2 - u.u)])
+template <bool ORDER3, typename R>
+__device__ __forceinline__ void feq_even_odd(R wr, R base, R cu, R usq, R& E, R& O) {
+    const R cu2 = cu * cu;
+    E = wr * (base + R(4.5) * cu2);
+    R pol = R(3);
+    if (ORDER3) pol += R(4.5) * (cu2 - usq);
+    O = wr * (cu * pol);
+}
+
 // The per-cell part of a step: moments, collision, turbulence and body-force increments, solid
 // swap.  rv holds the streamed populations of cell (i,j,k) (storage coordinates ca,cb,cc, device
 // cell offset oc); out receives its post-collision populations, tau the Vreman relaxation time.
+//
+// Directions come in opposite pairs (l, l+1) for odd l (src/mod_D3Q27setup.F90:31,43).  Every
+// quantity the collision forms per direction is even or odd in c, so each pair is evaluated
+// once: moments accumulate from f_l +- f_l+1, and the outputs are (even) +- (odd).
 template <int Q, class C, bool BGK3, bool HRR, bool VRE>
 __device__ __forceinline__ void collide_cell(const StepArgs<typename C::real>& a, const typename C::raw (&rv)[Q], int i,
                                              int j, int k, int ca, int cb, int cc, int oc,
@@ -84,34 +100,42 @@ __device__ __forceinline__ void collide_cell(const StepArgs<typename C::real>& a
     using R = typename C::real;
     using Lt = Lattice<Q>;
     const Layout& L = a.lay;
-    // Moments in the reference's order (src/m_macrovars_kernel.F90:41-49), plus the second
-    // moments of the weight-shifted populations for the non-equilibrium tensor.
+    constexpr int NP = (Q - 1) / 2;
+
+    // Moments.  sp = f_l + f_l+1 and dp = f_l - f_l+1 per pair; the shifted sum
+    // ss = (f_l - w) + (f_l+1 - w) feeds the second moments of the weight-shifted populations.
     R fv[Q];
-    R rho = 0, jx = 0, jy = 0, jz = 0;
-    R dr = 0, pxx = 0, pyy = 0, pzz = 0, pxy = 0, pxz = 0, pyz = 0;
 #pragma unroll
-    for (int l = 0; l < Q; ++l) {
-        const R f = C::template decode<Q>(rv[l], l);
-        fv[l] = f;
-        rho += f;
-        if (Lt::cx(l) > 0) jx += f; else if (Lt::cx(l) < 0) jx -= f;
-        if (Lt::cy(l) > 0) jy += f; else if (Lt::cy(l) < 0) jy -= f;
-        if (Lt::cz(l) > 0) jz += f; else if (Lt::cz(l) < 0) jz -= f;
+    for (int l = 0; l < Q; ++l) fv[l] = C::template decode<Q>(rv[l], l);
+    R rho = fv[0], jx = 0, jy = 0, jz = 0;
+    R dr = C::template shifted<Q>(rv[0], 0);
+    R pxx = 0, pyy = 0, pzz = 0, pxy = 0, pxz = 0, pyz = 0;
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+        const int l = 2 * p + 1;
+        const int cx = Lt::cx(l), cy = Lt::cy(l), cz = Lt::cz(l);
+        const R sp = fv[l] + fv[l + 1];
+        const R dp = fv[l] - fv[l + 1];
+        rho += sp;
+        if (cx > 0) jx += dp; else if (cx < 0) jx -= dp;
+        if (cy > 0) jy += dp; else if (cy < 0) jy -= dp;
+        if (cz > 0) jz += dp; else if (cz < 0) jz -= dp;
         if (HRR || VRE) {
-            const R d = C::template shifted<Q>(rv[l], l);
-            dr += d;
-            if (Lt::cx(l) != 0) pxx += d;
-            if (Lt::cy(l) != 0) pyy += d;
-            if (Lt::cz(l) != 0) pzz += d;
-            if (Lt::cx(l) * Lt::cy(l) > 0) pxy += d; else if (Lt::cx(l) * Lt::cy(l) < 0) pxy -= d;
-            if (Lt::cx(l) * Lt::cz(l) > 0) pxz += d; else if (Lt::cx(l) * Lt::cz(l) < 0) pxz -= d;
-            if (Lt::cy(l) * Lt::cz(l) > 0) pyz += d; else if (Lt::cy(l) * Lt::cz(l) < 0) pyz -= d;
+            const R ss = C::template shifted<Q>(rv[l], l) + C::template shifted<Q>(rv[l + 1], l + 1);
+            dr += ss;
+            if (cx != 0) pxx += ss;
+            if (cy != 0) pyy += ss;
+            if (cz != 0) pzz += ss;
+            if (cx * cy > 0) pxy += ss; else if (cx * cy < 0) pxy -= ss;
+            if (cx * cz > 0) pxz += ss; else if (cx * cz < 0) pxz -= ss;
+            if (cy * cz > 0) pyz += ss; else if (cy * cz < 0) pyz -= ss;
         }
     }
     const R invrho = R(1) / rho;
     R rh = rho, ux = jx * invrho, uy = jy * invrho, uz = jz * invrho;
     if (a.ext_rho) { rh = a.ext_rho[oc]; ux = a.ext_u[oc]; uy = a.ext_v[oc]; uz = a.ext_w[oc]; }
     const R usq = ux * ux + uy * uy + uz * uz;
+    const R base = R(1) - R(1.5) * usq;
 
     // Non-equilibrium second-order tensor a2 = sum_l H2_l (f_l - feq_l)
     //   = Pi' - rho u u - (sum_l (f_l - w_l)) I/3   (Pi' = sum_l c c (f_l - w_l)),
@@ -151,30 +175,43 @@ __device__ __forceinline__ void collide_cell(const StepArgs<typename C::real>& a
 
     // Regularized non-equilibrium (src/m_postcoll_kernel.F90:189-244) in closed form:
     //   fneq_l = 4.5 w_l [ cAc (1 + 3 cu) - T (1/3 + cu) - 2 c.g ],  T = tr a2,  g = a2 u
+    // even part 4.5 w (cAc - T/3), odd part 4.5 w (cu (3 cAc - T) - 2 c.g).
     const R T = axx + ayy + azz;
     const R gx = axx * ux + axy * uy + axz * uz;
     const R gy = axy * ux + ayy * uy + ayz * uz;
     const R gz = axz * ux + ayz * uy + azz * uz;
 
+    {   // rest population
+        const R w0 = weight<Q, R>(0), wr0 = w0 * rh;
+        const R feq0 = wr0 * base;
+        const R neq0 = HRR ? -(R(1.5) * w0) * T : fv[0] - feq0;
+        out[0] = feq0 + fac * neq0;
+    }
 #pragma unroll
-    for (int l = 0; l < Q; ++l) {
+    for (int p = 0; p < NP; ++p) {
+        const int l = 2 * p + 1;
         const int cx = Lt::cx(l), cy = Lt::cy(l), cz = Lt::cz(l);
-        const R feq = feq_dir<Q, BGK3, R>(l, rh, ux, uy, uz, usq);
-        R neq;
+        const R w = weight<Q, R>(l);
+        const R cu = sdot<R>(cx, cy, cz, ux, uy, uz);
+        R E, O;
+        feq_even_odd<BGK3, R>(w * rh, base, cu, usq, E, O);
         if (HRR) {
-            const R cu = sdot<R>(cx, cy, cz, ux, uy, uz);
             R cac = R(0);
             if (cx != 0) cac += axx;
             if (cy != 0) cac += ayy;
             if (cz != 0) cac += azz;
-            const R off = sdot<R>(cx * cy, cx * cz, cy * cz, axy, axz, ayz);
-            cac += R(2) * off;
+            cac += R(2) * sdot<R>(cx * cy, cx * cz, cy * cz, axy, axz, ayz);
             const R cg = sdot<R>(cx, cy, cz, gx, gy, gz);
-            neq = (R(4.5) * weight<Q, R>(l)) * (cac * (R(1) + R(3) * cu) - T * (R(1) / R(3) + cu) - R(2) * cg);
+            const R w45 = R(4.5) * w;
+            const R ne = w45 * (cac - T * (R(1) / R(3)));
+            const R no = w45 * (cu * (R(3) * cac - T) - R(2) * cg);
+            const R ev = E + fac * ne, od = O + fac * no;
+            out[l] = ev + od;
+            out[l + 1] = ev - od;
         } else {
-            neq = fv[l] - feq;
+            out[l] = (E + O) + fac * (fv[l] - (E + O));
+            out[l + 1] = (E - O) + fac * (fv[l + 1] - (E - O));
         }
-        out[l] = feq + fac * neq;
     }
 
     // Inflow-turbulence forcing on slice ip (src/m_inflow_turbulence_forcing.F90:57-128 and
@@ -183,9 +220,18 @@ __device__ __forceinline__ void collide_cell(const StepArgs<typename C::real>& a
         const int sl = (k - 1) * L.ny + (j - 1);
         const R vx = ux + a.ampl * a.tu[sl], vy = uy + a.ampl * a.tv[sl], vz = uz + a.ampl * a.tw[sl];
         const R vsq = vx * vx + vy * vy + vz * vz;
+        const R vbase = R(1) - R(1.5) * vsq;
+        out[0] = out[0] + (weight<Q, R>(0) * rh * vbase - weight<Q, R>(0) * rh * base);
 #pragma unroll
-        for (int l = 0; l < Q; ++l)
-            out[l] = out[l] + (feq_dir<Q, BGK3, R>(l, rh, vx, vy, vz, vsq) - feq_dir<Q, BGK3, R>(l, rh, ux, uy, uz, usq));
+        for (int p = 0; p < NP; ++p) {
+            const int l = 2 * p + 1;
+            const R wr = weight<Q, R>(l) * rh;
+            R E1, O1, E0, O0;
+            feq_even_odd<BGK3, R>(wr, vbase, sdot<R>(Lt::cx(l), Lt::cy(l), Lt::cz(l), vx, vy, vz), vsq, E1, O1);
+            feq_even_odd<BGK3, R>(wr, base, sdot<R>(Lt::cx(l), Lt::cy(l), Lt::cz(l), ux, uy, uz), usq, E0, O0);
+            out[l] = out[l] + ((E1 + O1) - (E0 + O0));
+            out[l + 1] = out[l + 1] + ((E1 - O1) - (E0 - O0));
+        }
     }
 
     // External body force, Kupershtokh exact difference with the 2nd-order equilibrium and
@@ -193,9 +239,18 @@ __device__ __forceinline__ void collide_cell(const StepArgs<typename C::real>& a
     if (a.Fx != nullptr && rh > R(0)) {
         const R vx = ux + (-a.Fx[oc] / rh), vy = uy + (-a.Fy[oc] / rh), vz = uz + (-a.Fz[oc] / rh);
         const R vsq = vx * vx + vy * vy + vz * vz;
+        const R vbase = R(1) - R(1.5) * vsq;
+        out[0] = out[0] + (weight<Q, R>(0) * rh * vbase - weight<Q, R>(0) * rh * base);
 #pragma unroll
-        for (int l = 0; l < Q; ++l)
-            out[l] = out[l] + (feq_dir<Q, false, R>(l, rh, vx, vy, vz, vsq) - feq_dir<Q, false, R>(l, rh, ux, uy, uz, usq));
+        for (int p = 0; p < NP; ++p) {
+            const int l = 2 * p + 1;
+            const R wr = weight<Q, R>(l) * rh;
+            R E1, O1, E0, O0;
+            feq_even_odd<false, R>(wr, vbase, sdot<R>(Lt::cx(l), Lt::cy(l), Lt::cz(l), vx, vy, vz), vsq, E1, O1);
+            feq_even_odd<false, R>(wr, base, sdot<R>(Lt::cx(l), Lt::cy(l), Lt::cz(l), ux, uy, uz), usq, E0, O0);
+            out[l] = out[l] + ((E1 + O1) - (E0 + O0));
+            out[l + 1] = out[l + 1] + ((E1 - O1) - (E0 - O0));
+        }
     }
 
     // Solid cells: swap the opposite pairs (2,3),(4,5),... (src/m_solids.F90:19-32).
@@ -211,7 +266,6 @@ __device__ __forceinline__ void collide_cell(const StepArgs<typename C::real>& a
             }
         }
     }
-
     tau_out = tau;
 }
 
