@@ -22,10 +22,17 @@ namespace lbm {
 
 enum StorageKind : int { STORE_F32 = 0, STORE_F64 = 1, STORE_F16S = 2, STORE_BF16S = 3 };
 
+// Every codec also works in "departure space": dep() returns S*(f - w_l) and enc_dep() stores
+// w_l + dev/S, with S = kDepScale a power of two, so that the collision never forms f itself.
 struct CodecF32 {
     using raw = float;
     using real = float;
     static constexpr int kind = STORE_F32;
+    static constexpr float kDepScale = 1.0f;
+    template <int Q>
+    __device__ static inline real dep(raw r, int l) { return r - weight<Q, real>(l); }
+    template <int Q>
+    __device__ static inline raw enc_dep(real dev, int l) { return weight<Q, real>(l) + dev; }
     template <int Q>
     __device__ static inline real decode(raw r, int) { return r; }
     template <int Q>
@@ -39,6 +46,11 @@ struct CodecF64 {
     using raw = double;
     using real = double;
     static constexpr int kind = STORE_F64;
+    static constexpr double kDepScale = 1.0;
+    template <int Q>
+    __device__ static inline real dep(raw r, int l) { return r - weight<Q, real>(l); }
+    template <int Q>
+    __device__ static inline raw enc_dep(real dev, int l) { return weight<Q, real>(l) + dev; }
     template <int Q>
     __device__ static inline real decode(raw r, int) { return r; }
     template <int Q>
@@ -54,6 +66,11 @@ struct CodecF16S {
     using raw = unsigned short;
     using real = float;
     static constexpr int kind = STORE_F16S;
+    static constexpr float kDepScale = kHalfScale;
+    template <int Q>
+    __device__ static inline real dep(raw r, int) { return __half2float(__ushort_as_half(r)); }
+    template <int Q>
+    __device__ static inline raw enc_dep(real dev, int) { return __half_as_ushort(__float2half_rn(dev)); }
     template <int Q>
     __device__ static inline real shifted(raw r, int) { return __half2float(__ushort_as_half(r)) * (1.0f / kHalfScale); }
     template <int Q>
@@ -77,6 +94,11 @@ struct CodecBF16S {
     using raw = unsigned short;
     using real = float;
     static constexpr int kind = STORE_BF16S;
+    static constexpr float kDepScale = 1.0f;
+    template <int Q>
+    __device__ static inline real dep(raw r, int) { return bf16_bits_to_float(r); }
+    template <int Q>
+    __device__ static inline raw enc_dep(real dev, int) { return float_to_bf16_bits(dev); }
     template <int Q>
     __device__ static inline real shifted(raw r, int) { return bf16_bits_to_float(r); }
     template <int Q>

@@ -16,6 +16,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "codec.h"
 #include "lattice.h"
 #include "layout.h"
@@ -74,68 +76,65 @@ __device__ __forceinline__ R feq_dir(int l, R rho, R ux, R uy, R uz, R usq) {
     return (weight<Q, R>(l) * rho) * ((R(1) - R(1.5) * usq) + cu * poly);
 }
 
-// Even and odd parts of the equilibrium of direction c (and of -c): feq(c) = E + O,
-// feq(-c) = E - O, with cu = c.u (src/m_postcoll_kernel.F90:136-183 in closed form):
-//   E = w rho (1 - 1.5 u.u + 4.5 cu^2),  O = w rho cu (3 [+ 4.5 (cu^2 - u.u)])
-template <bool ORDER3, typename R>
-__device__ __forceinline__ void feq_even_odd(R wr, R base, R cu, R usq, R& E, R& O) {
-    const R cu2 = cu * cu;
-    E = wr * (base + R(4.5) * cu2);
-    R pol = R(3);
-    if (ORDER3) pol += R(4.5) * (cu2 - usq);
-    O = wr * (cu * pol);
-}
-
 // The per-cell part of a step: moments, collision, turbulence and body-force increments, solid
 // swap.  rv holds the streamed populations of cell (i,j,k) (storage coordinates ca,cb,cc, device
-// cell offset oc); out receives its post-collision populations, tau the Vreman relaxation time.
+// cell offset oc); dev receives the departures S*(f_l - w_l) of its post-collision populations
+// (S = C::kDepScale), tau the Vreman relaxation time.
 //
 // Directions come in opposite pairs (l, l+1) for odd l (src/mod_D3Q27setup.F90:31,43).  Every
 // quantity the collision forms per direction is even or odd in c, so each pair is evaluated
-// once: moments accumulate from f_l +- f_l+1, and the outputs are (even) +- (odd).
+// once: moments accumulate from D_l +- D_l+1 and the outputs are (even) +- (odd), with
+//   feq_l - w_l = w (rho*base - 1 + 4.5 rho cu^2) + w rho cu (3 [+ 4.5 (cu^2 - u.u)])
+//   (src/m_postcoll_kernel.F90:136-183 in closed form; base = 1 - 1.5 u.u; the bracket is the
+//   reference's full H3:A3 contraction, applied for l > 0 when the third order is on)
+// and the regularized non-equilibrium (src/m_postcoll_kernel.F90:189-244)
+//   fneq_l = 4.5 w [ (cAc - T/3) + (cu (3 cAc - T) - 2 c.g) ],  T = tr a2,  g = a2 u.
+// Working on departures needs no decode/encode arithmetic for the shifted 16-bit formats and
+// forms rho = 1 + sum D/S, j = sum c D/S without the cancellation of sum f.
 template <int Q, class C, bool BGK3, bool HRR, bool VRE>
 __device__ __forceinline__ void collide_cell(const StepArgs<typename C::real>& a, const typename C::raw (&rv)[Q], int i,
                                              int j, int k, int ca, int cb, int cc, int oc,
-                                             typename C::real (&out)[Q], typename C::real& tau_out) {
+                                             typename C::real (&dev)[Q], typename C::real& tau_out) {
     using R = typename C::real;
     using Lt = Lattice<Q>;
     const Layout& L = a.lay;
     constexpr int NP = (Q - 1) / 2;
+    constexpr R S = C::kDepScale;
+    constexpr R invS = R(1) / S;
 
-    // Moments.  sp = f_l + f_l+1 and dp = f_l - f_l+1 per pair; the shifted sum
-    // ss = (f_l - w) + (f_l+1 - w) feeds the second moments of the weight-shifted populations.
-    R fv[Q];
+    R D[Q];
 #pragma unroll
-    for (int l = 0; l < Q; ++l) fv[l] = C::template decode<Q>(rv[l], l);
-    R rho = fv[0], jx = 0, jy = 0, jz = 0;
-    R dr = C::template shifted<Q>(rv[0], 0);
+    for (int l = 0; l < Q; ++l) D[l] = C::template dep<Q>(rv[l], l);
+    R dr = D[0], jx = 0, jy = 0, jz = 0;
     R pxx = 0, pyy = 0, pzz = 0, pxy = 0, pxz = 0, pyz = 0;
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
         const int l = 2 * p + 1;
         const int cx = Lt::cx(l), cy = Lt::cy(l), cz = Lt::cz(l);
-        const R sp = fv[l] + fv[l + 1];
-        const R dp = fv[l] - fv[l + 1];
-        rho += sp;
+        const R sp = D[l] + D[l + 1];
+        const R dp = D[l] - D[l + 1];
+        dr += sp;
         if (cx > 0) jx += dp; else if (cx < 0) jx -= dp;
         if (cy > 0) jy += dp; else if (cy < 0) jy -= dp;
         if (cz > 0) jz += dp; else if (cz < 0) jz -= dp;
         if (HRR || VRE) {
-            const R ss = C::template shifted<Q>(rv[l], l) + C::template shifted<Q>(rv[l + 1], l + 1);
-            dr += ss;
-            if (cx != 0) pxx += ss;
-            if (cy != 0) pyy += ss;
-            if (cz != 0) pzz += ss;
-            if (cx * cy > 0) pxy += ss; else if (cx * cy < 0) pxy -= ss;
-            if (cx * cz > 0) pxz += ss; else if (cx * cz < 0) pxz -= ss;
-            if (cy * cz > 0) pyz += ss; else if (cy * cz < 0) pyz -= ss;
+            if (cx != 0) pxx += sp;
+            if (cy != 0) pyy += sp;
+            if (cz != 0) pzz += sp;
+            if (cx * cy > 0) pxy += sp; else if (cx * cy < 0) pxy -= sp;
+            if (cx * cz > 0) pxz += sp; else if (cx * cz < 0) pxz -= sp;
+            if (cy * cz > 0) pyz += sp; else if (cy * cz < 0) pyz -= sp;
         }
     }
-    const R invrho = R(1) / rho;
-    R rh = rho, ux = jx * invrho, uy = jy * invrho, uz = jz * invrho;
+    dr *= invS;
+    R rh = R(1) + dr;
+    const R invrho = R(1) / rh;
+    R ux = (jx * invS) * invrho, uy = (jy * invS) * invrho, uz = (jz * invS) * invrho;
     if (a.ext_rho) { rh = a.ext_rho[oc]; ux = a.ext_u[oc]; uy = a.ext_v[oc]; uz = a.ext_w[oc]; }
     const R usq = ux * ux + uy * uy + uz * uz;
     const R base = R(1) - R(1.5) * usq;
+    const R rb1 = rh * base - R(1);      // rho*base - 1
+    const R r45 = R(4.5) * rh;
 
     // Non-equilibrium second-order tensor a2 = sum_l H2_l (f_l - feq_l)
     //   = Pi' - rho u u - (sum_l (f_l - w_l)) I/3   (Pi' = sum_l c c (f_l - w_l)),
@@ -146,12 +145,12 @@ __device__ __forceinline__ void collide_cell(const StepArgs<typename C::real>& a
     if (HRR || VRE) {
         const R third = R(1) / R(3);
         const R rux = rh * ux, ruy = rh * uy, ruz = rh * uz;
-        axx = pxx - rux * ux - dr * third;
-        ayy = pyy - ruy * uy - dr * third;
-        azz = pzz - ruz * uz - dr * third;
-        axy = pxy - rux * uy;
-        axz = pxz - rux * uz;
-        ayz = pyz - ruy * uz;
+        axx = pxx * invS - rux * ux - dr * third;
+        ayy = pyy * invS - ruy * uy - dr * third;
+        azz = pzz * invS - ruz * uz - dr * third;
+        axy = pxy * invS - rux * uy;
+        axz = pxz * invS - rux * uz;
+        ayz = pyz * invS - ruy * uz;
     }
 
     R fac = a.fac0;
@@ -173,28 +172,28 @@ __device__ __forceinline__ void collide_cell(const StepArgs<typename C::real>& a
         fac = R(1) - R(1) / tau;
     }
 
-    // Regularized non-equilibrium (src/m_postcoll_kernel.F90:189-244) in closed form:
-    //   fneq_l = 4.5 w_l [ cAc (1 + 3 cu) - T (1/3 + cu) - 2 c.g ],  T = tr a2,  g = a2 u
-    // even part 4.5 w (cAc - T/3), odd part 4.5 w (cu (3 cAc - T) - 2 c.g).
     const R T = axx + ayy + azz;
     const R gx = axx * ux + axy * uy + axz * uz;
     const R gy = axy * ux + ayy * uy + ayz * uz;
     const R gz = axz * ux + ayz * uy + azz * uz;
 
     {   // rest population
-        const R w0 = weight<Q, R>(0), wr0 = w0 * rh;
-        const R feq0 = wr0 * base;
-        const R neq0 = HRR ? -(R(1.5) * w0) * T : fv[0] - feq0;
-        out[0] = feq0 + fac * neq0;
+        const R wS = weight<Q, R>(0) * S;
+        const R e = wS * rb1;
+        const R ne = HRR ? -(R(1.5) * wS) * T : D[0] - e;
+        dev[0] = e + fac * ne;
     }
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
         const int l = 2 * p + 1;
         const int cx = Lt::cx(l), cy = Lt::cy(l), cz = Lt::cz(l);
-        const R w = weight<Q, R>(l);
+        const R wS = weight<Q, R>(l) * S;
         const R cu = sdot<R>(cx, cy, cz, ux, uy, uz);
-        R E, O;
-        feq_even_odd<BGK3, R>(w * rh, base, cu, usq, E, O);
+        const R cu2 = cu * cu;
+        R pol = R(3);
+        if (BGK3) pol += R(4.5) * (cu2 - usq);
+        const R e = wS * (rb1 + r45 * cu2);       // S (E - w)
+        const R o = wS * (rh * (cu * pol));       // S O
         if (HRR) {
             R cac = R(0);
             if (cx != 0) cac += axx;
@@ -202,56 +201,50 @@ __device__ __forceinline__ void collide_cell(const StepArgs<typename C::real>& a
             if (cz != 0) cac += azz;
             cac += R(2) * sdot<R>(cx * cy, cx * cz, cy * cz, axy, axz, ayz);
             const R cg = sdot<R>(cx, cy, cz, gx, gy, gz);
-            const R w45 = R(4.5) * w;
+            const R w45 = R(4.5) * wS;
             const R ne = w45 * (cac - T * (R(1) / R(3)));
             const R no = w45 * (cu * (R(3) * cac - T) - R(2) * cg);
-            const R ev = E + fac * ne, od = O + fac * no;
-            out[l] = ev + od;
-            out[l + 1] = ev - od;
+            const R ev = e + fac * ne, od = o + fac * no;
+            dev[l] = ev + od;
+            dev[l + 1] = ev - od;
         } else {
-            out[l] = (E + O) + fac * (fv[l] - (E + O));
-            out[l + 1] = (E - O) + fac * (fv[l + 1] - (E - O));
+            dev[l] = (e + o) + fac * (D[l] - (e + o));
+            dev[l + 1] = (e - o) + fac * (D[l + 1] - (e - o));
         }
     }
 
-    // Inflow-turbulence forcing on slice ip (src/m_inflow_turbulence_forcing.F90:57-128 and
-    // src/m_inflow_turbulence_apply.F90:21-26): f += feq(rho, u + ampl*uu) - feq(rho, u).
+    // Equilibrium increments feq(rho, v) - feq(rho, u) for the inflow-turbulence forcing on
+    // slice ip (src/m_inflow_turbulence_forcing.F90:57-128, added after the collision as
+    // src/m_inflow_turbulence_apply.F90:21-26; third order when ibgk = 3) and for the external
+    // body force (Kupershtokh exact difference with the 2nd-order equilibrium and du = -F/rho,
+    // src/m_forcings_apply_kernel.F90:142-271; a cell with rho <= 0 is skipped).
+    auto add_feq_diff = [&](R vx, R vy, R vz, auto order3) {
+        constexpr bool O3 = decltype(order3)::value;
+        const R vsq = vx * vx + vy * vy + vz * vz;
+        const R vrb1 = rh * (R(1) - R(1.5) * vsq) - R(1);
+        dev[0] = dev[0] + weight<Q, R>(0) * S * (vrb1 - rb1);
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            const int l = 2 * p + 1;
+            const R wS = weight<Q, R>(l) * S;
+            const R cv = sdot<R>(Lt::cx(l), Lt::cy(l), Lt::cz(l), vx, vy, vz);
+            const R cu = sdot<R>(Lt::cx(l), Lt::cy(l), Lt::cz(l), ux, uy, uz);
+            R pv = R(3), pu = R(3);
+            if (O3) { pv += R(4.5) * (cv * cv - vsq); pu += R(4.5) * (cu * cu - usq); }
+            const R de = wS * ((vrb1 + r45 * cv * cv) - (rb1 + r45 * cu * cu));
+            const R dodd = wS * (rh * (cv * pv) - rh * (cu * pu));
+            dev[l] = dev[l] + (de + dodd);
+            dev[l + 1] = dev[l + 1] + (de - dodd);
+        }
+    };
     if (a.tu != nullptr && i == a.ip) {
         const int sl = (k - 1) * L.ny + (j - 1);
-        const R vx = ux + a.ampl * a.tu[sl], vy = uy + a.ampl * a.tv[sl], vz = uz + a.ampl * a.tw[sl];
-        const R vsq = vx * vx + vy * vy + vz * vz;
-        const R vbase = R(1) - R(1.5) * vsq;
-        out[0] = out[0] + (weight<Q, R>(0) * rh * vbase - weight<Q, R>(0) * rh * base);
-#pragma unroll
-        for (int p = 0; p < NP; ++p) {
-            const int l = 2 * p + 1;
-            const R wr = weight<Q, R>(l) * rh;
-            R E1, O1, E0, O0;
-            feq_even_odd<BGK3, R>(wr, vbase, sdot<R>(Lt::cx(l), Lt::cy(l), Lt::cz(l), vx, vy, vz), vsq, E1, O1);
-            feq_even_odd<BGK3, R>(wr, base, sdot<R>(Lt::cx(l), Lt::cy(l), Lt::cz(l), ux, uy, uz), usq, E0, O0);
-            out[l] = out[l] + ((E1 + O1) - (E0 + O0));
-            out[l + 1] = out[l + 1] + ((E1 - O1) - (E0 - O0));
-        }
+        add_feq_diff(ux + a.ampl * a.tu[sl], uy + a.ampl * a.tv[sl], uz + a.ampl * a.tw[sl],
+                     std::integral_constant<bool, BGK3>());
     }
-
-    // External body force, Kupershtokh exact difference with the 2nd-order equilibrium and
-    // du = -F/rho (src/m_forcings_apply_kernel.F90:142-271); a cell with rho <= 0 is skipped.
-    if (a.Fx != nullptr && rh > R(0)) {
-        const R vx = ux + (-a.Fx[oc] / rh), vy = uy + (-a.Fy[oc] / rh), vz = uz + (-a.Fz[oc] / rh);
-        const R vsq = vx * vx + vy * vy + vz * vz;
-        const R vbase = R(1) - R(1.5) * vsq;
-        out[0] = out[0] + (weight<Q, R>(0) * rh * vbase - weight<Q, R>(0) * rh * base);
-#pragma unroll
-        for (int p = 0; p < NP; ++p) {
-            const int l = 2 * p + 1;
-            const R wr = weight<Q, R>(l) * rh;
-            R E1, O1, E0, O0;
-            feq_even_odd<false, R>(wr, vbase, sdot<R>(Lt::cx(l), Lt::cy(l), Lt::cz(l), vx, vy, vz), vsq, E1, O1);
-            feq_even_odd<false, R>(wr, base, sdot<R>(Lt::cx(l), Lt::cy(l), Lt::cz(l), ux, uy, uz), usq, E0, O0);
-            out[l] = out[l] + ((E1 + O1) - (E0 + O0));
-            out[l + 1] = out[l + 1] + ((E1 - O1) - (E0 - O0));
-        }
-    }
+    if (a.Fx != nullptr && rh > R(0))
+        add_feq_diff(ux + (-a.Fx[oc] / rh), uy + (-a.Fy[oc] / rh), uz + (-a.Fz[oc] / rh),
+                     std::integral_constant<bool, false>());
 
     // Solid cells: swap the opposite pairs (2,3),(4,5),... (src/m_solids.F90:19-32).
     if (a.solid != nullptr) {
@@ -260,9 +253,9 @@ __device__ __forceinline__ void collide_cell(const StepArgs<typename C::real>& a
         if ((word >> ((ca - 1) & 31)) & 1u) {
 #pragma unroll
             for (int l = 1; l + 1 < Q; l += 2) {
-                const R t = out[l];
-                out[l] = out[l + 1];
-                out[l + 1] = t;
+                const R t = dev[l];
+                dev[l] = dev[l + 1];
+                dev[l + 1] = t;
             }
         }
     }
@@ -313,8 +306,8 @@ __global__ void __launch_bounds__(1024) step_kernel(const StepArgs<typename C::r
 #pragma unroll
     for (int l = 0; l < Q; ++l) {
         raw* p = dst + (size_t)l * (size_t)L.plane + (size_t)oc;
-        if (VAR & 1) __builtin_nontemporal_store(C::template encode<Q>(out[l], l), p);
-        else *p = C::template encode<Q>(out[l], l);
+        if (VAR & 1) __builtin_nontemporal_store(C::template enc_dep<Q>(out[l], l), p);
+        else *p = C::template enc_dep<Q>(out[l], l);
     }
     if (VRE && a.tau_out != nullptr) a.tau_out[oc] = tau;
 }
@@ -411,7 +404,7 @@ __global__ void __launch_bounds__(256) step_kernel_pair(const StepArgs<typename 
 #pragma unroll
     for (int l = 0; l < Q; ++l) {
         W* p = reinterpret_cast<W*>(dst + (size_t)l * (size_t)L.plane + (size_t)oc);
-        const W w = PW::pack(C::template encode<Q>(outA[l], l), C::template encode<Q>(outB[l], l));
+        const W w = PW::pack(C::template enc_dep<Q>(outA[l], l), C::template enc_dep<Q>(outB[l], l));
         if (VAR & 1) __builtin_nontemporal_store(w, p);
         else *p = w;
     }
