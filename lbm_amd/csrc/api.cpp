@@ -52,6 +52,7 @@ struct lbm_ctx {
     int series_lo = 0, series_hi = -1;
     std::vector<double> ut_series, ud_series;
     double tau0 = 0.0, fac0 = 0.0;
+    int fold_dir = -1;             // closed axis whose face pass the step kernel performs
     int variant = 1;               // step-kernel variant (LBM_STEP_VARIANT), see impl.h; non-temporal
                                    // stores measured fastest on MI355X (5.32 vs 5.11 TB/s)
     // timing
@@ -254,7 +255,8 @@ hipEvent_t next_event(lbm_ctx* c, std::vector<hipEvent_t>& pool, size_t& used) {
 }
 
 // boundarycond(f1, f2) for the non-periodic directions (src/m_boundarycond.F90:166-365).
-int boundary(lbm_ctx* c, void* f1, const void* f2, int it) {
+// skip_dir: a closed axis whose face pass the step kernel has already performed.
+int boundary(lbm_ctx* c, void* f1, const void* f2, int it, int skip_dir = -1) {
     const lbm_desc& d = c->d;
     const Layout& L = c->lay;
     hipEvent_t e0 = nullptr;
@@ -269,11 +271,11 @@ int boundary(lbm_ctx* c, void* f1, const void* f2, int it) {
     if (c->snap_j.n) HIPCHK(c, c->ops->snapshot(L, f1, snap + half, c->snap_j, c->stream));
     int o1, oN;
     opts_of(d.ibnd, &o1, &oN);
-    if (o1) HIPCHK(c, c->ops->closed(L, f1, f2, 0, o1, oN, nullptr, none, c->stream));
+    if (o1 && skip_dir != 0) HIPCHK(c, c->ops->closed(L, f1, f2, 0, o1, oN, nullptr, none, c->stream));
     opts_of(d.jbnd, &o1, &oN);
-    if (o1) HIPCHK(c, c->ops->closed(L, f1, f2, 1, o1, oN, snap + half, c->snap_j, c->stream));
+    if (o1 && skip_dir != 1) HIPCHK(c, c->ops->closed(L, f1, f2, 1, o1, oN, snap + half, c->snap_j, c->stream));
     opts_of(d.kbnd, &o1, &oN);
-    if (o1) HIPCHK(c, c->ops->closed(L, f1, f2, 2, o1, oN, snap, c->snap_k, c->stream));
+    if (o1 && skip_dir != 2) HIPCHK(c, c->ops->closed(L, f1, f2, 2, o1, oN, snap, c->snap_k, c->stream));
     const EdgeList e = edge_list(d);
     if (e.n) HIPCHK(c, c->ops->edges(L, f1, f2, e, c->stream));
     if (d.ibnd == 1) {
@@ -351,6 +353,16 @@ int lbm_create(const lbm_desc* desc, lbm_ctx** out) {
     // storage format on MI355X (FP32 23.4k, FP16 38.3k MLUPS vs 22.1k / 35.7k with pairs).
     if (const char* v = getenv("LBM_STEP_VARIANT")) c->variant = atoi(v) & 5;
     if ((c->variant & 4) && (lay.n(lay.ax0) % 2 != 0 || ops->raw_bytes > 4)) c->variant &= ~4;
+    {
+        // The step kernel performs the face pass of the closed axis when it is the only closed
+        // one (no edges, corners or snapshots then; the inflow planes of i are written after).
+        const int codes[3] = {d.ibnd, d.jbnd, d.kbnd};
+        int nclosed = 0, cl = -1;
+        for (int a = 0; a < 3; ++a)
+            if (codes[a] > 10) { ++nclosed; cl = a; }
+        const char* nf = getenv("LBM_NO_FOLD");
+        if (nclosed == 1 && !(nf && atoi(nf))) c->fold_dir = cl;
+    }
     hipGetDevice(&c->device);
     c->lay = lay;
     c->tau0 = round_real(c, round_real(c, 3.0 * round_real(c, d.kinevisc)) + 0.5);
@@ -475,6 +487,12 @@ int lbm_step(lbm_ctx* c, int it0, int nsteps) {
     h.block_x = d.block_x;
     h.block_y = d.block_y;
     h.variant = c->variant;
+    h.fold_dir = c->fold_dir;
+    h.fold_opt_lo = h.fold_opt_hi = 0;
+    if (c->fold_dir >= 0) {
+        const int code = c->fold_dir == 0 ? d.ibnd : c->fold_dir == 1 ? d.jbnd : d.kbnd;
+        opts_of(code, &h.fold_opt_lo, &h.fold_opt_hi);
+    }
     const size_t slice = (size_t)d.ny * d.nz * c->ops->real_bytes;
     for (int it = it0 + 1; it <= it0 + nsteps; ++it) {
         if (c->pending) {
@@ -505,7 +523,7 @@ int lbm_step(lbm_ctx* c, int it0, int nsteps) {
         if (!c->pending) c->cur ^= 1;
         c->pending = false;
         c->use_ext = false;
-        if (int rc = boundary(c, c->buf[c->cur], c->buf[c->cur ^ 1], it)) return rc;
+        if (int rc = boundary(c, c->buf[c->cur], c->buf[c->cur ^ 1], it, c->fold_dir)) return rc;
         c->ghost_raw[c->cur] = false;
     }
     return LBM_OK;

@@ -45,6 +45,7 @@ struct StepHost {
     double ampl;
     const void* F[3];
     void* tau_out;
+    int fold_dir, fold_opt_lo, fold_opt_hi;   // closed face performed by the step kernel
     int bgk3, hrr, vre;
     int variant;      // step-kernel variant: bit 0 non-temporal stores, bit 2 two cells per lane
     int block_x, block_y;

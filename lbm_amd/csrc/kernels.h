@@ -50,6 +50,8 @@ struct StepArgs {
     const R* Fy;
     const R* Fz;
     R* tau_out;            // cell layout; nullptr: tau not stored
+    int fold_dir;          // closed axis whose face pass this kernel performs (-1: none)
+    int fold_opt_lo, fold_opt_hi;
 };
 
 // Signed sum  sx*a + sy*b + sz*c  for compile-time signs in {-1,0,1}: adds only.
@@ -74,6 +76,15 @@ __device__ __forceinline__ R feq_dir(int l, R rho, R ux, R uy, R uz, R usq) {
     R poly = R(3) + R(4.5) * cu;
     if (ORDER3 && l > 0) poly += R(4.5) * (cu * cu - usq);
     return (weight<Q, R>(l) * rho) * ((R(1) - R(1.5) * usq) + cu * poly);
+}
+
+// Reflected direction for a wall normal to axis d: normal component negated, tangential
+// components multiplied by opt (-1 no-slip, +1 free-slip).
+template <int Q>
+__host__ __device__ constexpr int mirror(int l, int d, int opt) {
+    using Lt = Lattice<Q>;
+    return Lt::find(d == 0 ? -Lt::cx(l) : opt * Lt::cx(l), d == 1 ? -Lt::cy(l) : opt * Lt::cy(l),
+                    d == 2 ? -Lt::cz(l) : opt * Lt::cz(l));
 }
 
 // The per-cell part of a step: moments, collision, turbulence and body-force increments, solid
@@ -262,6 +273,65 @@ __device__ __forceinline__ void collide_cell(const StepArgs<typename C::real>& a
     tau_out = tau;
 }
 
+// The closed-face pass of axis D (src/m_boundary_{i,j,k}_closed_kernel.F90) performed by the
+// step kernel for the cells next to the walls, when D is the only closed axis (the other axes
+// periodic, or i carrying the inflow/outflow planes, whose kernel runs after this one).  For
+// ghost cell G of the face and outgoing l the reference sets
+//   f1(m, G) = f2(l, G)                 (m = l reflected: normal negated, tangential * opt)
+//   f1(l, G) = f1(l, G - c_l tangentially, on the wall plane).
+// The cell x above G does the first; the second is pushed by the cell G - c_l that computed
+// the value (periodic tangential axes wrapped), or, when G - c_l is a stored ghost of the
+// inflow axis, copied by x from that ghost (still holding its value from before this step).
+template <int Q, class C>
+__device__ __forceinline__ void fold_closed_face(const StepArgs<typename C::real>& a, int i, int j, int k,
+                                                 const typename C::real (&dev)[Q]) {
+    using Lt = Lattice<Q>;
+    using raw = typename C::raw;
+    const Layout& L = a.lay;
+    const int D = a.fold_dir;
+    const int x[3] = {i, j, k};
+    const int n[3] = {L.nx, L.ny, L.nz};
+    const int per[3] = {L.per_i, L.per_j, L.per_k};
+    const raw* __restrict__ f2 = static_cast<const raw*>(a.src);
+    raw* __restrict__ f1 = static_cast<raw*>(a.dst);
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+        const int plane = side ? n[D] : 1;
+        if (x[D] != plane) continue;
+        const int ghost = side ? n[D] + 1 : 0;
+        const int opt = side ? a.fold_opt_hi : a.fold_opt_lo;
+        int g[3] = {i, j, k};
+        g[D] = ghost;
+        const int og = L.cell(g[0], g[1], g[2]);
+#pragma unroll
+        for (int l = 0; l < Q; ++l) {
+            const int cn = Lt::c(l, D);
+            if (!(side ? cn > 0 : cn < 0)) continue;
+            const int m = opt < 0 ? mirror<Q>(l, D, -1) : mirror<Q>(l, D, +1);
+            f1[(size_t)m * L.plane + og] = f2[(size_t)l * L.plane + og];
+            // push: the ghost this population streams to
+            int t[3], s[3];
+            bool push = true, from_ghost = false;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                if (d == D) { t[d] = ghost; s[d] = plane; continue; }
+                const int cd = Lt::c(l, d);
+                t[d] = x[d] + cd;
+                s[d] = x[d] - cd;
+                if (per[d]) {
+                    t[d] = Layout::wrap(t[d], n[d], 1);
+                    s[d] = Layout::wrap(s[d], n[d], 1);
+                } else {
+                    if (t[d] < 1 || t[d] > n[d]) push = false;
+                    if (s[d] < 1 || s[d] > n[d]) from_ghost = true;
+                }
+            }
+            if (push) f1[(size_t)l * L.plane + L.cell(t[0], t[1], t[2])] = C::template enc_dep<Q>(dev[l], l);
+            if (from_ghost) f1[(size_t)l * L.plane + og] = f1[(size_t)l * L.plane + L.cell(s[0], s[1], s[2])];
+        }
+    }
+}
+
 // VAR bit 0: non-temporal stores, bit 1: non-temporal loads (measured slower; kept for tuning).
 template <int Q, class C, bool BGK3, bool HRR, bool VRE, int VAR>
 __global__ void __launch_bounds__(1024) step_kernel(const StepArgs<typename C::real> a) {
@@ -310,6 +380,7 @@ __global__ void __launch_bounds__(1024) step_kernel(const StepArgs<typename C::r
         else *p = C::template enc_dep<Q>(out[l], l);
     }
     if (VRE && a.tau_out != nullptr) a.tau_out[oc] = tau;
+    if (a.fold_dir >= 0) fold_closed_face<Q, C>(a, i, j, k, out);
 }
 
 // Two raw values packed in one aligned word: 32 bits for 16-bit storage, 64 bits for FP32.
@@ -435,14 +506,6 @@ __device__ __forceinline__ void cp(const Layout& L, void* fdst, int ldst, int od
         static_cast<const raw*>(fsrc)[(size_t)lsrc * (size_t)L.plane + (size_t)osrc];
 }
 
-// Reflected direction for a wall normal to axis d: normal component negated, tangential
-// components multiplied by opt (-1 no-slip, +1 free-slip).
-template <int Q>
-__host__ __device__ constexpr int mirror(int l, int d, int opt) {
-    using Lt = Lattice<Q>;
-    return Lt::find(d == 0 ? -Lt::cx(l) : opt * Lt::cx(l), d == 1 ? -Lt::cy(l) : opt * Lt::cy(l),
-                    d == 2 ? -Lt::cz(l) : opt * Lt::cz(l));
-}
 
 // Closed faces normal to axis DIR (src/m_boundary_i_closed_kernel.F90:44-58, _j_:103-114,
 // _k_:157-168).  One thread per tangential position handles both faces: it first loads every

@@ -32,7 +32,7 @@ struct Launch {
     static hipError_t step_t(const Layout& L, const StepArgs<R>& a, dim3 grid, dim3 block, int var, hipStream_t s) {
         const bool nt = var & 1;
         if constexpr (sizeof(typename C::raw) <= 4) {
-            if (var & 4) {
+            if ((var & 4) && a.fold_dir < 0) {
                 // two cells per lane along the contiguous axis
                 const long long nt_slab = (long long)(L.n(L.ax0) / 2) * L.n(L.ax1);
                 const dim3 g((unsigned)((nt_slab + 255) / 256), L.n(L.ax2), 1);
@@ -79,6 +79,9 @@ struct Launch {
         a.Fy = static_cast<const R*>(h.F[1]);
         a.Fz = static_cast<const R*>(h.F[2]);
         a.tau_out = static_cast<R*>(h.tau_out);
+        a.fold_dir = h.fold_dir;
+        a.fold_opt_lo = h.fold_opt_lo;
+        a.fold_opt_hi = h.fold_opt_hi;
         const dim3 block(h.block_x, h.block_y, 1);
         const dim3 grid((L.n(L.ax0) + h.block_x - 1) / h.block_x, (L.n(L.ax1) + h.block_y - 1) / h.block_y, L.n(L.ax2));
         const int key = (h.bgk3 ? 4 : 0) | (h.hrr ? 2 : 0) | (h.vre ? 1 : 0);
