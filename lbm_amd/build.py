@@ -4,7 +4,8 @@
 
 Every HIP translation unit is compiled with -Xclang -target-feature -Xclang -packed-fp32-ops
 (no packed FP32 instructions).  The objects go to lbm_amd/_build/, the library to
-lbm_amd/liblbm_hip.so.
+lbm_amd/liblbm_hip.so.  When amdflang is present the Fortran binding module and driver
+(lbm_amd/fortran) are built too: _build/lbm_driver_f32 and _build/lbm_driver_f64.
 """
 from __future__ import annotations
 
@@ -17,6 +18,8 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "liblbm_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FLANG = os.environ.get("FLANG", "/opt/rocm/bin/amdflang")
+FORTRAN = os.path.join(HERE, "fortran")
 ARCH = "gfx950"
 
 SOURCES = ["kernels_f32.hip", "kernels_f64.hip", "kernels_f16s.hip", "kernels_bf16s.hip", "api.cpp"]
@@ -54,7 +57,27 @@ def build(verbose: bool = False) -> str:
     cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    if os.path.exists(FLANG):
+        build_fortran(verbose)
     return LIB
+
+
+def build_fortran(verbose: bool = False) -> list:
+    """The bind(C) module and the driver, FP32 and FP64 (-DDOUBLE_PRECISION), linked against
+    liblbm_hip.so with an rpath to lbm_amd/."""
+    out = []
+    for prec, defs in (("f32", []), ("f64", ["-DDOUBLE_PRECISION"])):
+        mdir = os.path.join(BUILD, "fmod_" + prec)
+        os.makedirs(mdir, exist_ok=True)
+        exe = os.path.join(BUILD, "lbm_driver_" + prec)
+        cmd = [FLANG, "-cpp", "-O2"] + defs + ["-module-dir", mdir,
+               os.path.join(FORTRAN, "lbm_hip_mod.F90"), os.path.join(FORTRAN, "lbm_driver.F90"),
+               "-L", HERE, "-llbm_hip", "-Wl,-rpath,$ORIGIN/..", "-o", exe]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True, cwd=mdir)
+        out.append(exe)
+    return out
 
 
 if __name__ == "__main__":

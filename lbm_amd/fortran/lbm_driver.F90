@@ -1,0 +1,356 @@
+! Driver program for the MI355X lattice-update library, keeping the reference driver's inputs
+! and files: infile.in in the line order of src/m_readinfile.F90:84-160, restart files
+! restart/restart_<tile4>_<it6>.uf (src/m_saverestart.F90:125-136) and
+! restart/turbulence_<tile4>_<it6>.uf, the end-of-run testing file (src/m_testing.F90), and the
+! flow of src/main.F90: read inputs, restart or equilibrium start, time loop, final restart.
+!
+! The grid comes from the restart header when nt0 > 0 (the reference fixes it at compile time in
+! mod_dimensions); for nt0 = 0 give it on the command line:  lbm_driver nx ny nz [nl].
+! Not carried over (outside the lattice update): turbines, buoyancy/ABL, tracers, Tecplot and
+! NetCDF output, averaging, the FFTW turbulence generator (turbulence fields come from the
+! turbulence restart file), the random perturbation of inipert (nt0 = 0 starts from the
+! inflow profile).  Geometry: 'none' or the reference's 'cylinder' (src/m_cylinder.F90).
+program lbm_driver
+   use, intrinsic :: iso_c_binding
+   use lbm_hip
+   implicit none
+#ifdef DOUBLE_PRECISION
+   integer, parameter :: rk = c_double
+#else
+   integer, parameter :: rk = c_float
+#endif
+   integer(c_int), parameter :: realkind = rk
+
+   ! infile.in
+   logical :: ltiming, ltesting, ldump, lmeasurements, inflowturbulence, laveraging, laveturb
+   integer :: ntx, nty, ntz, ibgk, ihrr, ivreman, iablvisc, istable, nt0, nt1, iout, irestart
+   integer :: iprt1, iprt2, dprt, itecout, ibnd, jbnd, kbnd, nrturb, avestart, avesave, nturbines
+   real(rk) :: smagorinsky, ablheight, uini, udir, turbulence_ampl, visc, p_rho, p_len, p_vel
+   real(rk) :: p_time, p_visc, kinevisc
+   character(len=20) :: experiment
+   character(len=1) :: ver
+
+   integer :: nx, ny, nz, nl, i, j, k, it, iunit, nchunk, nargs, ios
+   character(len=32) :: arg
+   character(len=100) :: fname
+   real(rk), allocatable, target :: f(:,:,:,:), rho(:,:,:), u(:,:,:), v(:,:,:), w(:,:,:)
+   real(rk), allocatable, target :: uu(:,:,:), vv(:,:,:), ww(:,:,:), rr(:,:,:)
+   real(rk), allocatable, target :: uvel_shear(:), uvel_time(:), udir_time(:)
+   integer(c_int8_t), allocatable, target :: mask(:,:,:)
+   logical :: ex, lsolids
+   type(lbm_desc) :: desc
+   type(c_ptr) :: ctx
+   integer(c_int) :: rc
+   integer(8) :: t0, t1, rate
+   real(8) :: wall, mass, mom(3)
+
+   ! ---------------------------------------------------------------- infile.in
+   inquire(file='infile.in', exist=ex)
+   if (.not. ex) stop 'lbm_driver: infile.in not found'
+   open(newunit=iunit, file='infile.in', status='old')
+   read(iunit,'(a)') ver
+   read(iunit,*) ltiming
+   read(iunit,*) ltesting
+   read(iunit,*) ldump
+   read(iunit,*) ntx, nty, ntz
+   read(iunit,*) lmeasurements
+   read(iunit,'(a)') ver
+   read(iunit,*) experiment
+   read(iunit,*) ibgk
+   read(iunit,*) ihrr
+   read(iunit,*) ivreman, smagorinsky
+   read(iunit,'(a)') ver
+   read(iunit,*) iablvisc
+   read(iunit,*) ablheight
+   read(iunit,*) istable
+   read(iunit,'(a)') ver
+   read(iunit,*) nt0
+   read(iunit,*) nt1
+   if (nt1 <= nt0) stop 'lbm_driver: nt1 <= nt0'
+   read(iunit,*) iout
+   read(iunit,*) irestart
+   read(iunit,*) iprt1, iprt2, dprt
+   read(iunit,*) itecout
+   read(iunit,'(a)') ver
+   read(iunit,*) ibnd
+   read(iunit,*) jbnd
+   read(iunit,*) kbnd
+   read(iunit,'(a)') ver
+   read(iunit,*) uini, udir
+   read(iunit,*) inflowturbulence, turbulence_ampl, nrturb
+   read(iunit,'(a)') ver
+   read(iunit,*) visc
+   read(iunit,*) p_rho
+   read(iunit,*) p_len
+   read(iunit,*) p_vel
+   read(iunit,'(a)') ver
+   read(iunit,'(1x,l1,1x,l1)') laveraging, laveturb
+   read(iunit,*) avestart
+   read(iunit,*) avesave
+   read(iunit,'(a)') ver
+   read(iunit,*) nturbines
+   close(iunit)
+   if (nturbines > 0) stop 'lbm_driver: turbines are not part of this build'
+   if (iablvisc == 2) stop 'lbm_driver: the buoyancy scheme is not part of this build'
+   ! conversions as src/m_readinfile.F90:133,171-186
+   uini = uini / p_vel
+   p_time = p_len / p_vel
+   p_visc = p_len**2 / p_time
+   kinevisc = visc / p_visc
+
+   ! ---------------------------------------------------------------- grid and state
+   nl = 27
+   if (nt0 > 0) then
+      write(fname,'(a,i6.6,a)') 'restart/restart_0000_', nt0, '.uf'
+      inquire(file=trim(fname), exist=ex)
+      if (.not. ex) then
+         print '(2a)', 'lbm_driver: restart file does not exist: ', trim(fname)
+         stop 1
+      endif
+      open(newunit=iunit, file=trim(fname), form='unformatted', status='old')
+      read(iunit) nx, ny, nz, nl
+      close(iunit)
+   else
+      nargs = command_argument_count()
+      if (nargs < 3) stop 'lbm_driver: nt0 = 0 needs the grid on the command line: nx ny nz [nl]'
+      call get_command_argument(1, arg); read(arg,*) nx
+      call get_command_argument(2, arg); read(arg,*) ny
+      call get_command_argument(3, arg); read(arg,*) nz
+      if (nargs >= 4) then
+         call get_command_argument(4, arg); read(arg,*) nl
+      endif
+   endif
+   print '(a,3i6,a,i3)', 'lbm_driver: grid ', nx, ny, nz, '  nl=', nl
+
+   desc%nx = nx; desc%ny = ny; desc%nz = nz; desc%q = nl
+   desc%storage = merge(LBM_STORE_F64, LBM_STORE_F32, rk == c_double)
+   desc%ibgk = ibgk; desc%ihrr = ihrr; desc%ivreman = ivreman
+   desc%ibnd = ibnd; desc%jbnd = jbnd; desc%kbnd = kbnd
+   desc%iturb_pos = 10; desc%store_tau = 0; desc%block_x = 0; desc%block_y = 0
+   desc%kinevisc = real(kinevisc, c_double)
+   desc%smagorinsky = real(smagorinsky, c_double)
+   desc%rho0 = 1.0d0
+   rc = lbm_create(desc, ctx)
+   call lbm_check(c_null_ptr, rc, 'lbm_create')
+
+   allocate(f(nl,0:nx+1,0:ny+1,0:nz+1))
+   allocate(rho(0:nx+1,0:ny+1,0:nz+1), u(0:nx+1,0:ny+1,0:nz+1), v(0:nx+1,0:ny+1,0:nz+1), w(0:nx+1,0:ny+1,0:nz+1))
+
+   ! solids (src/m_solid_objects_init.F90:43-58)
+   allocate(mask(0:nx+1,0:ny+1,0:nz+1))
+   mask = 0
+   lsolids = .false.
+   select case (trim(experiment))
+   case ('cylinder')
+      ! radius 7 at i = 50, j = ny/2, every k (src/m_cylinder.F90:8-17)
+      do j = 1, ny
+         do i = 1, nx
+            if ((i - 50)**2 + (j - ny/2)**2 < 49) mask(i, j, :) = 1
+         enddo
+      enddo
+      lsolids = .true.
+   case ('none', '')
+   case default
+      print '(2a)', 'lbm_driver: geometry not part of this build: ', trim(experiment)
+      stop 1
+   end select
+   if (lsolids) then
+      rc = lbm_set_blanking(ctx, c_loc(mask))
+      call lbm_check(ctx, rc, 'lbm_set_blanking')
+   endif
+
+   ! inflow profile and time series (src/m_inflow.F90)
+   allocate(uvel_shear(nz), uvel_time(nt0:nt1), udir_time(nt0:nt1))
+   call read_inflow()
+   rc = lbm_set_inflow(ctx, c_loc(uvel_shear), real(uini, c_double), real(udir, c_double), realkind)
+   call lbm_check(ctx, rc, 'lbm_set_inflow')
+   rc = lbm_set_inflow_series(ctx, nt0, nt1, c_loc(uvel_time), c_loc(udir_time), realkind)
+   call lbm_check(ctx, rc, 'lbm_set_inflow_series')
+
+   if (nt0 == 0) then
+      ! equilibrium start from the inflow profile (src/main.F90:214,243-247; no random part)
+      rho = 1.0_rk
+      do k = 0, nz + 1
+         u(:,:,k) = uvel_time(nt0) * uvel_shear(min(max(k,1),nz)) * cos(udir_time(nt0) * 3.1415927410125732_rk / 180.0_rk)
+         v(:,:,k) = uvel_time(nt0) * uvel_shear(min(max(k,1),nz)) * sin(udir_time(nt0) * 3.1415927410125732_rk / 180.0_rk)
+      enddo
+      w = 0.0_rk
+      rc = lbm_init_equilibrium(ctx, c_loc(rho), c_loc(u), c_loc(v), c_loc(w), realkind, nt0)
+      call lbm_check(ctx, rc, 'lbm_init_equilibrium')
+   else
+      ! restart (src/m_readrestart.F90:163-185)
+      write(fname,'(a,i6.6,a)') 'restart/restart_0000_', nt0, '.uf'
+      open(newunit=iunit, file=trim(fname), form='unformatted', status='old')
+      read(iunit) i, j, k, nl, f
+      close(iunit)
+      rc = lbm_set_state(ctx, c_loc(f), realkind)
+      call lbm_check(ctx, rc, 'lbm_set_state')
+   endif
+
+   if (inflowturbulence) then
+      ! turbulence fields from restart/turbulence_0000_<nt0>.uf (src/m_readrestart.F90:66-100)
+      allocate(uu(ny,nz,0:nrturb), vv(ny,nz,0:nrturb), ww(ny,nz,0:nrturb), rr(ny,nz,0:nrturb))
+      write(fname,'(a,i6.6,a)') 'restart/turbulence_0000_', nt0, '.uf'
+      inquire(file=trim(fname), exist=ex)
+      if (.not. ex) then
+         print '(2a)', 'lbm_driver: no turbulence fields: ', trim(fname)
+         stop 1
+      endif
+      open(newunit=iunit, file=trim(fname), form='unformatted', status='old')
+      read(iunit) j, k, i
+      rewind(iunit)
+      if (j /= ny .or. k /= nz .or. i /= nrturb) stop 'lbm_driver: incompatible turbulence restart file'
+      read(iunit) j, k, i, uu, vv, ww, rr
+      close(iunit)
+      rc = lbm_set_turbulence(ctx, c_loc(uu), c_loc(vv), c_loc(ww), int(nrturb, c_int), &
+                              real(turbulence_ampl, c_double), realkind)
+      call lbm_check(ctx, rc, 'lbm_set_turbulence')
+   endif
+
+   ! ---------------------------------------------------------------- time loop
+   print '(a)', ' Start timestepping loop'
+   call system_clock(t0, rate)
+   it = nt0
+   do while (it < nt1)
+      nchunk = min(10 - mod(it, 10), nt1 - it)
+      if (ldump .and. irestart > 0) nchunk = min(nchunk, irestart - mod(it, irestart))
+      rc = lbm_step(ctx, int(it, c_int), int(nchunk, c_int))
+      call lbm_check(ctx, rc, 'lbm_step')
+      it = it + nchunk
+      if (mod(it, 10) == 0 .or. it == nt1) print '(a,i6,a,f10.2,a)', 'Iteration: ', it, ' Time: ', real(it)*p_time, ' s.'
+      if (ldump .and. irestart > 0 .and. mod(it, irestart) == 0 .and. it /= nt1) call save_restart(it)
+   enddo
+   rc = lbm_synchronize(ctx)
+   call lbm_check(ctx, rc, 'lbm_synchronize')
+   call system_clock(t1)
+   wall = real(t1 - t0, 8) / real(rate, 8)
+   rc = lbm_get_totals(ctx, mass, mom)
+   call lbm_check(ctx, rc, 'lbm_get_totals')
+   print '(a,es16.8,a,3es14.6)', 'mass =', mass, '  momentum =', mom
+   if (ltiming) then
+      print '(a,1x,f13.5)', 'Total TS routines  time =', wall
+      print '(a,1x,f13.2)', 'MLUPS                   =', real(nx,8)*ny*nz*(nt1 - nt0) / wall / 1.0d6
+   endif
+
+   if (ldump) call save_restart(nt1)
+   call testing(nt1)
+   rc = lbm_destroy(ctx)
+
+contains
+
+   subroutine save_restart(itw)
+      ! src/m_saverestart.F90:125-136 (and the turbulence fields, :66-83)
+      integer, intent(in) :: itw
+      integer :: iu
+      character(len=100) :: fn
+      rc = lbm_get_state(ctx, c_loc(f), realkind)
+      call lbm_check(ctx, rc, 'lbm_get_state')
+      call execute_command_line('mkdir -p restart')
+      write(fn,'(a,i6.6,a)') 'restart/restart_0000_', itw, '.uf'
+      open(newunit=iu, file=trim(fn), form='unformatted', status='replace')
+      write(iu) nx, ny, nz, nl, f
+      close(iu)
+      if (inflowturbulence) then
+         write(fn,'(a,i6.6,a)') 'restart/turbulence_0000_', itw, '.uf'
+         open(newunit=iu, file=trim(fn), form='unformatted', status='replace')
+         write(iu) ny, nz, nrturb, uu, vv, ww, rr
+         close(iu)
+      endif
+   end subroutine
+
+   subroutine testing(itw)
+      ! src/m_testing.F90: FP32 state with ghosts; compare when the file exists
+      integer, intent(in) :: itw
+      integer :: iu
+      character(len=100) :: fn
+      real(kind=4), allocatable :: fh(:,:,:,:)
+      real(kind=4) :: fsum, fmax, diff
+      integer :: l2, i2, j2, k2
+      if (.not. ltesting) return
+      rc = lbm_get_state(ctx, c_loc(f), realkind)
+      call lbm_check(ctx, rc, 'lbm_get_state')
+      call execute_command_line('mkdir -p testing')
+      write(fn,'(a,i6.6,a)') 'testing/testing_0000_', itw, '.uf'
+      allocate(fh(nl,0:nx+1,0:ny+1,0:nz+1))
+      inquire(file=trim(fn), exist=ex)
+      if (ex) then
+         open(newunit=iu, file=trim(fn), form='unformatted', status='old')
+         read(iu) fh
+         close(iu)
+         fsum = 0.0; fmax = 0.0
+         do k2 = 1, nz
+         do j2 = 1, ny
+         do i2 = 1, nx
+         do l2 = 1, nl
+            diff = abs(real(f(l2,i2,j2,k2), 4) - fh(l2,i2,j2,k2))
+            fsum = fsum + diff
+            fmax = max(diff, fmax)
+         enddo
+         enddo
+         enddo
+         enddo
+         fsum = fsum / real(nl*(nx+2)*(ny+2)*(nz+2))
+         print '(a,i4,a,g12.5,a,g12.5)', 'Total misfit ', 0, ': Mean abs error=', fsum, ' max error=', fmax
+      else
+         fh = real(f, 4)
+         open(newunit=iu, file=trim(fn), form='unformatted', status='replace')
+         print '(a,a)', 'Writing new:', trim(fn)
+         write(iu) fh
+         close(iu)
+      endif
+   end subroutine
+
+   subroutine read_inflow()
+      ! uvel_shear.dat and uvel_time.dat as src/m_inflow.F90:27-86 reads them
+      real(rk) :: uh(nz), z, tmp, t
+      real(rk), allocatable :: td(:), ut(:), ud(:)
+      integer :: kk, n, m, iu2
+      uh = 1.0_rk
+      inquire(file='uvel_shear.dat', exist=ex)
+      if (ex) then
+         open(newunit=iu2, file='uvel_shear.dat')
+         do kk = 1, nz
+            read(iu2,*) n, z, uh(kk)
+         enddo
+         close(iu2)
+      endif
+      uvel_shear = uh / uh(nz)
+      uvel_time = uini
+      udir_time = udir
+      inquire(file='uvel_time.dat', exist=ex)
+      if (ex) then
+         open(newunit=iu2, file='uvel_time.dat')
+         n = 0
+         do
+            read(iu2,*,iostat=ios) tmp
+            if (ios /= 0) exit
+            n = n + 1
+         enddo
+         rewind(iu2)
+         allocate(td(n), ut(n), ud(n))
+         do kk = 1, n
+            read(iu2,*) td(kk), ut(kk), ud(kk)
+            ut(kk) = ut(kk) / p_vel
+         enddo
+         close(iu2)
+         do kk = nt0, nt1
+            t = real(kk - 1) * p_time
+            if (t <= td(1)) then
+               uvel_time(kk) = ut(1); udir_time(kk) = ud(1)
+            else if (t >= td(n)) then
+               uvel_time(kk) = ut(n); udir_time(kk) = ud(n)
+            else
+               do m = 1, n - 1
+                  if (t >= td(m) .and. t <= td(m+1)) then
+                     tmp = (t - td(m)) / (td(m+1) - td(m))
+                     uvel_time(kk) = ut(m) + tmp * (ut(m+1) - ut(m))
+                     udir_time(kk) = ud(m) + tmp * (ud(m+1) - ud(m))
+                     exit
+                  endif
+               enddo
+            endif
+         enddo
+      endif
+   end subroutine
+
+end program

@@ -1,0 +1,149 @@
+! Fortran binding of the lattice-update library (include/lbm_hip.h) through iso_c_binding.
+!
+! The reference's time loop (src/main.F90:290-329) calls postcoll, the inflow-turbulence
+! forcing, forcings_apply, solids, boundarycond, drift and macrovars on device arrays it owns.
+! With this module the driver creates one context per domain and calls lbm_step; state moves in
+! and out in the reference's layout f(nl,0:nx+1,0:ny+1,0:nz+1).  Every function returns an
+! integer status (0 = success); lbm_check stops with the library's message otherwise, the
+! reference's convention for fatal paths.
+module lbm_hip
+   use, intrinsic :: iso_c_binding
+   implicit none
+   private
+
+   integer(c_int), parameter, public :: LBM_STORE_F32 = 0, LBM_STORE_F64 = 1, &
+                                        LBM_STORE_F16S = 2, LBM_STORE_BF16S = 3
+
+   type, bind(C), public :: lbm_desc
+      integer(c_int32_t) :: nx, ny, nz, q, storage
+      integer(c_int32_t) :: ibgk, ihrr, ivreman
+      integer(c_int32_t) :: ibnd, jbnd, kbnd
+      integer(c_int32_t) :: iturb_pos, store_tau, block_x, block_y
+      real(c_double)     :: kinevisc, smagorinsky, rho0
+   end type
+
+   public :: lbm_create, lbm_destroy, lbm_last_error, lbm_set_state, lbm_get_state, lbm_get_macros, &
+             lbm_init_equilibrium, lbm_set_blanking, lbm_set_ablvisc, lbm_set_inflow, &
+             lbm_set_inflow_series, lbm_set_turbulence, lbm_set_external_forcing, lbm_step, &
+             lbm_synchronize, lbm_get_totals, lbm_check, lbm_error_message
+
+   interface
+      integer(c_int) function lbm_create(desc, ctx) bind(C, name='lbm_create')
+         import :: c_int, c_ptr, lbm_desc
+         type(lbm_desc), intent(in) :: desc
+         type(c_ptr), intent(out) :: ctx
+      end function
+      integer(c_int) function lbm_destroy(ctx) bind(C, name='lbm_destroy')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
+      type(c_ptr) function lbm_last_error(ctx) bind(C, name='lbm_last_error')
+         import :: c_ptr
+         type(c_ptr), value :: ctx
+      end function
+      integer(c_int) function lbm_set_state(ctx, f, realkind) bind(C, name='lbm_set_state')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+         type(c_ptr), value :: f
+         integer(c_int), value :: realkind
+      end function
+      integer(c_int) function lbm_get_state(ctx, f, realkind) bind(C, name='lbm_get_state')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+         type(c_ptr), value :: f
+         integer(c_int), value :: realkind
+      end function
+      integer(c_int) function lbm_get_macros(ctx, rho, u, v, w, realkind) bind(C, name='lbm_get_macros')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx, rho, u, v, w
+         integer(c_int), value :: realkind
+      end function
+      integer(c_int) function lbm_init_equilibrium(ctx, rho, u, v, w, realkind, it0) bind(C, name='lbm_init_equilibrium')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx, rho, u, v, w
+         integer(c_int), value :: realkind, it0
+      end function
+      integer(c_int) function lbm_set_blanking(ctx, mask) bind(C, name='lbm_set_blanking')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx, mask
+      end function
+      integer(c_int) function lbm_set_ablvisc(ctx, abl, realkind) bind(C, name='lbm_set_ablvisc')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx, abl
+         integer(c_int), value :: realkind
+      end function
+      integer(c_int) function lbm_set_inflow(ctx, shear, uini, udir, realkind) bind(C, name='lbm_set_inflow')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx, shear
+         real(c_double), value :: uini, udir
+         integer(c_int), value :: realkind
+      end function
+      integer(c_int) function lbm_set_inflow_series(ctx, it_lo, it_hi, ut, ud, realkind) bind(C, name='lbm_set_inflow_series')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: it_lo, it_hi
+         type(c_ptr), value :: ut, ud
+         integer(c_int), value :: realkind
+      end function
+      integer(c_int) function lbm_set_turbulence(ctx, uu, vv, ww, nrturb, ampl, realkind) bind(C, name='lbm_set_turbulence')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx, uu, vv, ww
+         integer(c_int), value :: nrturb
+         real(c_double), value :: ampl
+         integer(c_int), value :: realkind
+      end function
+      integer(c_int) function lbm_set_external_forcing(ctx, force, realkind) bind(C, name='lbm_set_external_forcing')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx, force
+         integer(c_int), value :: realkind
+      end function
+      integer(c_int) function lbm_step(ctx, it0, nsteps) bind(C, name='lbm_step')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: it0, nsteps
+      end function
+      integer(c_int) function lbm_synchronize(ctx) bind(C, name='lbm_synchronize')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
+      integer(c_int) function lbm_get_totals(ctx, mass, mom) bind(C, name='lbm_get_totals')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(out) :: mass
+         real(c_double), intent(out) :: mom(3)
+      end function
+      integer(c_size_t) function c_strlen(s) bind(C, name='strlen')
+         import :: c_size_t, c_ptr
+         type(c_ptr), value :: s
+      end function
+   end interface
+
+contains
+
+   ! The library's message for ctx (c_null_ptr: the last failed lbm_create).
+   function lbm_error_message(ctx) result(msg)
+      type(c_ptr), intent(in) :: ctx
+      character(len=:), allocatable :: msg
+      type(c_ptr) :: p
+      character(kind=c_char), pointer :: chars(:)
+      integer :: n, i
+      p = lbm_last_error(ctx)
+      n = int(c_strlen(p))
+      call c_f_pointer(p, chars, [n])
+      allocate(character(len=n) :: msg)
+      do i = 1, n
+         msg(i:i) = chars(i)
+      enddo
+   end function
+
+   subroutine lbm_check(ctx, rc, what)
+      type(c_ptr), intent(in) :: ctx
+      integer(c_int), intent(in) :: rc
+      character(len=*), intent(in) :: what
+      if (rc /= 0) then
+         print '(a,a,i3,a,a)', what, ' failed, status', rc, ': ', lbm_error_message(ctx)
+         stop 1
+      endif
+   end subroutine
+
+end module
