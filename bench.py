@@ -13,6 +13,13 @@ kernels) over the grid.  Inputs are synthetic and resident in HBM before timing 
     python bench.py [--steps K] [--warmup W] [--storage f32|f16s|bf16s|f64] [--no-cpu-baseline]
 
 Prints ONE JSON line.  --gpus N > 1 is refused: this build targets one GPU.
+
+    python bench.py --workload ensemble [--members 64] [--steps 500]
+
+times the batched ensemble instead: 64 D3Q19 regularized + Vreman members with sizes drawn by
+seed from {96,128,160}x{64,96}x{64,96}, a seeded box obstacle (side 8-16) and seeded inflow
+forcing each, FP32, graph-captured batched steps with the autotuned launch shape, against the
+same members stepped with one launch sequence per member.
 """
 from __future__ import annotations
 
@@ -105,6 +112,74 @@ def cpu_baseline(w, f0, turb, sample_steps, threads):
                       f"'Total TS routines' timer {secs:.2f} s"}
 
 
+def bench_ensemble(a):
+    from lbm_amd import inputs, workloads
+    from lbm_amd.domain import Ensemble
+    specs = inputs.ensemble_specs(4, a.members)
+    cells = workloads.cells(specs)
+    print(f"[bench] building {a.members} ensemble members, {cells / 1e6:.1f} M cells", file=sys.stderr, flush=True)
+    doms = workloads.ensemble_domains(specs, a.storage, nrturb=100)
+    ens = Ensemble(doms)
+    it = 0
+    ens.step(it, a.warmup)
+    it += a.warmup
+    ens.synchronize()
+    print("[bench] autotuning the launch shape", file=sys.stderr, flush=True)
+    block, bpc, tune_ms = ens.tune(it, trial_steps=5)
+    print(f"[bench] tuned: block {block}, blocks/CU {bpc}, {tune_ms:.3f} ms/step", file=sys.stderr, flush=True)
+    # one launch sequence per member and step (the members' own contexts)
+    n_pm = max(10, a.steps // 5)
+    for d in doms:
+        d.synchronize()
+    t0 = time.perf_counter()
+    for s_ in range(n_pm):
+        for d in doms:
+            d.step(it + s_, 1)
+    for d in doms:
+        d.synchronize()
+    t_pm = time.perf_counter() - t0
+    it += n_pm
+    # batched
+    print(f"[bench] timed {a.steps} batched steps", file=sys.stderr, flush=True)
+    ens.set_timing(True)
+    ens.synchronize()
+    t0 = time.perf_counter()
+    ens.step(it, a.steps)
+    ens.synchronize()
+    secs = time.perf_counter() - t0
+    ev_ms, ev_steps = ens.timing_read()
+    finite = all(bool(np.isfinite(d.totals()[0])) for d in doms)
+    ens.close()
+    for d in doms:
+        d.close()
+    mlups = cells * a.steps / secs / 1e6
+    mlups_pm = cells * n_pm / t_pm / 1e6
+    step_ms = ev_ms / max(ev_steps, 1)
+    bpc_bytes = bytes_per_cell(19, a.storage)
+    achieved = bpc_bytes * cells / (step_ms * 1e-3) / 1e9
+    line = {
+        "metric": "MLUPS (batched ensemble, D3Q19 reg+Vreman)", "value": round(mlups, 1), "unit": "MLUPS",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(secs / a.steps * 1e3, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": {"f32": "f32", "f64": "f64", "f16s": "f16-store/f32", "bf16s": "bf16-store/f32"}[a.storage],
+        "data": "synthetic (seeded sizes, box obstacles, perturbed moments and inflow forcing)",
+        "config": {"workload": f"{a.members} D3Q19 regularized + Vreman members, sizes from "
+                               "{96,128,160}x{64,96}x{64,96}, box obstacle side 8-16, inflow/outflow i, periodic j, "
+                               "no-slip k=1 / free-slip k=nz, inflow forcing at i=10, nu=1e-4",
+                   "cells": cells, "members": a.members, "storage": a.storage, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "step_kernel_batch + inflow_kernel_batch (graph)", "bytes_per_cell": bpc_bytes,
+                     "kernel_ms": round(step_ms, 5)},
+        "tuned_launch": {"block": block, "blocks_per_cu": bpc, "ms_per_step": round(tune_ms, 5)},
+        "per_member_launches": {"value": round(mlups_pm, 1), "unit": "MLUPS", "steps": n_pm,
+                                "ms_per_step": round(t_pm / n_pm * 1e3, 5)},
+        "speedup_vs_per_member": round(mlups / mlups_pm, 3),
+        "cpu_baseline": None, "finite": finite,
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,11 +190,16 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
     ap.add_argument("--block", type=int, nargs=2, default=(0, 0))
+    ap.add_argument("--workload", default="headline", choices=["headline", "ensemble"])
+    ap.add_argument("--members", type=int, default=64)
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
                                    "described in DESIGN.md, not built)"}))
         sys.exit(2)
+    if a.workload == "ensemble":
+        bench_ensemble(a)
+        return
 
     from lbm_amd.domain import Domain
     w = workload(a.storage)

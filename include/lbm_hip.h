@@ -128,6 +128,39 @@ int lbm_get_totals(lbm_ctx* ctx, double* mass, double* momentum3);
 /* Bytes of HBM the context holds, and the device layout pitch (elements per row). */
 int lbm_get_info(lbm_ctx* ctx, int64_t* device_bytes, int32_t* row_pitch, int64_t* plane_elems);
 
+/*
+ * Ensembles.  No counterpart in the reference, which runs one domain per process: many contexts
+ * of one lattice, storage and collision model (sizes, boundary codes, solids, inflow and
+ * turbulence inputs free per member) advance together.  Each step is one launch of a persistent
+ * batched step kernel over every member's cells, one launch of a batched inflow/outflow kernel
+ * and a device clock tick, captured in HIP graphs.  Members keep behaving as their own contexts:
+ * lbm_get_state / lbm_get_macros / lbm_set_* on a member work between ensemble steps (a member
+ * changed with lbm_set_* is picked up by the next lbm_ensemble_step), and the result is exactly
+ * that of lbm_step on each member.  A member may have at most one closed (11/12/21/22) axis,
+ * whose face pass the step kernel performs; the members stay owned by the caller and must outlive
+ * the ensemble.
+ */
+typedef struct lbm_ens lbm_ens;
+/* On failure *out may still be set: read lbm_ensemble_last_error(*out), then destroy it. */
+int lbm_ensemble_create(lbm_ctx* const* members, int n, lbm_ens** out);
+int lbm_ensemble_destroy(lbm_ens* ens);
+const char* lbm_ensemble_last_error(const lbm_ens* ens);
+/* Steps it0+1 .. it0+nsteps of every member. */
+int lbm_ensemble_step(lbm_ens* ens, int it0, int nsteps);
+int lbm_ensemble_synchronize(lbm_ens* ens);
+/* Launch shape: block threads per tile (multiple of 64, <= 1024) and a persistent grid of
+ * blocks_per_cu blocks per compute unit (0: one block per tile).  Default 256, 0. */
+int lbm_ensemble_set_launch(lbm_ens* ens, int block, int blocks_per_cu);
+int lbm_ensemble_get_launch(lbm_ens* ens, int* block, int* blocks_per_cu);
+/* Autotuner: times trial_steps batched steps (after 2 untimed ones) for each candidate launch
+ * shape from the same saved state, keeps the fastest and restores every member's state.
+ * best_ms: milliseconds per step of the chosen shape. */
+int lbm_ensemble_tune(lbm_ens* ens, int it0, int trial_steps, int* best_block, int* best_blocks_per_cu,
+                      double* best_ms);
+/* HIP-event timing of the batched steps (summed milliseconds and steps since the last read). */
+int lbm_ensemble_set_timing(lbm_ens* ens, int on);
+int lbm_ensemble_timing_read(lbm_ens* ens, double* ms, int* nsteps);
+
 #ifdef __cplusplus
 }
 #endif

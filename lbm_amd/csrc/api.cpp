@@ -11,58 +11,16 @@
 #include <vector>
 
 #include "../../include/lbm_hip.h"
+#include "ctx.h"
 #include "impl.h"
 #include "layout.h"
 
 using namespace lbm;
+using namespace lbm_host;
 
-namespace {
+static thread_local std::string g_create_error;
 
-thread_local std::string g_create_error;
-
-struct DevBuf {
-    void* p = nullptr;
-    size_t n = 0;
-};
-
-}  // namespace
-
-struct lbm_ctx {
-    lbm_desc d{};
-    Layout lay{};
-    const Ops* ops = nullptr;
-    hipStream_t stream = nullptr;
-    int device = 0;
-    // population buffers
-    void* buf[2] = {nullptr, nullptr};
-    size_t buf_bytes = 0;
-    int cur = 0;
-    bool have_state = false;
-    bool pending = false;          // buf[cur] holds an uncollided state (next step collides in place)
-    bool ghost_raw[2] = {true, true};
-    bool use_ext = false;          // first collision uses the moments given at initialisation
-    // inputs
-    DevBuf ablvisc, shear, solid, turb[3], force[3], tau, ext[4], stage, partial, snap;
-    SnapList snap_k{}, snap_j{};
-    int solid_wpr = 0;
-    bool has_solid = false, has_turb = false, has_force = false;
-    int nrturb = 0;
-    double ampl = 0.0;
-    double uini = 0.0, udir = 0.0;
-    int series_lo = 0, series_hi = -1;
-    std::vector<double> ut_series, ud_series;
-    double tau0 = 0.0, fac0 = 0.0;
-    int fold_dir = -1;             // closed axis whose face pass the step kernel performs
-    int variant = 1;               // step-kernel variant (LBM_STEP_VARIANT), see impl.h; non-temporal
-                                   // stores measured fastest on MI355X (5.32 vs 5.11 TB/s)
-    // timing
-    bool timing = false;
-    std::vector<hipEvent_t> ev_step, ev_bc;   // pairs
-    size_t n_step_ev = 0, n_bc_ev = 0;
-    std::string err;
-};
-
-namespace {
+namespace lbm_host {
 
 int fail(lbm_ctx* c, int code, const std::string& msg) {
     if (c) c->err = msg;
@@ -74,11 +32,6 @@ int hipfail(lbm_ctx* c, hipError_t e, const char* where) {
     return fail(c, LBM_EHIP, std::string(where) + ": " + hipGetErrorString(e));
 }
 
-#define HIPCHK(c, call)                                   \
-    do {                                                  \
-        hipError_t e_ = (call);                           \
-        if (e_ != hipSuccess) return hipfail((c), e_, #call); \
-    } while (0)
 
 int dev_alloc(lbm_ctx* c, DevBuf& b, size_t n) {
     if (b.n >= n && b.p) return LBM_OK;
@@ -254,9 +207,51 @@ hipEvent_t next_event(lbm_ctx* c, std::vector<hipEvent_t>& pool, size_t& used) {
     return pool[used++];
 }
 
+void inflow_values(const lbm_ctx* c, int it, double* ut, double* cs, double* sn) {
+    // cos(udir*pi/180.0) in the reference's REAL precision (src/m_boundary_i_inflow_kernel.F90:71-73)
+    *ut = round_real(c, utime_at(c, it));
+    if (c->ops->real_bytes == 4) {
+        const float ang = (float)udir_at(c, it) * 3.1415927410125732f / 180.0f;
+        *cs = cosf(ang);
+        *sn = sinf(ang);
+    } else {
+        const double ang = udir_at(c, it) * 3.1415927410125732 / 180.0;
+        *cs = cos(ang);
+        *sn = sin(ang);
+    }
+}
+
+StepHost step_host(const lbm_ctx* c) {
+    const lbm_desc& d = c->d;
+    StepHost h{};
+    h.fac0 = c->fac0;
+    h.kinevisc = d.kinevisc;
+    h.vconst = round_real(c, 2.5 * round_real(c, round_real(c, d.smagorinsky) * round_real(c, d.smagorinsky)));
+    h.ablvisc = c->ablvisc.p;
+    h.solid = c->has_solid ? static_cast<const uint32_t*>(c->solid.p) : nullptr;
+    h.solid_wpr = c->solid_wpr;
+    h.ip = d.iturb_pos;
+    h.ampl = c->ampl;
+    for (int q = 0; q < 3; ++q) h.F[q] = c->has_force ? c->force[q].p : nullptr;
+    h.tau_out = (d.store_tau && d.ivreman) ? c->tau.p : nullptr;
+    h.bgk3 = d.ibgk == 3;
+    h.hrr = d.ihrr == 1;
+    h.vre = d.ivreman == 1;
+    h.block_x = d.block_x;
+    h.block_y = d.block_y;
+    h.variant = c->variant;
+    h.fold_dir = c->fold_dir;
+    h.fold_opt_lo = h.fold_opt_hi = 0;
+    if (c->fold_dir >= 0) {
+        const int code = c->fold_dir == 0 ? d.ibnd : c->fold_dir == 1 ? d.jbnd : d.kbnd;
+        opts_of(code, &h.fold_opt_lo, &h.fold_opt_hi);
+    }
+    return h;
+}
+
 // boundarycond(f1, f2) for the non-periodic directions (src/m_boundarycond.F90:166-365).
 // skip_dir: a closed axis whose face pass the step kernel has already performed.
-int boundary(lbm_ctx* c, void* f1, const void* f2, int it, int skip_dir = -1) {
+int boundary(lbm_ctx* c, void* f1, const void* f2, int it, int skip_dir) {
     const lbm_desc& d = c->d;
     const Layout& L = c->lay;
     hipEvent_t e0 = nullptr;
@@ -279,18 +274,8 @@ int boundary(lbm_ctx* c, void* f1, const void* f2, int it, int skip_dir = -1) {
     const EdgeList e = edge_list(d);
     if (e.n) HIPCHK(c, c->ops->edges(L, f1, f2, e, c->stream));
     if (d.ibnd == 1) {
-        // cos(udir*pi/180.0) in the reference's REAL precision (src/m_boundary_i_inflow_kernel.F90:71-73)
-        const double ut = round_real(c, utime_at(c, it));
-        double cs, sn;
-        if (c->ops->real_bytes == 4) {
-            const float ang = (float)udir_at(c, it) * 3.1415927410125732f / 180.0f;
-            cs = cosf(ang);
-            sn = sinf(ang);
-        } else {
-            const double ang = udir_at(c, it) * 3.1415927410125732 / 180.0;
-            cs = cos(ang);
-            sn = sin(ang);
-        }
+        double ut, cs, sn;
+        inflow_values(c, it, &ut, &cs, &sn);
         HIPCHK(c, c->ops->inflow(L, f1, c->shear.p, ut, cs, sn, d.rho0, c->stream));
     }
     if (c->timing && e0) {
@@ -470,29 +455,7 @@ int lbm_step(lbm_ctx* c, int it0, int nsteps) {
     if (!c->have_state) return fail(c, LBM_ESTATE, "no state: call lbm_set_state or lbm_init_equilibrium first");
     if (nsteps < 0) return fail(c, LBM_EINVAL, "nsteps < 0");
     const lbm_desc& d = c->d;
-    StepHost h{};
-    h.fac0 = c->fac0;
-    h.kinevisc = d.kinevisc;
-    h.vconst = round_real(c, 2.5 * round_real(c, round_real(c, d.smagorinsky) * round_real(c, d.smagorinsky)));
-    h.ablvisc = c->ablvisc.p;
-    h.solid = c->has_solid ? static_cast<const uint32_t*>(c->solid.p) : nullptr;
-    h.solid_wpr = c->solid_wpr;
-    h.ip = d.iturb_pos;
-    h.ampl = c->ampl;
-    for (int q = 0; q < 3; ++q) h.F[q] = c->has_force ? c->force[q].p : nullptr;
-    h.tau_out = (d.store_tau && d.ivreman) ? c->tau.p : nullptr;
-    h.bgk3 = d.ibgk == 3;
-    h.hrr = d.ihrr == 1;
-    h.vre = d.ivreman == 1;
-    h.block_x = d.block_x;
-    h.block_y = d.block_y;
-    h.variant = c->variant;
-    h.fold_dir = c->fold_dir;
-    h.fold_opt_lo = h.fold_opt_hi = 0;
-    if (c->fold_dir >= 0) {
-        const int code = c->fold_dir == 0 ? d.ibnd : c->fold_dir == 1 ? d.jbnd : d.kbnd;
-        opts_of(code, &h.fold_opt_lo, &h.fold_opt_hi);
-    }
+    StepHost h = step_host(c);
     const size_t slice = (size_t)d.ny * d.nz * c->ops->real_bytes;
     for (int it = it0 + 1; it <= it0 + nsteps; ++it) {
         if (c->pending) {

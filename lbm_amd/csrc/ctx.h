@@ -1,0 +1,81 @@
+// Internal: the context behind lbm_ctx* and the host helpers shared by the C-ABI translation
+// units (api.cpp: single domains, ensemble.cpp: ensembles).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/lbm_hip.h"
+#include "impl.h"
+#include "layout.h"
+
+namespace lbm_host {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t n = 0;
+};
+
+}  // namespace lbm_host
+
+struct lbm_ctx {
+    lbm_desc d{};
+    lbm::Layout lay{};
+    const lbm::Ops* ops = nullptr;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    // population buffers
+    void* buf[2] = {nullptr, nullptr};
+    size_t buf_bytes = 0;
+    int cur = 0;
+    bool have_state = false;
+    bool pending = false;          // buf[cur] holds an uncollided state (next step collides in place)
+    bool ghost_raw[2] = {true, true};
+    bool use_ext = false;          // first collision uses the moments given at initialisation
+    // inputs
+    lbm_host::DevBuf ablvisc, shear, solid, turb[3], force[3], tau, ext[4], stage, partial, snap;
+    lbm::SnapList snap_k{}, snap_j{};
+    int solid_wpr = 0;
+    bool has_solid = false, has_turb = false, has_force = false;
+    int nrturb = 0;
+    double ampl = 0.0;
+    double uini = 0.0, udir = 0.0;
+    int series_lo = 0, series_hi = -1;
+    std::vector<double> ut_series, ud_series;
+    double tau0 = 0.0, fac0 = 0.0;
+    int fold_dir = -1;             // closed axis whose face pass the step kernel performs
+    int variant = 1;               // step-kernel variant (LBM_STEP_VARIANT), see impl.h; non-temporal
+                                   // stores measured fastest on MI355X (5.32 vs 5.11 TB/s)
+    // timing
+    bool timing = false;
+    std::vector<hipEvent_t> ev_step, ev_bc;   // pairs
+    size_t n_step_ev = 0, n_bc_ev = 0;
+    std::string err;
+};
+
+namespace lbm_host {
+
+int fail(lbm_ctx* c, int code, const std::string& msg);
+int hipfail(lbm_ctx* c, hipError_t e, const char* where);
+int dev_alloc(lbm_ctx* c, DevBuf& b, size_t n);
+void dev_free(DevBuf& b);
+double round_real(const lbm_ctx* c, double v);
+void opts_of(int code, int* o1, int* oN);
+double utime_at(const lbm_ctx* c, int it);
+double udir_at(const lbm_ctx* c, int it);
+// (utime, cos(udir), sin(udir)) of step it in the compute type's rounding
+void inflow_values(const lbm_ctx* c, int it, double* ut, double* cs, double* sn);
+// the step-kernel parameters of c that do not change from step to step
+lbm::StepHost step_host(const lbm_ctx* c);
+// boundarycond(f1, f2) for the non-periodic directions; skip_dir: face done by the step kernel
+int boundary(lbm_ctx* c, void* f1, const void* f2, int it, int skip_dir = -1);
+
+}  // namespace lbm_host
+
+#define HIPCHK(c, call)                                                   \
+    do {                                                                  \
+        hipError_t e_ = (call);                                           \
+        if (e_ != hipSuccess) return lbm_host::hipfail((c), e_, #call);   \
+    } while (0)

@@ -51,6 +51,33 @@ struct StepHost {
     int block_x, block_y;
 };
 
+// One ensemble member for the batched kernels: its step parameters (src, dst and the turbulence
+// slice are chosen on the device per step) and the buffers.
+struct MemberHost {
+    Layout lay;
+    StepHost h;
+    void* buf[2];
+    int cur0;
+    int nrturb;
+    const void* turb[3];   // slice 0 of uu, vv, ww (nullptr: off)
+    const void* shear;     // uvel_shear(nz)
+    double rho0;
+};
+
+// One batched step: the step kernel over every member's tiles, the inflow kernel over the
+// inflow members' planes, the clock tick.
+struct EnsLaunch {
+    const void* tab;                 // device MemberTab<R>[nmem]
+    const int* start;                // device tile starts [nmem]
+    int nmem, total, block, grid;
+    const int* istart;               // device inflow tile starts [ninf]
+    const int* imem;                 // device inflow member indices [ninf]
+    int ninf, itotal, iblock, igrid;
+    const void* vals;                // device (utime, cos, sin)[steps][nmem]
+    int* clock;                      // device {it, steps since set}
+    int bgk3, hrr, vre;
+};
+
 struct Ops {
     hipError_t (*step)(const Layout&, const StepHost&, hipStream_t);
     hipError_t (*closed)(const Layout&, void* f1, const void* f2, int dir, int opt_lo, int opt_hi, const void* snap,
@@ -66,6 +93,10 @@ struct Ops {
                            int realkind, void* dst, void* mr, void* mu, void* mv, void* mw, hipStream_t);
     hipError_t (*totals)(const Layout&, const void* cur, int stream_interior, double* partial, int nblocks,
                          hipStream_t);
+    // ensembles
+    size_t member_bytes;   // sizeof(MemberTab<R>)
+    void (*ens_pack)(const MemberHost&, int block, void* out, int* tiles, int* inflow_tiles);
+    hipError_t (*ens_step)(const EnsLaunch&, hipStream_t);
     int real_bytes;   // compute type size
     int raw_bytes;    // storage size
 };

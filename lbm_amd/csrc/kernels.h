@@ -332,17 +332,15 @@ __device__ __forceinline__ void fold_closed_face(const StepArgs<typename C::real
     }
 }
 
+// One cell of the step at storage coordinates (ca, cb, cc) (1-based along ax0, ax1, ax2):
+// gather the streamed populations, collide, store, and the folded face pass.
 // VAR bit 0: non-temporal stores, bit 1: non-temporal loads (measured slower; kept for tuning).
 template <int Q, class C, bool BGK3, bool HRR, bool VRE, int VAR>
-__global__ void __launch_bounds__(1024) step_kernel(const StepArgs<typename C::real> a) {
+__device__ __forceinline__ void step_cell(const StepArgs<typename C::real>& a, int ca, int cb, int cc) {
     using R = typename C::real;
     using raw = typename C::raw;
     using Lt = Lattice<Q>;
     const Layout& L = a.lay;
-    // Threads run along the storage axes: x along the contiguous axis ax0, y along ax1, z along ax2.
-    const int ca = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int cb = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const int cc = blockIdx.z + 1;
     if (ca > L.n(L.ax0) || cb > L.n(L.ax1) || cc > L.n(L.ax2)) return;
     int i, j, k;
     L.ijk(ca, cb, cc, i, j, k);
@@ -381,6 +379,13 @@ __global__ void __launch_bounds__(1024) step_kernel(const StepArgs<typename C::r
     }
     if (VRE && a.tau_out != nullptr) a.tau_out[oc] = tau;
     if (a.fold_dir >= 0) fold_closed_face<Q, C>(a, i, j, k, out);
+}
+
+// Threads run along the storage axes: x along the contiguous axis ax0, y along ax1, z along ax2.
+template <int Q, class C, bool BGK3, bool HRR, bool VRE, int VAR>
+__global__ void __launch_bounds__(1024) step_kernel(const StepArgs<typename C::real> a) {
+    step_cell<Q, C, BGK3, HRR, VRE, VAR>(a, blockIdx.x * blockDim.x + threadIdx.x + 1,
+                                         blockIdx.y * blockDim.y + threadIdx.y + 1, blockIdx.z + 1);
 }
 
 // Two raw values packed in one aligned word: 32 bits for 16-bit storage, 64 bits for FP32.
@@ -620,19 +625,15 @@ __global__ void bc_edges_kernel(Layout L, void* f1, const void* f2, const EdgeLi
     }
 }
 
-// Inflow at i = 0 and zero-gradient outflow at i = nx+1 (src/m_boundary_i_inflow_kernel.F90:61-111).
-// Threads run along j (jfast = 1) or k, whichever is stored first; all loads precede the stores.
+// Inflow at i = 0 and zero-gradient outflow at i = nx+1 (src/m_boundary_i_inflow_kernel.F90:61-111)
+// for the ghost row (j, k); all loads precede the stores.
 template <int Q, class C>
-__global__ void bc_inflow_kernel(Layout L, void* f1, const typename C::real* shear, typename C::real utime,
-                                 typename C::real cosd, typename C::real sind, typename C::real rho0, int jfast) {
+__device__ __forceinline__ void inflow_cell(const Layout& L, void* f1, const typename C::real* shear,
+                                            typename C::real utime, typename C::real cosd, typename C::real sind,
+                                            typename C::real rho0, int j, int k) {
     using R = typename C::real;
     using raw = typename C::raw;
     using Lt = Lattice<Q>;
-    const int x = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const int j = jfast ? x : y;
-    const int k = jfast ? y : x;
-    if (j > L.ny || k > L.nz) return;
     raw* __restrict__ p = static_cast<raw*>(f1);
     const R uu = utime * shear[k - 1];        // uvel(k) = uvel_time(it)*uvel_shear(k), taper = 1
     const R cs2 = R(1) / R(3);
@@ -654,6 +655,111 @@ __global__ void bc_inflow_kernel(Layout L, void* f1, const typename C::real* she
     for (int l = 0; l < Q; ++l) {
         p[(size_t)l * L.plane + o0] = C::template encode<Q>(Lt::cx(l) <= 0 ? fg[l] : fg[Lt::opposite(l)], l);
         p[(size_t)l * L.plane + oe] = fo[l];
+    }
+}
+
+// Threads run along j (jfast = 1) or k, whichever is stored first.
+template <int Q, class C>
+__global__ void bc_inflow_kernel(Layout L, void* f1, const typename C::real* shear, typename C::real utime,
+                                 typename C::real cosd, typename C::real sind, typename C::real rho0, int jfast) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int j = jfast ? x : y;
+    const int k = jfast ? y : x;
+    if (j > L.ny || k > L.nz) return;
+    inflow_cell<Q, C>(L, f1, shear, utime, cosd, sind, rho0, j, k);
+}
+
+// ------------------------------------------------------------------------------------------
+// Ensembles: many domains of one lattice, storage and collision model advanced by one launch of
+// a persistent kernel per step phase.  Every domain's cells are cut into tiles of blockDim.x
+// consecutive cells of one (ax0, ax1) plane; the tiles of all members form one list that the
+// blocks of the grid walk with stride gridDim.x.  Per-step values are read from a device clock
+// (clock[0] = it of the step being computed, clock[1] = steps done since the clock was set) so
+// that a captured graph of steps replays without host updates.
+// ------------------------------------------------------------------------------------------
+template <typename R>
+struct MemberTab {
+    StepArgs<R> a;          // src, dst, tu, tv, tw are set per step
+    void* buf[2];
+    int cur0;               // buffer holding the current state when the clock was set
+    int nrturb, slice;      // turbulence slices of ny*nz reals; nrturb = 0: off
+    const R* turb[3];       // slice 0 of uu, vv, ww
+    int n0, tiles_plane;    // cells along ax0; tiles per (ax0, ax1) plane
+    const R* shear;         // inflow members: uvel_shear(nz)
+    R rho0;
+    int jfast;
+};
+
+// Member owning tile t: the last m with start[m] <= t (start[0] = 0, increasing).
+__device__ __forceinline__ int owner(const int* __restrict__ start, int n, int t) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <int Q, class C, bool BGK3, bool HRR, bool VRE>
+__global__ void __launch_bounds__(1024) step_kernel_batch(const MemberTab<typename C::real>* __restrict__ tab,
+                                                          const int* __restrict__ start, int nmem, int total,
+                                                          const int* __restrict__ clock) {
+    using R = typename C::real;
+    const int it = clock[0];
+    const int s = clock[1];
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+        const int m = owner(start, nmem, t);
+        const MemberTab<R>& M = tab[m];
+        StepArgs<R> a = M.a;
+        const int p = (M.cur0 + s) & 1;
+        a.src = M.buf[p];
+        a.dst = M.buf[p ^ 1];
+        if (M.nrturb > 0) {
+            int lit = it % M.nrturb;             // src/m_inflow_turbulence_forcing.F90:45-46
+            if (lit == 0) lit = M.nrturb;
+            const size_t off = (size_t)lit * M.slice;
+            a.tu = M.turb[0] + off;
+            a.tv = M.turb[1] + off;
+            a.tw = M.turb[2] + off;
+        }
+        const int local = t - start[m];
+        const int cc = local / M.tiles_plane;
+        const int e = (local - cc * M.tiles_plane) * (int)blockDim.x + (int)threadIdx.x;
+        const int cb = e / M.n0;
+        step_cell<Q, C, BGK3, HRR, VRE, 1>(a, e - cb * M.n0 + 1, cb + 1, cc + 1);
+    }
+}
+
+// The inflow/outflow planes of the members listed in mem[0..ninf) (tiles from istart), on the
+// buffer the step just wrote.  vals holds (uvel_time(it), cos, sin) per step and member.
+template <int Q, class C>
+__global__ void inflow_kernel_batch(const MemberTab<typename C::real>* __restrict__ tab, const int* __restrict__ istart,
+                                    const int* __restrict__ mem, int ninf, int total, int nmem,
+                                    const typename C::real* __restrict__ vals, const int* __restrict__ clock) {
+    using R = typename C::real;
+    const int s = clock[1];
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+        const int q = owner(istart, ninf, t);
+        const int m = mem[q];
+        const MemberTab<R>& M = tab[m];
+        const Layout& L = M.a.lay;
+        const int nf = M.jfast ? L.ny : L.nz;
+        const int e = (t - istart[q]) * (int)blockDim.x + (int)threadIdx.x;
+        const int x = e % nf + 1, y = e / nf + 1;
+        const int j = M.jfast ? x : y, k = M.jfast ? y : x;
+        if (j > L.ny || k > L.nz) continue;
+        const R* v = vals + ((size_t)s * nmem + m) * 3;
+        inflow_cell<Q, C>(L, M.buf[((M.cur0 + s) & 1) ^ 1], M.shear, v[0], v[1], v[2], M.rho0, j, k);
+    }
+}
+
+template <int Q, class C>
+__global__ void clock_tick_kernel(int* clock) {
+    if (threadIdx.x == 0) {
+        clock[0] += 1;
+        clock[1] += 1;
     }
 }
 

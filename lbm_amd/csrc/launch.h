@@ -3,6 +3,7 @@
 
 #include <cfloat>
 #include <cmath>
+#include <cstring>
 
 #include "impl.h"
 #include "kernels.h"
@@ -52,7 +53,7 @@ struct Launch {
         return hipGetLastError();
     }
 
-    static hipError_t step(const Layout& L, const StepHost& h, hipStream_t s) {
+    static StepArgs<R> args_of(const Layout& L, const StepHost& h) {
         StepArgs<R> a;
         a.lay = L;
         a.src = h.src;
@@ -82,6 +83,11 @@ struct Launch {
         a.fold_dir = h.fold_dir;
         a.fold_opt_lo = h.fold_opt_lo;
         a.fold_opt_hi = h.fold_opt_hi;
+        return a;
+    }
+
+    static hipError_t step(const Layout& L, const StepHost& h, hipStream_t s) {
+        const StepArgs<R> a = args_of(L, h);
         const dim3 block(h.block_x, h.block_y, 1);
         const dim3 grid((L.n(L.ax0) + h.block_x - 1) / h.block_x, (L.n(L.ax1) + h.block_y - 1) / h.block_y, L.n(L.ax2));
         const int key = (h.bgk3 ? 4 : 0) | (h.hrr ? 2 : 0) | (h.vre ? 1 : 0);
@@ -191,8 +197,59 @@ struct Launch {
         return hipGetLastError();
     }
 
+    // ---- ensembles
+    static void ens_pack(const MemberHost& m, int block, void* out, int* tiles, int* inflow_tiles) {
+        const Layout& L = m.lay;
+        MemberTab<R> t{};
+        t.a = args_of(L, m.h);
+        t.a.stream = 1;
+        t.a.tu = t.a.tv = t.a.tw = nullptr;
+        t.buf[0] = m.buf[0];
+        t.buf[1] = m.buf[1];
+        t.cur0 = m.cur0;
+        t.nrturb = m.turb[0] ? m.nrturb : 0;
+        t.slice = L.ny * L.nz;
+        for (int q = 0; q < 3; ++q) t.turb[q] = static_cast<const R*>(m.turb[q]);
+        t.n0 = L.n(L.ax0);
+        const long long plane = (long long)L.n(L.ax0) * L.n(L.ax1);
+        t.tiles_plane = (int)((plane + block - 1) / block);
+        t.shear = static_cast<const R*>(m.shear);
+        t.rho0 = (R)m.rho0;
+        t.jfast = (L.ax0 == 1) || (L.ax0 == 0 && L.ax1 == 1);
+        std::memcpy(out, &t, sizeof(t));
+        *tiles = t.tiles_plane * L.n(L.ax2);
+        *inflow_tiles = (L.ny * L.nz + 255) / 256;
+    }
+
+    template <bool B3, bool H, bool V>
+    static void ens_step_t(const EnsLaunch& e, hipStream_t s) {
+        hipLaunchKernelGGL((step_kernel_batch<Q, C, B3, H, V>), dim3(e.grid), dim3(e.block), 0, s,
+                           static_cast<const MemberTab<R>*>(e.tab), e.start, e.nmem, e.total, e.clock);
+    }
+
+    static hipError_t ens_step(const EnsLaunch& e, hipStream_t s) {
+        const int key = (e.bgk3 ? 4 : 0) | (e.hrr ? 2 : 0) | (e.vre ? 1 : 0);
+        switch (key) {
+            case 0: ens_step_t<false, false, false>(e, s); break;
+            case 1: ens_step_t<false, false, true>(e, s); break;
+            case 2: ens_step_t<false, true, false>(e, s); break;
+            case 3: ens_step_t<false, true, true>(e, s); break;
+            case 4: ens_step_t<true, false, false>(e, s); break;
+            case 5: ens_step_t<true, false, true>(e, s); break;
+            case 6: ens_step_t<true, true, false>(e, s); break;
+            default: ens_step_t<true, true, true>(e, s); break;
+        }
+        if (e.ninf > 0)
+            hipLaunchKernelGGL((inflow_kernel_batch<Q, C>), dim3(e.igrid), dim3(e.iblock), 0, s,
+                               static_cast<const MemberTab<R>*>(e.tab), e.istart, e.imem, e.ninf, e.itotal, e.nmem,
+                               static_cast<const R*>(e.vals), e.clock);
+        hipLaunchKernelGGL((clock_tick_kernel<Q, C>), dim3(1), dim3(64), 0, s, e.clock);
+        return hipGetLastError();
+    }
+
     static const Ops* ops() {
         static const Ops o = {&step, &closed, &snapshot, &edges, &inflow, &aos_in, &state_out, &feq_init, &totals,
+                              sizeof(MemberTab<R>), &ens_pack, &ens_step,
                               (int)sizeof(R), (int)sizeof(typename C::raw)};
         return &o;
     }

@@ -27,7 +27,10 @@ EXPORTS = ["lbm_create", "lbm_destroy", "lbm_last_error", "lbm_set_state", "lbm_
            "lbm_get_state", "lbm_get_macros", "lbm_get_tau", "lbm_set_blanking", "lbm_set_ablvisc",
            "lbm_set_inflow", "lbm_set_inflow_series", "lbm_set_turbulence", "lbm_set_external_forcing",
            "lbm_step", "lbm_synchronize", "lbm_stream", "lbm_set_timing", "lbm_timing_read",
-           "lbm_get_totals", "lbm_get_info"]
+           "lbm_get_totals", "lbm_get_info",
+           "lbm_ensemble_create", "lbm_ensemble_destroy", "lbm_ensemble_last_error", "lbm_ensemble_step",
+           "lbm_ensemble_synchronize", "lbm_ensemble_set_launch", "lbm_ensemble_get_launch",
+           "lbm_ensemble_tune", "lbm_ensemble_set_timing", "lbm_ensemble_timing_read"]
 
 
 class Desc(C.Structure):
@@ -76,6 +79,16 @@ def load_library(path: str = LIB_PATH):
         "lbm_timing_read": ([vp, C.POINTER(dbl), C.POINTER(i32), C.POINTER(dbl), C.POINTER(i32)], i32),
         "lbm_get_totals": ([vp, C.POINTER(dbl), C.POINTER(dbl)], i32),
         "lbm_get_info": ([vp, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_int64)], i32),
+        "lbm_ensemble_create": ([C.POINTER(vp), i32, C.POINTER(vp)], i32),
+        "lbm_ensemble_destroy": ([vp], i32),
+        "lbm_ensemble_last_error": ([vp], C.c_char_p),
+        "lbm_ensemble_step": ([vp, i32, i32], i32),
+        "lbm_ensemble_synchronize": ([vp], i32),
+        "lbm_ensemble_set_launch": ([vp, i32, i32], i32),
+        "lbm_ensemble_get_launch": ([vp, C.POINTER(i32), C.POINTER(i32)], i32),
+        "lbm_ensemble_tune": ([vp, i32, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(dbl)], i32),
+        "lbm_ensemble_set_timing": ([vp, i32], i32),
+        "lbm_ensemble_timing_read": ([vp, C.POINTER(dbl), C.POINTER(i32)], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -238,3 +251,67 @@ class Domain:
         self._chk(self.lib.lbm_timing_read(self.h, C.byref(s), C.byref(ns), C.byref(b), C.byref(nb)),
                   "lbm_timing_read")
         return dict(step_ms=s.value, n_step=ns.value, bc_ms=b.value, n_bc=nb.value)
+
+
+class Ensemble:
+    """Members (Domain objects of one lattice, storage and collision model) advanced together by
+    the batched kernels (lbm_ensemble_* in include/lbm_hip.h).  The members stay usable as
+    Domains between ensemble steps and must outlive the ensemble."""
+
+    def __init__(self, members):
+        self.lib = load_library()
+        self.members = list(members)
+        arr = (C.c_void_p * len(self.members))(*[m.h.value for m in self.members])
+        h = C.c_void_p()
+        rc = self.lib.lbm_ensemble_create(arr, len(self.members), C.byref(h))
+        if rc != 0:
+            msg = self.lib.lbm_ensemble_last_error(h).decode() if h.value else "invalid arguments"
+            if h.value:
+                self.lib.lbm_ensemble_destroy(h)
+            raise LBMError(f"lbm_ensemble_create failed ({rc}): {msg}")
+        self.h = h
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise LBMError(f"{what} failed ({rc}): {self.lib.lbm_ensemble_last_error(self.h).decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.lbm_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, it0, nsteps=1):
+        self._chk(self.lib.lbm_ensemble_step(self.h, it0, nsteps), "lbm_ensemble_step")
+
+    def synchronize(self):
+        self._chk(self.lib.lbm_ensemble_synchronize(self.h), "lbm_ensemble_synchronize")
+
+    def set_launch(self, block, blocks_per_cu):
+        self._chk(self.lib.lbm_ensemble_set_launch(self.h, block, blocks_per_cu), "lbm_ensemble_set_launch")
+
+    def launch(self):
+        b, p = C.c_int(), C.c_int()
+        self._chk(self.lib.lbm_ensemble_get_launch(self.h, C.byref(b), C.byref(p)), "lbm_ensemble_get_launch")
+        return b.value, p.value
+
+    def tune(self, it0, trial_steps=10):
+        """Returns (block, blocks_per_cu, ms_per_step) of the fastest launch shape; the members'
+        states are unchanged."""
+        b, p, ms = C.c_int(), C.c_int(), C.c_double()
+        self._chk(self.lib.lbm_ensemble_tune(self.h, it0, trial_steps, C.byref(b), C.byref(p), C.byref(ms)),
+                  "lbm_ensemble_tune")
+        return b.value, p.value, ms.value
+
+    def set_timing(self, on=True):
+        self._chk(self.lib.lbm_ensemble_set_timing(self.h, int(on)), "lbm_ensemble_set_timing")
+
+    def timing_read(self):
+        ms, n = C.c_double(), C.c_int()
+        self._chk(self.lib.lbm_ensemble_timing_read(self.h, C.byref(ms), C.byref(n)), "lbm_ensemble_timing_read")
+        return ms.value, n.value

@@ -139,3 +139,34 @@ def box_mask(dims, lo, hi) -> np.ndarray:
     m = np.zeros((nz + 2, ny + 2, nx + 2), np.uint8)
     m[lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1] = 1
     return m
+
+
+def ensemble_specs(seed: int, n: int, sizes_i=(96, 128, 160), sizes_j=(64, 96), sizes_k=(64, 96),
+                   box_side=(8, 16)):
+    """Seeded members of the batched-ensemble workload: dims drawn uniformly from the size sets,
+    a solid box of side s in [box_side] standing on the k = 1 wall at i0 in [nx/6, nx/3],
+    j0 anywhere, and a per-member seed for the initial perturbation and the inflow forcing."""
+    u = uniform01(seed, 8 * n).reshape(n, 8)
+    out = []
+    for m in range(n):
+        r = u[m]
+        nx = sizes_i[int(r[0] * len(sizes_i))]
+        ny = sizes_j[int(r[1] * len(sizes_j))]
+        nz = sizes_k[int(r[2] * len(sizes_k))]
+        s = box_side[0] + int(r[3] * (box_side[1] - box_side[0] + 1))
+        i0 = nx // 6 + int(r[4] * (nx // 3 - nx // 6))
+        j0 = 1 + int(r[5] * (ny - s))
+        out.append(dict(dims=(nx, ny, nz), box_lo=(i0, j0, 1), box_hi=(i0 + s - 1, j0 + s - 1, s),
+                        seed=seed * 1000 + m + 1))
+    return out
+
+
+def perturbed_moments(dims, seed: int, u0=(0.05, 0.0, 0.0), amp: float = 1e-3, dtype=np.float32):
+    """rho = 1 and u = u0 + U(-amp, amp) per component on every cell including ghosts, for
+    lbm_init_equilibrium.  Shapes (nz+2, ny+2, nx+2)."""
+    nx, ny, nz = dims
+    n = (nx + 2) * (ny + 2) * (nz + 2)
+    r = uniform01(seed, 3 * n).reshape(3, nz + 2, ny + 2, nx + 2)
+    rho = np.ones((nz + 2, ny + 2, nx + 2), dtype)
+    vel = [(u0[c] + amp * (2.0 * r[c] - 1.0)).astype(dtype) for c in range(3)]
+    return rho, vel[0], vel[1], vel[2]
