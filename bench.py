@@ -20,6 +20,14 @@ times the batched ensemble instead: 64 D3Q19 regularized + Vreman members with s
 seed from {96,128,160}x{64,96}x{64,96}, a seeded box obstacle (side 8-16) and seeded inflow
 forcing each, FP32, graph-captured batched steps with the autotuned launch shape, against the
 same members stepped with one launch sequence per member.
+
+    python bench.py --workload adjoint [--steps 400]
+
+times the hand-written adjoint (forward with checkpoints every 20 steps, recomputation, reverse
+sweep) of D3Q19 regularized + Vreman, FP32, 192x96x96, inflow/outflow i, periodic j, no-slip
+k=1, with 64 Gaussian body-force blobs and an 8-coefficient power-law inflow profile, objective
+the time-mean kinetic energy over i in [140,180]; the gradient is checked against FP64 central
+differences (h = 1e-6) on a subset of the parameters and against the FP64 adjoint on all.
 """
 from __future__ import annotations
 
@@ -180,6 +188,72 @@ def bench_ensemble(a):
     print(json.dumps(line))
 
 
+def bench_adjoint(a):
+    import torch
+    from lbm_amd.torch_adjoint import ForcingProblem
+    nsteps = a.steps
+    print(f"[bench] adjoint problem 192x96x96, {nsteps} steps", file=sys.stderr, flush=True)
+    p32 = ForcingProblem(storage="f32", nsteps=nsteps, every=20)
+    F = p32.force_field(p32.a0).numpy()
+    s = p32.shear_profile(p32.c0).numpy()
+    p32.evaluate(F, s, adjoint=False)          # warm-up (allocations, first launches)
+    t0 = time.perf_counter()
+    J_fwd = p32.evaluate(F, s, adjoint=False)
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    J_adj, gF32, gS32 = p32.evaluate(F, s, adjoint=True)
+    t_adj = time.perf_counter() - t0
+    print(f"[bench] forward {t_fwd:.3f} s, adjoint run {t_adj:.3f} s", file=sys.stderr, flush=True)
+    # parameter gradients through the torch harness
+    def grads(prob):
+        av = prob.a0.clone().requires_grad_(True)
+        cv = prob.c0.clone().requires_grad_(True)
+        J = prob.J(av, cv)
+        J.backward()
+        return float(J), av.grad.numpy().copy(), cv.grad.numpy().copy()
+    J32, ga32, gc32 = grads(p32)
+    p32.close()
+    p64 = ForcingProblem(storage="f64", nsteps=nsteps, every=20)
+    J64, ga64, gc64 = grads(p64)
+    which = [("a", 0), ("a", 17), ("a", 40), ("c", 0), ("c", 1), ("c", 5)]
+    h = 1e-6
+    fd = []
+    print("[bench] FP64 central differences", file=sys.stderr, flush=True)
+    for kind, i in which:
+        ap, am, cp, cm = p64.a0.clone(), p64.a0.clone(), p64.c0.clone(), p64.c0.clone()
+        (ap if kind == "a" else cp)[i] += h
+        (am if kind == "a" else cm)[i] -= h
+        fd.append((p64.J_value(ap, cp) - p64.J_value(am, cm)) / (2 * h))
+    p64.close()
+    fd = np.array(fd)
+    pick = lambda ga, gc: np.array([ga[i] if k == "a" else gc[i] for k, i in which])
+    scale = np.abs(fd).max()
+    e32 = float(np.abs(pick(ga32, gc32) - fd).max() / scale)
+    e64 = float(np.abs(pick(ga64, gc64) - fd).max() / scale)
+    full64 = np.concatenate([ga64, gc64])
+    full32 = np.concatenate([ga32, gc32])
+    e_full = float(np.abs(full32 - full64).max() / np.abs(full64).max())
+    cells = 192 * 96 * 96
+    line = {
+        "metric": "adjoint MLUPS (D3Q19 reg+Vreman, checkpoint + recompute + reverse)",
+        "value": round(cells * nsteps / t_adj / 1e6, 1), "unit": "MLUPS", "n_gpus": 1, "steps": nsteps, "warmup": 1,
+        "ms_per_step": round(t_adj / nsteps * 1e3, 5), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded populations, blob centres and amplitudes)",
+        "config": {"workload": "D3Q19 regularized + Vreman, 192x96x96, inflow/outflow i, periodic j, no-slip k=1 / "
+                               "free-slip k=nz, 64 Gaussian force blobs (sigma 4) near i=40, 8-coefficient power-law "
+                               "inflow, objective: time-mean KE over i in [140,180], checkpoints every 20",
+                   "cells": cells, "parallelism": "single GPU"},
+        "forward_mlups": round(cells * nsteps / t_fwd / 1e6, 1), "adjoint_over_forward": round(t_adj / t_fwd, 3),
+        "J": {"f32": J_adj, "f64": J64, "forward_f32": J_fwd},
+        "grad_check": {"checked": [f"{k}{i}" for k, i in which], "fd64": fd.tolist(),
+                       "adj32": pick(ga32, gc32).tolist(), "adj64": pick(ga64, gc64).tolist(),
+                       "max_rel_err_f32_vs_fd64": e32, "max_rel_err_f64_vs_fd64": e64,
+                       "max_rel_err_f32_vs_f64_all_72": e_full},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -190,7 +264,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
     ap.add_argument("--block", type=int, nargs=2, default=(0, 0))
-    ap.add_argument("--workload", default="headline", choices=["headline", "ensemble"])
+    ap.add_argument("--workload", default="headline", choices=["headline", "ensemble", "adjoint"])
     ap.add_argument("--members", type=int, default=64)
     a = ap.parse_args()
     if a.gpus != 1:
@@ -199,6 +273,9 @@ def main():
         sys.exit(2)
     if a.workload == "ensemble":
         bench_ensemble(a)
+        return
+    if a.workload == "adjoint":
+        bench_adjoint(a)
         return
 
     from lbm_amd.domain import Domain

@@ -161,6 +161,31 @@ int lbm_ensemble_tune(lbm_ens* ens, int it0, int trial_steps, int* best_block, i
 int lbm_ensemble_set_timing(lbm_ens* ens, int on);
 int lbm_ensemble_timing_read(lbm_ens* ens, double* ms, int* nsteps);
 
+/*
+ * Objective and adjoint.  No counterpart in the reference, which has no derivatives: they
+ * differentiate its time loop (src/main.F90:290-329) as this library performs it.
+ *
+ * Objective: J = (1/nsteps) sum_{n=1..nsteps} (1/|B|) sum_{x in B} 0.5 rho |u|^2 (x, n), the
+ * time-mean kinetic energy over box B of the moments after each step (macrovars).  box =
+ * {i_lo, i_hi, j_lo, j_hi, k_lo, k_hi}, 1-based, inclusive.
+ *
+ * lbm_objective_run advances steps it0+1 .. it0+nsteps and returns J.
+ *
+ * lbm_adjoint_run does the same and also returns, by reverse-mode differentiation with a
+ * checkpoint every `every` steps and recomputation in between:
+ *   grad_force  dJ/d external_forcing, layout (3,0:nx+1,0:ny+1,0:nz+1) (ghost cells 0); needs
+ *               lbm_set_external_forcing (else zeros); may be NULL;
+ *   grad_shear  dJ/d uvel_shear(k), k = 1..nz (inflow/outflow i only, else zeros); may be NULL.
+ * The context then holds the state after the nsteps steps, as after lbm_step.  Needs FP32 or
+ * FP64 storage, a stepped state (not the first step after set_state / init_equilibrium), no
+ * inflow turbulence, and at most one closed axis.  Device memory: (nsteps/every + 4) state
+ * pairs + every single states.  Adjoint arithmetic in the storage precision, gradient
+ * accumulation in FP64.
+ */
+int lbm_objective_run(lbm_ctx* ctx, int it0, int nsteps, const int32_t* box, double* J);
+int lbm_adjoint_run(lbm_ctx* ctx, int it0, int nsteps, int every, const int32_t* box, double* J,
+                    double* grad_force, double* grad_shear);
+
 #ifdef __cplusplus
 }
 #endif

@@ -22,7 +22,8 @@ FLANG = os.environ.get("FLANG", "/opt/rocm/bin/amdflang")
 FORTRAN = os.path.join(HERE, "fortran")
 ARCH = "gfx950"
 
-SOURCES = ["kernels_f32.hip", "kernels_f64.hip", "kernels_f16s.hip", "kernels_bf16s.hip", "api.cpp", "ensemble.cpp"]
+SOURCES = ["kernels_f32.hip", "kernels_f64.hip", "kernels_f16s.hip", "kernels_bf16s.hip", "api.cpp", "ensemble.cpp",
+           "adjoint.cpp"]
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wall", "-Wno-unused-function", "-I", CSRC, "-I", os.path.join(os.path.dirname(HERE), "include")]
@@ -52,6 +53,10 @@ def build(verbose: bool = False) -> str:
         raise RuntimeError("hipcc failed for " + ", ".join(failed))
     # calibration copy kernel (plain copy of a known size), a separate executable
     subprocess.run([HIPCC] + FLAGS + ["-o", os.path.join(BUILD, "copy_bw"), os.path.join(CSRC, "copy_bw.hip")],
+                   check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    # host-only check of the per-cell adjoint against differences of the collision (no GPU use)
+    subprocess.run([HIPCC] + FLAGS + ["-x", "hip", "-o", os.path.join(BUILD, "adjoint_check"),
+                                      os.path.join(CSRC, "adjoint_check.cpp")],
                    check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs

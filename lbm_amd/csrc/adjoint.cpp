@@ -1,0 +1,218 @@
+// Objective and adjoint entry points (lbm_objective_run, lbm_adjoint_run in include/lbm_hip.h).
+// No counterpart in the reference: they differentiate its time loop.
+//
+// The adjoint run advances the context nsteps steps from its current state, storing the full
+// buffer pair every `every` steps (checkpoints).  The reverse sweep walks the segments from last
+// to first: it restores the segment's checkpoint, recomputes the segment's states (kept as full
+// buffers, ghosts included: they are the inputs of the steps to reverse), and reverses the steps
+// with the kernels of adjoint.h.  At the end the forward state after nsteps is restored, so the
+// context continues as if lbm_step had been called.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/lbm_hip.h"
+#include "ctx.h"
+#include "impl.h"
+
+using namespace lbm;
+using namespace lbm_host;
+
+namespace {
+
+int check_box(lbm_ctx* c, const int32_t* box, AdjBox* b) {
+    if (!box) return fail(c, LBM_EINVAL, "null box");
+    const int n[3] = {c->d.nx, c->d.ny, c->d.nz};
+    for (int d = 0; d < 3; ++d) {
+        b->lo[d] = box[2 * d];
+        b->hi[d] = box[2 * d + 1];
+        if (b->lo[d] < 1 || b->hi[d] > n[d] || b->lo[d] > b->hi[d])
+            return fail(c, LBM_EINVAL, "box must satisfy 1 <= lo <= hi <= n on every axis");
+    }
+    return LBM_OK;
+}
+
+double box_cells(const AdjBox& b) {
+    return (double)(b.hi[0] - b.lo[0] + 1) * (b.hi[1] - b.lo[1] + 1) * (b.hi[2] - b.lo[2] + 1);
+}
+
+int copy_dev(lbm_ctx* c, void* dst, const void* src, size_t n) {
+    HIPCHK(c, hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToDevice, c->stream));
+    return LBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_objective_run(lbm_ctx* c, int it0, int nsteps, const int32_t* box, double* J) {
+    if (!c || !J) return fail(c, LBM_EINVAL, "null argument");
+    if (nsteps < 1) return fail(c, LBM_EINVAL, "nsteps must be >= 1");
+    AdjBox b;
+    if (int rc = check_box(c, box, &b)) return rc;
+    DevBuf acc;
+    if (int rc = dev_alloc(c, acc, sizeof(double))) return rc;
+    int rc = LBM_OK;
+    if (hipMemsetAsync(acc.p, 0, sizeof(double), c->stream) != hipSuccess) rc = fail(c, LBM_EHIP, "hipMemsetAsync");
+    for (int n = 1; n <= nsteps && rc == LBM_OK; ++n) {
+        if ((rc = lbm_step(c, it0 + n - 1, 1))) break;
+        const hipError_t e = c->ops->objective(c->lay, c->buf[c->cur], b, static_cast<double*>(acc.p), c->stream);
+        if (e != hipSuccess) rc = hipfail(c, e, "objective kernel");
+    }
+    double s = 0.0;
+    if (rc == LBM_OK) {
+        if (hipMemcpyAsync(&s, acc.p, sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+            rc = fail(c, LBM_EHIP, "objective read-back");
+    }
+    dev_free(acc);
+    if (rc == LBM_OK) *J = s / ((double)nsteps * box_cells(b));
+    return rc;
+}
+
+int lbm_adjoint_run(lbm_ctx* c, int it0, int nsteps, int every, const int32_t* box, double* J, double* grad_force,
+                    double* grad_shear) {
+    if (!c || !J) return fail(c, LBM_EINVAL, "null argument");
+    const lbm_desc& d = c->d;
+    if (!c->have_state) return fail(c, LBM_ESTATE, "no state");
+    if (c->pending || c->use_ext)
+        return fail(c, LBM_ESTATE, "the adjoint starts from a stepped state: call lbm_step once after set_state / init_equilibrium");
+    if (!c->ops->adj_step) return fail(c, LBM_EINVAL, "the adjoint needs FP32 or FP64 storage");
+    if (c->has_turb) return fail(c, LBM_EINVAL, "the adjoint does not differentiate the inflow-turbulence forcing: switch it off");
+    const int closed = (d.ibnd > 10) + (d.jbnd > 10) + (d.kbnd > 10);
+    if (closed > 1 || (closed == 1 && c->fold_dir < 0))
+        return fail(c, LBM_EINVAL, "the adjoint supports at most one closed axis (its face pass runs inside the step kernel)");
+    if (nsteps < 1 || every < 1) return fail(c, LBM_EINVAL, "nsteps and every must be >= 1");
+    AdjBox b;
+    if (int rc = check_box(c, box, &b)) return rc;
+
+    const size_t bb = c->buf_bytes;
+    const int nck = (nsteps + every - 1) / every;
+    const size_t plane = (size_t)c->lay.plane;
+    DevBuf ck, snaps, lam, fin, gF, gS, acc;
+    auto cleanup = [&]() {
+        for (DevBuf* x : {&ck, &snaps, &lam, &fin, &gF, &gS, &acc}) dev_free(*x);
+    };
+    int rc = LBM_OK;
+    if ((rc = dev_alloc(c, ck, (size_t)nck * 2 * bb)) || (rc = dev_alloc(c, snaps, (size_t)every * bb)) ||
+        (rc = dev_alloc(c, lam, 2 * bb)) || (rc = dev_alloc(c, fin, 2 * bb)) ||
+        (rc = dev_alloc(c, acc, sizeof(double))) || (rc = dev_alloc(c, gS, (size_t)d.ny * d.nz * sizeof(double))) ||
+        (grad_force && c->has_force && (rc = dev_alloc(c, gF, 3 * plane * sizeof(double))))) {
+        cleanup();
+        return rc;
+    }
+    char* ckp = static_cast<char*>(ck.p);
+    char* snp = static_cast<char*>(snaps.p);
+    char* lmp = static_cast<char*>(lam.p);
+    std::vector<int> ck_cur(nck, 0);
+    auto run = [&]() -> int {
+        HIPCHK(c, hipMemsetAsync(acc.p, 0, sizeof(double), c->stream));
+        // ---- forward sweep with checkpoints and the objective
+        for (int n = 1; n <= nsteps; ++n) {
+            if ((n - 1) % every == 0) {
+                const int q = (n - 1) / every;
+                ck_cur[q] = c->cur;
+                if (int r = copy_dev(c, ckp + (size_t)q * 2 * bb, c->buf[0], bb)) return r;
+                if (int r = copy_dev(c, ckp + (size_t)q * 2 * bb + bb, c->buf[1], bb)) return r;
+            }
+            if (int r = lbm_step(c, it0 + n - 1, 1)) return r;
+            HIPCHK(c, c->ops->objective(c->lay, c->buf[c->cur], b, static_cast<double*>(acc.p), c->stream));
+        }
+        const int fin_cur = c->cur;
+        const bool fin_raw0 = c->ghost_raw[0], fin_raw1 = c->ghost_raw[1];
+        if (int r = copy_dev(c, fin.p, c->buf[0], bb)) return r;
+        if (int r = copy_dev(c, static_cast<char*>(fin.p) + bb, c->buf[1], bb)) return r;
+
+        // ---- reverse sweep
+        HIPCHK(c, hipMemsetAsync(lam.p, 0, 2 * bb, c->stream));
+        HIPCHK(c, hipMemsetAsync(gS.p, 0, (size_t)d.ny * d.nz * sizeof(double), c->stream));
+        if (gF.p) HIPCHK(c, hipMemsetAsync(gF.p, 0, 3 * plane * sizeof(double), c->stream));
+        const double kappa = 1.0 / ((double)nsteps * box_cells(b));
+        StepHost h = step_host(c);
+        h.stream = 1;
+        h.tau_out = nullptr;
+        for (int q = 0; q < 4; ++q) h.ext[q] = nullptr;
+        h.turb[0] = h.turb[1] = h.turb[2] = nullptr;
+        // the objective's last term: moments of X(N)
+        h.src = c->buf[fin_cur];
+        HIPCHK(c, c->ops->adj_step(c->lay, h, nullptr, lmp + (size_t)fin_cur * bb, b, kappa, 0, nullptr, c->stream));
+        int opt_lo = 0, opt_hi = 0;
+        if (c->fold_dir >= 0) {
+            const int code = c->fold_dir == 0 ? d.ibnd : c->fold_dir == 1 ? d.jbnd : d.kbnd;
+            opts_of(code, &opt_lo, &opt_hi);
+        }
+        for (int seg = nck - 1; seg >= 0; --seg) {
+            const int n0 = seg * every, n1 = std::min(n0 + every, nsteps);
+            if (int r = copy_dev(c, c->buf[0], ckp + (size_t)seg * 2 * bb, bb)) return r;
+            if (int r = copy_dev(c, c->buf[1], ckp + (size_t)seg * 2 * bb + bb, bb)) return r;
+            c->cur = ck_cur[seg];
+            c->pending = false;
+            c->use_ext = false;
+            // the inputs X(n0) .. X(n1-1) of the segment's steps
+            if (int r = copy_dev(c, snp, c->buf[c->cur], bb)) return r;
+            for (int n = n0 + 1; n < n1; ++n) {
+                if (int r = lbm_step(c, it0 + n - 1, 1)) return r;
+                if (int r = copy_dev(c, snp + (size_t)(n - n0) * bb, c->buf[c->cur], bb)) return r;
+            }
+            for (int n = n1; n > n0; --n) {
+                const int bd = ck_cur[seg] ^ ((n - n0) & 1), bs = bd ^ 1;
+                char* ly = lmp + (size_t)bd * bb;
+                char* lx = lmp + (size_t)bs * bb;
+                if (d.ibnd == 1) {
+                    double ut, cs, sn;
+                    inflow_values(c, it0 + n, &ut, &cs, &sn);
+                    HIPCHK(c, c->ops->adj_inflow(c->lay, ly, ut, cs, sn, d.rho0, static_cast<double*>(gS.p), c->stream));
+                }
+                if (c->fold_dir >= 0) HIPCHK(c, c->ops->adj_face(c->lay, ly, lx, c->fold_dir, opt_lo, opt_hi, c->stream));
+                h.src = snp + (size_t)(n - 1 - n0) * bb;
+                HIPCHK(c, c->ops->adj_step(c->lay, h, ly, lx, b, n - 1 >= 1 ? kappa : 0.0, 1, static_cast<double*>(gF.p),
+                                           c->stream));
+            }
+        }
+        // back to the forward state after nsteps
+        if (int r = copy_dev(c, c->buf[0], fin.p, bb)) return r;
+        if (int r = copy_dev(c, c->buf[1], static_cast<char*>(fin.p) + bb, bb)) return r;
+        c->cur = fin_cur;
+        c->ghost_raw[0] = fin_raw0;
+        c->ghost_raw[1] = fin_raw1;
+        // results
+        double s = 0.0;
+        std::vector<double> hs((size_t)d.ny * d.nz), hf;
+        HIPCHK(c, hipMemcpyAsync(&s, acc.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(hs.data(), gS.p, hs.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        if (gF.p) {
+            hf.resize(3 * plane);
+            HIPCHK(c, hipMemcpyAsync(hf.data(), gF.p, hf.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        }
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        *J = s * kappa;
+        if (grad_shear)
+            for (int k = 0; k < d.nz; ++k) {
+                double t = 0.0;
+                for (int j = 0; j < d.ny; ++j) t += hs[(size_t)k * d.ny + j];
+                grad_shear[k] = t;
+            }
+        if (grad_force) {
+            const size_t nxp = d.nx + 2, nyp = d.ny + 2, nzp = d.nz + 2;
+            std::fill(grad_force, grad_force + 3 * nxp * nyp * nzp, 0.0);
+            if (gF.p)
+                for (int k = 1; k <= d.nz; ++k)
+                    for (int j = 1; j <= d.ny; ++j)
+                        for (int i = 1; i <= d.nx; ++i) {
+                            const size_t o = c->lay.cell(i, j, k);
+                            double* g = grad_force + 3 * ((k * nyp + j) * nxp + i);
+                            g[0] = hf[o];
+                            g[1] = hf[plane + o];
+                            g[2] = hf[2 * plane + o];
+                        }
+        }
+        return LBM_OK;
+    };
+    rc = run();
+    cleanup();
+    return rc;
+}
+
+}  // extern "C"

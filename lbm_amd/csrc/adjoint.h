@@ -1,0 +1,436 @@
+// Hand-written adjoint of the time step (reverse mode), for gradients of a time-mean kinetic
+// energy functional with respect to the external body force and the inflow shear profile.
+//
+// The forward step n reads buffer X(n-1) (interior and ghosts) and writes buffer X(n): the
+// interior by pull-streaming + collision (step_cell), then its ghosts by the folded closed-face
+// pass and the inflow/outflow pass.  Ghosts that no pass rewrites keep the value the buffer held
+// two steps earlier.  The adjoint keeps one adjoint buffer per forward buffer (same layout) and
+// reverses step n as the sequence of writes in reverse order:
+//   1. inflow/outflow pass  -> adjoint_inflow_kernel
+//   2. closed-face pass     -> adjoint_face_kernel
+//   3. stream + collide     -> adjoint_step_kernel
+// Every write "y = f(x)" becomes "lambda_x += (df/dx)^T lambda_y; lambda_y = 0" for the slots it
+// overwrote.  The pull gather is a permutation of the source slots it reads, so its reverse is a
+// conflict-free scatter: interior slots of the source buffer are assigned, ghost slots (which may
+// already hold adjoint from the face pass) accumulate, and interior slots that no cell pulls are
+// cleared by their own cell.
+//
+// collide_adjoint() is the reverse of collide_cell() (kernels.h) for one cell: the transposed
+// Jacobian of moments -> equilibrium -> regularized non-equilibrium (Malaspinas recursion) ->
+// Vreman eddy viscosity -> relaxation -> Kupershtokh body-force increment -> solid swap,
+// applied to the adjoint of the post-collision populations.  It is host-callable so that it can
+// be checked against central differences of collide_cell on the CPU.
+#pragma once
+
+#include <cmath>
+
+#include "kernels.h"
+
+namespace lbm {
+
+// Adjoint of one cell.  D: departures g_l - w_l of the streamed (pre-collision) populations;
+// ly: adjoint of the post-collision populations; kappa: weight of the kinetic energy
+// 0.5*rho*|u|^2 of g in the objective (0: not in the objective box); coll = false: objective term
+// only.  Out: lg (adjoint of g), lF (adjoint of the body force of this cell).
+template <int Q, bool BGK3, bool HRR, bool VRE, typename R>
+__host__ __device__ inline void collide_adjoint(const StepArgs<R>& a, const R (&D)[Q], int k, int oc, bool solid,
+                                                const R (&ly_in)[Q], R kappa, bool coll, R (&lg)[Q], R (&lF)[3]) {
+    using Lt = Lattice<Q>;
+    // ---- forward recompute (collide_cell's quantities, in plain rather than paired form)
+    R dr = 0, jx = 0, jy = 0, jz = 0, pxx = 0, pyy = 0, pzz = 0, pxy = 0, pxz = 0, pyz = 0;
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+        const R cx = R(Lt::cx(l)), cy = R(Lt::cy(l)), cz = R(Lt::cz(l));
+        dr += D[l];
+        jx += cx * D[l];
+        jy += cy * D[l];
+        jz += cz * D[l];
+        pxx += cx * cx * D[l];
+        pyy += cy * cy * D[l];
+        pzz += cz * cz * D[l];
+        pxy += cx * cy * D[l];
+        pxz += cx * cz * D[l];
+        pyz += cy * cz * D[l];
+    }
+    const R rho = R(1) + dr;
+    const R inv = R(1) / rho;
+    const R ux = jx * inv, uy = jy * inv, uz = jz * inv;
+    const R usq = ux * ux + uy * uy + uz * uz;
+    lF[0] = lF[1] = lF[2] = R(0);
+    if (!coll) {
+        // objective only: d(0.5 |j|^2 / rho)/dg_l = c.u - |u|^2/2
+#pragma unroll
+        for (int l = 0; l < Q; ++l)
+            lg[l] = kappa * (sdot<R>(Lt::cx(l), Lt::cy(l), Lt::cz(l), ux, uy, uz) - R(0.5) * usq);
+        return;
+    }
+    const R third = R(1) / R(3);
+    const R axx = pxx - rho * ux * ux - dr * third, ayy = pyy - rho * uy * uy - dr * third,
+            azz = pzz - rho * uz * uz - dr * third;
+    const R axy = pxy - rho * ux * uy, axz = pxz - rho * ux * uz, ayz = pyz - rho * uy * uz;
+    R fac = a.fac0, tau = 0, amag = 0, amag_raw = 0, t = 0, b11 = 0, b22 = 0, b33 = 0, b12 = 0, b13 = 0, b23 = 0;
+    if (VRE) {
+        amag_raw = axx * axx + ayy * ayy + azz * azz + R(2) * (axy * axy + axz * axz + ayz * ayz);
+        amag = amag_raw > a.tiny ? amag_raw : a.tiny;
+        b11 = a.eps + (axx * axx + axy * axy + axz * axz);
+        b22 = a.eps + (axy * axy + ayy * ayy + ayz * ayz);
+        b33 = a.eps + (axz * axz + ayz * ayz + azz * azz);
+        b12 = a.eps + (axx * axy + axy * ayy + axz * ayz);
+        b13 = a.eps + (axx * axz + axy * ayz + axz * azz);
+        b23 = a.eps + (axy * axz + ayy * ayz + ayz * azz);
+        const R bb = b11 * b22 - b12 * b12 + b11 * b33 - b13 * b13 + b22 * b33 - b23 * b23;
+        t = bb / amag;
+        const R eddy = t > a.eps ? a.vconst * std::sqrt(t) : R(0);
+        tau = R(3) * (a.kinevisc + eddy + a.ablvisc[k - 1]) + R(0.5);
+        fac = R(1) - R(1) / tau;
+    }
+    const R T = axx + ayy + azz;
+    const R gx = axx * ux + axy * uy + axz * uz, gy = axy * ux + ayy * uy + ayz * uz, gz = axz * ux + ayz * uy + azz * uz;
+
+    // ---- reverse
+    R ly[Q];
+#pragma unroll
+    for (int l = 0; l < Q; ++l) ly[l] = ly_in[l];
+    if (solid) {
+#pragma unroll
+        for (int l = 1; l + 1 < Q; l += 2) {
+            const R s = ly[l];
+            ly[l] = ly[l + 1];
+            ly[l + 1] = s;
+        }
+    }
+    R lrho = 0, lux = 0, luy = 0, luz = 0;
+    R lxx = 0, lyy = 0, lzz = 0, lxy = 0, lxz = 0, lyz = 0;   // adjoint of a2
+    R lfac = 0;
+
+    // body force: y += feq2(rho, v) - feq2(rho, u), v = u - F/rho (skipped when rho <= 0)
+    if (a.Fx != nullptr && rho > R(0)) {
+        const R Fx = a.Fx[oc], Fy = a.Fy[oc], Fz = a.Fz[oc];
+        const R vx = ux - Fx * inv, vy = uy - Fy * inv, vz = uz - Fz * inv;
+        const R vsq = vx * vx + vy * vy + vz * vz;
+        R lvx = 0, lvy = 0, lvz = 0;
+#pragma unroll
+        for (int l = 0; l < Q; ++l) {
+            const int cx = Lt::cx(l), cy = Lt::cy(l), cz = Lt::cz(l);
+            const R w = weight<Q, R>(l);
+            const R cv = sdot<R>(cx, cy, cz, vx, vy, vz), cu = sdot<R>(cx, cy, cz, ux, uy, uz);
+            const R Pv = R(1) - R(1.5) * vsq + R(3) * cv + R(4.5) * cv * cv;
+            const R Pu = R(1) - R(1.5) * usq + R(3) * cu + R(4.5) * cu * cu;
+            const R wl = w * ly[l];
+            lrho += wl * (Pv - Pu);
+            // grad P2(v) = -3 v + 3 c + 9 (c.v) c
+            const R gvx = R(-3) * vx + R(3) * R(cx) + R(9) * cv * R(cx);
+            const R gvy = R(-3) * vy + R(3) * R(cy) + R(9) * cv * R(cy);
+            const R gvz = R(-3) * vz + R(3) * R(cz) + R(9) * cv * R(cz);
+            const R gux = R(-3) * ux + R(3) * R(cx) + R(9) * cu * R(cx);
+            const R guy = R(-3) * uy + R(3) * R(cy) + R(9) * cu * R(cy);
+            const R guz = R(-3) * uz + R(3) * R(cz) + R(9) * cu * R(cz);
+            lvx += wl * rho * gvx;
+            lvy += wl * rho * gvy;
+            lvz += wl * rho * gvz;
+            lux -= wl * rho * gux;
+            luy -= wl * rho * guy;
+            luz -= wl * rho * guz;
+        }
+        lux += lvx;
+        luy += lvy;
+        luz += lvz;
+        // dv/drho = F / rho^2, dv/dF = -1/rho
+        lrho += (lvx * Fx + lvy * Fy + lvz * Fz) * inv * inv;
+        lF[0] = -lvx * inv;
+        lF[1] = -lvy * inv;
+        lF[2] = -lvz * inv;
+    }
+
+    // relaxation y = feq + fac * fneq
+    R lgd[Q];
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+        const int cx = Lt::cx(l), cy = Lt::cy(l), cz = Lt::cz(l);
+        const R w = weight<Q, R>(l);
+        const R cu = sdot<R>(cx, cy, cz, ux, uy, uz);
+        R P = R(1) - R(1.5) * usq + R(3) * cu + R(4.5) * cu * cu;
+        // grad P = -3u + 3c + 9 cu c [+ 4.5 ((3 cu^2 - u.u) c - 2 cu u)]
+        R gpx = R(-3) * ux + R(3) * R(cx) + R(9) * cu * R(cx);
+        R gpy = R(-3) * uy + R(3) * R(cy) + R(9) * cu * R(cy);
+        R gpz = R(-3) * uz + R(3) * R(cz) + R(9) * cu * R(cz);
+        if (BGK3 && l > 0) {
+            P += R(4.5) * cu * (cu * cu - usq);
+            const R e = R(4.5) * (R(3) * cu * cu - usq), f2 = R(9) * cu;
+            gpx += e * R(cx) - f2 * ux;
+            gpy += e * R(cy) - f2 * uy;
+            gpz += e * R(cz) - f2 * uz;
+        }
+        const R feq = w * rho * P;
+        if (HRR) {
+            R cac = R(0);
+            if (cx != 0) cac += axx;
+            if (cy != 0) cac += ayy;
+            if (cz != 0) cac += azz;
+            cac += R(2) * sdot<R>(cx * cy, cx * cz, cy * cz, axy, axz, ayz);
+            const R cg = sdot<R>(cx, cy, cz, gx, gy, gz);
+            const R w45 = R(4.5) * w;
+            const R fneq = w45 * ((cac - T * third) + (cu * (R(3) * cac - T) - R(2) * cg));
+            const R le = ly[l];
+            lrho += le * w * P;
+            lux += le * w * rho * gpx;
+            luy += le * w * rho * gpy;
+            luz += le * w * rho * gpz;
+            lfac += le * fneq;
+            const R phi = fac * le * w45;
+            const R fx = R(cx), fy = R(cy), fz = R(cz);
+            // d fneq / d a2 (six symmetric entries)
+            lxx += phi * ((fx * fx - third) + cu * (R(3) * fx * fx - R(1)) - R(2) * fx * ux);
+            lyy += phi * ((fy * fy - third) + cu * (R(3) * fy * fy - R(1)) - R(2) * fy * uy);
+            lzz += phi * ((fz * fz - third) + cu * (R(3) * fz * fz - R(1)) - R(2) * fz * uz);
+            lxy += phi * (R(2) * fx * fy + cu * (R(6) * fx * fy) - R(2) * (fx * uy + fy * ux));
+            lxz += phi * (R(2) * fx * fz + cu * (R(6) * fx * fz) - R(2) * (fx * uz + fz * ux));
+            lyz += phi * (R(2) * fy * fz + cu * (R(6) * fy * fz) - R(2) * (fy * uz + fz * uy));
+            // d fneq / d u = c (3 cAc - T) - 2 a2 c
+            const R s3 = R(3) * cac - T;
+            lux += phi * (fx * s3 - R(2) * (axx * fx + axy * fy + axz * fz));
+            luy += phi * (fy * s3 - R(2) * (axy * fx + ayy * fy + ayz * fz));
+            luz += phi * (fz * s3 - R(2) * (axz * fx + ayz * fy + azz * fz));
+            lgd[l] = R(0);
+        } else {
+            // y = feq + fac (g - feq)
+            const R le = ly[l];
+            const R lfe = (R(1) - fac) * le;
+            lrho += lfe * w * P;
+            lux += lfe * w * rho * gpx;
+            luy += lfe * w * rho * gpy;
+            luz += lfe * w * rho * gpz;
+            lfac += le * ((w + D[l]) - feq);
+            lgd[l] = fac * le;
+        }
+    }
+
+    if (VRE) {
+        // fac = 1 - 1/tau, tau = 3 (nu + nu_t + abl) + 1/2, nu_t = c sqrt(t) (t > eps)
+        const R ltau = lfac / (tau * tau);
+        const R lnut = R(3) * ltau;
+        const R lt = t > a.eps ? lnut * a.vconst / (R(2) * std::sqrt(t)) : R(0);
+        const R lbb = lt / amag;
+        const R lam = amag_raw > a.tiny ? -lt * t / amag : R(0);
+        const R l11 = lbb * (b22 + b33), l22 = lbb * (b11 + b33), l33 = lbb * (b11 + b22);
+        const R l12 = R(-2) * b12 * lbb, l13 = R(-2) * b13 * lbb, l23 = R(-2) * b23 * lbb;
+        lxx += R(2) * axx * l11 + axy * l12 + axz * l13 + R(2) * axx * lam;
+        lyy += R(2) * ayy * l22 + axy * l12 + ayz * l23 + R(2) * ayy * lam;
+        lzz += R(2) * azz * l33 + axz * l13 + ayz * l23 + R(2) * azz * lam;
+        lxy += R(2) * axy * (l11 + l22) + (axx + ayy) * l12 + ayz * l13 + axz * l23 + R(4) * axy * lam;
+        lxz += R(2) * axz * (l11 + l33) + ayz * l12 + (axx + azz) * l13 + axy * l23 + R(4) * axz * lam;
+        lyz += R(2) * ayz * (l22 + l33) + axz * l12 + axy * l13 + (ayy + azz) * l23 + R(4) * ayz * lam;
+    }
+
+    // a2 = Pi' - rho u u - (rho - 1) I/3
+    lrho -= lxx * ux * ux + lyy * uy * uy + lzz * uz * uz + lxy * ux * uy + lxz * ux * uz + lyz * uy * uz;
+    lrho -= (lxx + lyy + lzz) * third;
+    lux -= rho * (R(2) * ux * lxx + uy * lxy + uz * lxz);
+    luy -= rho * (R(2) * uy * lyy + ux * lxy + uz * lyz);
+    luz -= rho * (R(2) * uz * lzz + ux * lxz + uy * lyz);
+    // u = j / rho
+    const R ljx = lux * inv, ljy = luy * inv, ljz = luz * inv;
+    lrho -= (lux * ux + luy * uy + luz * uz) * inv;
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+        const int cx = Lt::cx(l), cy = Lt::cy(l), cz = Lt::cz(l);
+        R v = lgd[l] + lrho + sdot<R>(cx, cy, cz, ljx, ljy, ljz);
+        if (cx != 0) v += lxx;
+        if (cy != 0) v += lyy;
+        if (cz != 0) v += lzz;
+        v += sdot<R>(cx * cy, cx * cz, cy * cz, lxy, lxz, lyz);
+        if (kappa != R(0)) v += kappa * (sdot<R>(cx, cy, cz, ux, uy, uz) - R(0.5) * usq);
+        lg[l] = v;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Reverse of the stream + collide kernel for step n.  src: forward buffer X(n-1) (the step's
+// input); lam_y: adjoint buffer of X(n) (read, interior); lam_x: adjoint buffer of X(n-1)
+// (interior slots assigned, ghost slots accumulated).  box: objective cells (inclusive, 1-based);
+// kappa: objective weight of the moments of X(n-1) streamed, i.e. this step's gathered input.
+// gF: FP64 accumulators of the body-force adjoint (3 cell planes) or nullptr.
+// ------------------------------------------------------------------------------------------
+template <int Q, class C, bool BGK3, bool HRR, bool VRE>
+__global__ void __launch_bounds__(256) adjoint_step_kernel(const StepArgs<typename C::real> a, const typename C::raw* lam_y,
+                                                           typename C::raw* lam_x, AdjBox box, typename C::real kappa,
+                                                           int coll, double* gF) {
+    using R = typename C::real;
+    using raw = typename C::raw;
+    using Lt = Lattice<Q>;
+    const Layout& L = a.lay;
+    const int ca = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int cb = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int cc = blockIdx.z + 1;
+    if (ca > L.n(L.ax0) || cb > L.n(L.ax1) || cc > L.n(L.ax2)) return;
+    int i, j, k;
+    L.ijk(ca, cb, cc, i, j, k);
+    const int x[3] = {i, j, k};
+    const int n[3] = {L.nx, L.ny, L.nz};
+    const int per[3] = {L.per_i, L.per_j, L.per_k};
+    const raw* __restrict__ src = static_cast<const raw*>(a.src);
+    R D[Q];
+    int so[Q];
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+        so[l] = L.cell(Layout::wrap(i - Lt::cx(l), L.nx, L.per_i), Layout::wrap(j - Lt::cy(l), L.ny, L.per_j),
+                       Layout::wrap(k - Lt::cz(l), L.nz, L.per_k));
+        D[l] = C::template dep<Q>(src[(size_t)l * L.plane + so[l]], l);
+    }
+    const int oc = L.cell(i, j, k);
+    R ly[Q];
+#pragma unroll
+    for (int l = 0; l < Q; ++l) ly[l] = coll ? C::template decode<Q>(lam_y[(size_t)l * L.plane + oc], l) : R(0);
+    bool solid = false;
+    if (a.solid != nullptr) {
+        const uint32_t word = a.solid[((cc - 1) * L.n(L.ax1) + (cb - 1)) * a.solid_wpr + ((ca - 1) >> 5)];
+        solid = (word >> ((ca - 1) & 31)) & 1u;
+    }
+    const bool inbox = i >= box.lo[0] && i <= box.hi[0] && j >= box.lo[1] && j <= box.hi[1] && k >= box.lo[2] && k <= box.hi[2];
+    R lg[Q], lF[3];
+    collide_adjoint<Q, BGK3, HRR, VRE, R>(a, D, k, oc, solid, ly, inbox ? kappa : R(0), coll != 0, lg, lF);
+    raw* __restrict__ lx = lam_x;
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+        // the source slot: interior -> assign; ghost -> accumulate
+        bool ghost = false, unhit = false;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            if (per[d]) continue;
+            const int cd = Lt::c(l, d);
+            const int s = x[d] - cd, t = x[d] + cd;
+            if (s < 1 || s > n[d]) ghost = true;
+            if (t < 1 || t > n[d]) unhit = true;
+        }
+        raw* p = lx + (size_t)l * L.plane + so[l];
+        if (ghost) *p = C::template encode<Q>(C::template decode<Q>(*p, l) + lg[l], l);
+        else *p = C::template encode<Q>(lg[l], l);
+        // this cell's own slot of direction l is pulled by no interior cell: clear it
+        if (unhit) lx[(size_t)l * L.plane + oc] = C::template encode<Q>(R(0), l);
+    }
+    if (gF != nullptr && a.Fx != nullptr) {
+        gF[oc] += (double)lF[0];
+        gF[(size_t)L.plane + oc] += (double)lF[1];
+        gF[2 * (size_t)L.plane + oc] += (double)lF[2];
+    }
+}
+
+// Reverse of the inflow/outflow pass (inflow_cell) of step n on the adjoint buffer lam of X(n).
+// gS: FP64 accumulators of d/d uvel_shear per (j,k) (j fastest).
+template <int Q, class C>
+__global__ void adjoint_inflow_kernel(Layout L, typename C::raw* lam, typename C::real utime, typename C::real cosd,
+                                      typename C::real sind, typename C::real rho0, double* gS) {
+    using R = typename C::real;
+    using Lt = Lattice<Q>;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int k = blockIdx.y + 1;
+    if (j > L.ny || k > L.nz) return;
+    const int o1 = L.cell(1, j, k), o0 = L.cell(0, j, k), on = L.cell(L.nx, j, k), oe = L.cell(L.nx + 1, j, k);
+    R l0[Q], le[Q], lfg[Q];
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+        l0[l] = C::template decode<Q>(lam[(size_t)l * L.plane + o0], l);
+        le[l] = C::template decode<Q>(lam[(size_t)l * L.plane + oe], l);
+        lfg[l] = R(0);
+    }
+#pragma unroll
+    for (int l = 0; l < Q; ++l) lfg[Lt::cx(l) <= 0 ? l : Lt::opposite(l)] += l0[l];
+    R s = R(0);
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+        // fg_l = f_l - 2 w rho0 (cx cos + cy sin) uu / cs2,  uu = utime * shear(k)
+        s += lfg[l] * (R(-6) * weight<Q, R>(l) * rho0 * (R(Lt::cx(l)) * cosd + R(Lt::cy(l)) * sind));
+        lam[(size_t)l * L.plane + o1] = C::template encode<Q>(C::template decode<Q>(lam[(size_t)l * L.plane + o1], l) + lfg[l], l);
+        lam[(size_t)l * L.plane + on] = C::template encode<Q>(C::template decode<Q>(lam[(size_t)l * L.plane + on], l) + le[l], l);
+        lam[(size_t)l * L.plane + o0] = C::template encode<Q>(R(0), l);
+        lam[(size_t)l * L.plane + oe] = C::template encode<Q>(R(0), l);
+    }
+    if (gS != nullptr) gS[(size_t)(k - 1) * L.ny + (j - 1)] += (double)(s * utime);
+}
+
+// Reverse of the closed-face pass of axis D (fold_closed_face) of step n: lam_y adjoint of X(n),
+// lam_x adjoint of X(n-1).  One thread per ghost cell of the two faces (tangential coordinates
+// interior), t0/t1 the tangential axes.
+template <int Q, class C>
+__global__ void adjoint_face_kernel(Layout L, typename C::raw* lam_y, typename C::raw* lam_x, int D, int t0, int t1,
+                                    int opt_lo, int opt_hi) {
+    using R = typename C::real;
+    using Lt = Lattice<Q>;
+    const int n[3] = {L.nx, L.ny, L.nz};
+    const int per[3] = {L.per_i, L.per_j, L.per_k};
+    const int a0 = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int a1 = blockIdx.y + 1;
+    const int side = blockIdx.z;
+    if (a0 > n[t0] || a1 > n[t1]) return;
+    int g[3];
+    g[t0] = a0;
+    g[t1] = a1;
+    g[D] = side ? n[D] + 1 : 0;
+    const int plane = side ? n[D] : 1;
+    const int opt = side ? opt_hi : opt_lo;
+    const int og = L.cell(g[0], g[1], g[2]);
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+        const int cn = Lt::c(l, D);
+        if (!(side ? cn > 0 : cn < 0)) continue;
+        const int m = opt < 0 ? mirror<Q>(l, D, -1) : mirror<Q>(l, D, +1);
+        const R lm = C::template decode<Q>(lam_y[(size_t)m * L.plane + og], m);
+        const R ll = C::template decode<Q>(lam_y[(size_t)l * L.plane + og], l);
+        // (m, G) was copied from X(n-1)'s (l, G)
+        lam_x[(size_t)l * L.plane + og] = C::template encode<Q>(C::template decode<Q>(lam_x[(size_t)l * L.plane + og], l) + lm, l);
+        // (l, G) came from the wall-plane cell G - c_l (tangential), an interior cell or a ghost of
+        // the inflow axis
+        int s[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            if (d == D) { s[d] = plane; continue; }
+            s[d] = g[d] - Lt::c(l, d);
+            if (per[d]) s[d] = Layout::wrap(s[d], n[d], 1);
+        }
+        const size_t os = (size_t)l * L.plane + L.cell(s[0], s[1], s[2]);
+        lam_y[os] = C::template encode<Q>(C::template decode<Q>(lam_y[os], l) + ll, l);
+        lam_y[(size_t)m * L.plane + og] = C::template encode<Q>(R(0), m);
+        lam_y[(size_t)l * L.plane + og] = C::template encode<Q>(R(0), l);
+    }
+}
+
+// Objective: sum over the box of 0.5*rho*|u|^2 of the streamed populations of buffer f (the
+// reference's macrovars after a step), added to acc[0] (FP64).
+template <int Q, class C>
+__global__ void __launch_bounds__(256) objective_kernel(Layout L, const void* f, AdjBox box, double* acc) {
+    using R = typename C::real;
+    using raw = typename C::raw;
+    using Lt = Lattice<Q>;
+    const int bx = box.hi[0] - box.lo[0] + 1, by = box.hi[1] - box.lo[1] + 1, bz = box.hi[2] - box.lo[2] + 1;
+    const long long nb = (long long)bx * by * bz;
+    const raw* __restrict__ p = static_cast<const raw*>(f);
+    double sum = 0.0;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < nb; e += (long long)gridDim.x * blockDim.x) {
+        const int i = box.lo[0] + (int)(e % bx);
+        const int j = box.lo[1] + (int)((e / bx) % by);
+        const int k = box.lo[2] + (int)(e / ((long long)bx * by));
+        R dr = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+        for (int l = 0; l < Q; ++l) {
+            const R d = C::template dep<Q>(p[(size_t)l * L.plane + L.cell(Layout::wrap(i - Lt::cx(l), L.nx, L.per_i),
+                                                                           Layout::wrap(j - Lt::cy(l), L.ny, L.per_j),
+                                                                           Layout::wrap(k - Lt::cz(l), L.nz, L.per_k))], l) /
+                          C::kDepScale;
+            dr += d;
+            jx += R(Lt::cx(l)) * d;
+            jy += R(Lt::cy(l)) * d;
+            jz += R(Lt::cz(l)) * d;
+        }
+        sum += 0.5 * ((double)jx * jx + (double)jy * jy + (double)jz * jz) / (1.0 + (double)dr);
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = sum;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(acc, red[0]);
+}
+
+}  // namespace lbm

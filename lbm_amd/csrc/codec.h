@@ -22,6 +22,8 @@ namespace lbm {
 
 enum StorageKind : int { STORE_F32 = 0, STORE_F64 = 1, STORE_F16S = 2, STORE_BF16S = 3 };
 
+// The FP32 and FP64 codecs are also host-callable, so that the per-cell collision can run on the
+// CPU (the adjoint's finite-difference check).
 // Every codec also works in "departure space": dep() returns S*(f - w_l) and enc_dep() stores
 // w_l + dev/S, with S = kDepScale a power of two, so that the collision never forms f itself.
 struct CodecF32 {
@@ -30,16 +32,16 @@ struct CodecF32 {
     static constexpr int kind = STORE_F32;
     static constexpr float kDepScale = 1.0f;
     template <int Q>
-    __device__ static inline real dep(raw r, int l) { return r - weight<Q, real>(l); }
+    __host__ __device__ static inline real dep(raw r, int l) { return r - weight<Q, real>(l); }
     template <int Q>
-    __device__ static inline raw enc_dep(real dev, int l) { return weight<Q, real>(l) + dev; }
+    __host__ __device__ static inline raw enc_dep(real dev, int l) { return weight<Q, real>(l) + dev; }
     template <int Q>
-    __device__ static inline real decode(raw r, int) { return r; }
+    __host__ __device__ static inline real decode(raw r, int) { return r; }
     template <int Q>
-    __device__ static inline raw encode(real v, int) { return v; }
+    __host__ __device__ static inline raw encode(real v, int) { return v; }
     // value minus the lattice weight, computed in the compute type
     template <int Q>
-    __device__ static inline real shifted(raw r, int l) { return r - weight<Q, real>(l); }
+    __host__ __device__ static inline real shifted(raw r, int l) { return r - weight<Q, real>(l); }
 };
 
 struct CodecF64 {
@@ -48,15 +50,15 @@ struct CodecF64 {
     static constexpr int kind = STORE_F64;
     static constexpr double kDepScale = 1.0;
     template <int Q>
-    __device__ static inline real dep(raw r, int l) { return r - weight<Q, real>(l); }
+    __host__ __device__ static inline real dep(raw r, int l) { return r - weight<Q, real>(l); }
     template <int Q>
-    __device__ static inline raw enc_dep(real dev, int l) { return weight<Q, real>(l) + dev; }
+    __host__ __device__ static inline raw enc_dep(real dev, int l) { return weight<Q, real>(l) + dev; }
     template <int Q>
-    __device__ static inline real decode(raw r, int) { return r; }
+    __host__ __device__ static inline real decode(raw r, int) { return r; }
     template <int Q>
-    __device__ static inline raw encode(real v, int) { return v; }
+    __host__ __device__ static inline raw encode(real v, int) { return v; }
     template <int Q>
-    __device__ static inline real shifted(raw r, int l) { return r - weight<Q, real>(l); }
+    __host__ __device__ static inline real shifted(raw r, int l) { return r - weight<Q, real>(l); }
 };
 
 constexpr float kHalfScale = 1024.0f;

@@ -78,6 +78,11 @@ struct EnsLaunch {
     int bgk3, hrr, vre;
 };
 
+// Objective box (inclusive, 1-based interior indices i, j, k).
+struct AdjBox {
+    int lo[3], hi[3];
+};
+
 struct Ops {
     hipError_t (*step)(const Layout&, const StepHost&, hipStream_t);
     hipError_t (*closed)(const Layout&, void* f1, const void* f2, int dir, int opt_lo, int opt_hi, const void* snap,
@@ -97,6 +102,13 @@ struct Ops {
     size_t member_bytes;   // sizeof(MemberTab<R>)
     void (*ens_pack)(const MemberHost&, int block, void* out, int* tiles, int* inflow_tiles);
     hipError_t (*ens_step)(const EnsLaunch&, hipStream_t);
+    // adjoint (nullptr for the 16-bit storage codecs)
+    hipError_t (*adj_step)(const Layout&, const StepHost&, const void* lam_y, void* lam_x, const AdjBox&, double kappa,
+                           int coll, double* gF, hipStream_t);
+    hipError_t (*adj_inflow)(const Layout&, void* lam, double utime, double cosd, double sind, double rho0, double* gS,
+                             hipStream_t);
+    hipError_t (*adj_face)(const Layout&, void* lam_y, void* lam_x, int dir, int opt_lo, int opt_hi, hipStream_t);
+    hipError_t (*objective)(const Layout&, const void* f, const AdjBox&, double* acc, hipStream_t);
     int real_bytes;   // compute type size
     int raw_bytes;    // storage size
 };

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cmath>
 #include <type_traits>
 
 #include "codec.h"
@@ -56,7 +57,7 @@ struct StepArgs {
 
 // Signed sum  sx*a + sy*b + sz*c  for compile-time signs in {-1,0,1}: adds only.
 template <typename R>
-__device__ __forceinline__ R sdot(int sx, int sy, int sz, R a, R b, R c) {
+__host__ __device__ __forceinline__ R sdot(int sx, int sy, int sz, R a, R b, R c) {
     R s = 0;
     bool first = true;
     if (sx != 0) { s = sx > 0 ? a : -a; first = false; }
@@ -103,7 +104,7 @@ __host__ __device__ constexpr int mirror(int l, int d, int opt) {
 // Working on departures needs no decode/encode arithmetic for the shifted 16-bit formats and
 // forms rho = 1 + sum D/S, j = sum c D/S without the cancellation of sum f.
 template <int Q, class C, bool BGK3, bool HRR, bool VRE>
-__device__ __forceinline__ void collide_cell(const StepArgs<typename C::real>& a, const typename C::raw (&rv)[Q], int i,
+__host__ __device__ __forceinline__ void collide_cell(const StepArgs<typename C::real>& a, const typename C::raw (&rv)[Q], int i,
                                              int j, int k, int ca, int cb, int cc, int oc,
                                              typename C::real (&dev)[Q], typename C::real& tau_out) {
     using R = typename C::real;
@@ -178,7 +179,7 @@ __device__ __forceinline__ void collide_cell(const StepArgs<typename C::real>& a
         const R b23 = a.eps + (axy * axz + ayy * ayz + ayz * azz);
         const R bb = b11 * b22 - b12 * b12 + b11 * b33 - b13 * b13 + b22 * b33 - b23 * b23;
         const R t = bb / amag;
-        const R eddy = t > a.eps ? a.vconst * sqrt(t) : R(0);
+        const R eddy = t > a.eps ? a.vconst * std::sqrt(t) : R(0);
         tau = R(3) * (a.kinevisc + eddy + a.ablvisc[k - 1]) + R(0.5);
         fac = R(1) - R(1) / tau;
     }

@@ -3,10 +3,12 @@
 
 #include <cfloat>
 #include <cmath>
+#include <algorithm>
 #include <cstring>
 
 #include "impl.h"
 #include "kernels.h"
+#include "adjoint.h"
 
 namespace lbm {
 
@@ -247,10 +249,71 @@ struct Launch {
         return hipGetLastError();
     }
 
+    // ---- adjoint
+    static constexpr bool kAdj = sizeof(typename C::raw) >= 4;
+
+    template <bool B3, bool H, bool V>
+    static void adj_step_t(const Layout& L, const StepArgs<R>& a, const void* ly, void* lx, const AdjBox& box, R kappa,
+                           int coll, double* gF, hipStream_t s) {
+        if constexpr (kAdj) {
+            const dim3 block(64, 4, 1);
+            const dim3 grid((L.n(L.ax0) + 63) / 64, (L.n(L.ax1) + 3) / 4, L.n(L.ax2));
+            hipLaunchKernelGGL((adjoint_step_kernel<Q, C, B3, H, V>), grid, block, 0, s, a,
+                               static_cast<const typename C::raw*>(ly), static_cast<typename C::raw*>(lx), box, kappa,
+                               coll, gF);
+        }
+    }
+
+    static hipError_t adj_step(const Layout& L, const StepHost& h, const void* ly, void* lx, const AdjBox& box,
+                               double kappa, int coll, double* gF, hipStream_t s) {
+        const StepArgs<R> a = args_of(L, h);
+        const int key = (h.bgk3 ? 4 : 0) | (h.hrr ? 2 : 0) | (h.vre ? 1 : 0);
+        const R k = (R)kappa;
+        switch (key) {
+            case 0: adj_step_t<false, false, false>(L, a, ly, lx, box, k, coll, gF, s); break;
+            case 1: adj_step_t<false, false, true>(L, a, ly, lx, box, k, coll, gF, s); break;
+            case 2: adj_step_t<false, true, false>(L, a, ly, lx, box, k, coll, gF, s); break;
+            case 3: adj_step_t<false, true, true>(L, a, ly, lx, box, k, coll, gF, s); break;
+            case 4: adj_step_t<true, false, false>(L, a, ly, lx, box, k, coll, gF, s); break;
+            case 5: adj_step_t<true, false, true>(L, a, ly, lx, box, k, coll, gF, s); break;
+            case 6: adj_step_t<true, true, false>(L, a, ly, lx, box, k, coll, gF, s); break;
+            default: adj_step_t<true, true, true>(L, a, ly, lx, box, k, coll, gF, s); break;
+        }
+        return hipGetLastError();
+    }
+
+    static hipError_t adj_inflow(const Layout& L, void* lam, double utime, double cosd, double sind, double rho0,
+                                 double* gS, hipStream_t s) {
+        if constexpr (kAdj) {
+            const dim3 grid((L.ny + 63) / 64, L.nz, 1);
+            hipLaunchKernelGGL((adjoint_inflow_kernel<Q, C>), grid, dim3(64), 0, s, L, static_cast<typename C::raw*>(lam),
+                               (R)utime, (R)cosd, (R)sind, (R)rho0, gS);
+        }
+        return hipGetLastError();
+    }
+
+    static hipError_t adj_face(const Layout& L, void* ly, void* lx, int dir, int opt_lo, int opt_hi, hipStream_t s) {
+        if constexpr (kAdj) {
+            const int t0 = dir == 0 ? 1 : 0, t1 = dir == 2 ? 1 : 2;
+            const dim3 grid((L.n(t0) + 63) / 64, L.n(t1), 2);
+            hipLaunchKernelGGL((adjoint_face_kernel<Q, C>), grid, dim3(64), 0, s, L, static_cast<typename C::raw*>(ly),
+                               static_cast<typename C::raw*>(lx), dir, t0, t1, opt_lo, opt_hi);
+        }
+        return hipGetLastError();
+    }
+
+    static hipError_t objective(const Layout& L, const void* f, const AdjBox& box, double* acc, hipStream_t s) {
+        const long long nb = (long long)(box.hi[0] - box.lo[0] + 1) * (box.hi[1] - box.lo[1] + 1) * (box.hi[2] - box.lo[2] + 1);
+        const int blocks = (int)std::min<long long>((nb + 255) / 256, 1024);
+        hipLaunchKernelGGL((objective_kernel<Q, C>), dim3(blocks), dim3(256), 0, s, L, f, box, acc);
+        return hipGetLastError();
+    }
+
     static const Ops* ops() {
         static const Ops o = {&step, &closed, &snapshot, &edges, &inflow, &aos_in, &state_out, &feq_init, &totals,
                               sizeof(MemberTab<R>), &ens_pack, &ens_step,
-                              (int)sizeof(R), (int)sizeof(typename C::raw)};
+                              kAdj ? &adj_step : nullptr, kAdj ? &adj_inflow : nullptr, kAdj ? &adj_face : nullptr,
+                              &objective, (int)sizeof(R), (int)sizeof(typename C::raw)};
         return &o;
     }
 };

@@ -30,7 +30,8 @@ EXPORTS = ["lbm_create", "lbm_destroy", "lbm_last_error", "lbm_set_state", "lbm_
            "lbm_get_totals", "lbm_get_info",
            "lbm_ensemble_create", "lbm_ensemble_destroy", "lbm_ensemble_last_error", "lbm_ensemble_step",
            "lbm_ensemble_synchronize", "lbm_ensemble_set_launch", "lbm_ensemble_get_launch",
-           "lbm_ensemble_tune", "lbm_ensemble_set_timing", "lbm_ensemble_timing_read"]
+           "lbm_ensemble_tune", "lbm_ensemble_set_timing", "lbm_ensemble_timing_read",
+           "lbm_objective_run", "lbm_adjoint_run"]
 
 
 class Desc(C.Structure):
@@ -89,6 +90,8 @@ def load_library(path: str = LIB_PATH):
         "lbm_ensemble_tune": ([vp, i32, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(dbl)], i32),
         "lbm_ensemble_set_timing": ([vp, i32], i32),
         "lbm_ensemble_timing_read": ([vp, C.POINTER(dbl), C.POINTER(i32)], i32),
+        "lbm_objective_run": ([vp, i32, i32, C.POINTER(C.c_int32), C.POINTER(dbl)], i32),
+        "lbm_adjoint_run": ([vp, i32, i32, i32, C.POINTER(C.c_int32), C.POINTER(dbl), vp, vp], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -235,6 +238,30 @@ class Domain:
         F = self._arr(force, self.shape_cell + (3,))
         self._chk(self.lib.lbm_set_external_forcing(self.h, _p(F), F.itemsize), "lbm_set_external_forcing")
 
+    # -- objective and adjoint -------------------------------------------------------------
+    @staticmethod
+    def _box(box):
+        (i0, i1), (j0, j1), (k0, k1) = box
+        return (C.c_int32 * 6)(i0, i1, j0, j1, k0, k1)
+
+    def objective(self, it0, nsteps, box):
+        """Steps it0+1 .. it0+nsteps; returns the time-mean kinetic energy 0.5 rho|u|^2 over box
+        ((i0,i1),(j0,j1),(k0,k1)), 1-based inclusive (lbm_objective_run)."""
+        J = C.c_double()
+        self._chk(self.lib.lbm_objective_run(self.h, it0, nsteps, self._box(box), C.byref(J)), "lbm_objective_run")
+        return J.value
+
+    def adjoint(self, it0, nsteps, every, box, grad_force=True):
+        """As objective(), plus the reverse sweep: returns (J, dJ/d external forcing with the
+        forcing's shape (nz+2, ny+2, nx+2, 3) or None, dJ/d uvel_shear (nz,))."""
+        nx, ny, nz = self.dims
+        J = C.c_double()
+        gF = np.zeros((nz + 2, ny + 2, nx + 2, 3)) if grad_force else None
+        gS = np.zeros(nz)
+        self._chk(self.lib.lbm_adjoint_run(self.h, it0, nsteps, every, self._box(box), C.byref(J), _p(gF), _p(gS)),
+                  "lbm_adjoint_run")
+        return J.value, gF, gS
+
     # -- time stepping -------------------------------------------------------------------
     def step(self, it0, nsteps=1):
         self._chk(self.lib.lbm_step(self.h, int(it0), int(nsteps)), "lbm_step")
@@ -286,6 +313,31 @@ class Ensemble:
         except Exception:
             pass
 
+    # -- objective and adjoint -------------------------------------------------------------
+    @staticmethod
+    def _box(box):
+        (i0, i1), (j0, j1), (k0, k1) = box
+        return (C.c_int32 * 6)(i0, i1, j0, j1, k0, k1)
+
+    def objective(self, it0, nsteps, box):
+        """Steps it0+1 .. it0+nsteps; returns the time-mean kinetic energy 0.5 rho|u|^2 over box
+        ((i0,i1),(j0,j1),(k0,k1)), 1-based inclusive (lbm_objective_run)."""
+        J = C.c_double()
+        self._chk(self.lib.lbm_objective_run(self.h, it0, nsteps, self._box(box), C.byref(J)), "lbm_objective_run")
+        return J.value
+
+    def adjoint(self, it0, nsteps, every, box, grad_force=True):
+        """As objective(), plus the reverse sweep: returns (J, dJ/d external forcing with the
+        forcing's shape (nz+2, ny+2, nx+2, 3) or None, dJ/d uvel_shear (nz,))."""
+        nx, ny, nz = self.dims
+        J = C.c_double()
+        gF = np.zeros((nz + 2, ny + 2, nx + 2, 3)) if grad_force else None
+        gS = np.zeros(nz)
+        self._chk(self.lib.lbm_adjoint_run(self.h, it0, nsteps, every, self._box(box), C.byref(J), _p(gF), _p(gS)),
+                  "lbm_adjoint_run")
+        return J.value, gF, gS
+
+    # -- time stepping -------------------------------------------------------------------
     def step(self, it0, nsteps=1):
         self._chk(self.lib.lbm_ensemble_step(self.h, it0, nsteps), "lbm_ensemble_step")
 

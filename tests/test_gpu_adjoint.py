@@ -1,0 +1,99 @@
+"""Hand-written adjoint (lbm_adjoint_run) through the torch.autograd harness, against central
+finite differences of the FP64 forward (h = 1e-6), on small grids with the config-5 physics
+(D3Q19 regularized + Vreman, inflow/outflow i, periodic j, no-slip k=1 / free-slip k=nz, Gaussian
+body-force blobs, power-law inflow profile, kinetic energy over a downstream box).
+
+Tolerances: FP64 adjoint vs FP64 differences 1e-6 relative to the largest checked gradient
+entry (differences of this size are round-off of J); FP32 adjoint vs FP64 differences 1e-3
+The per-cell collision adjoint is checked separately on the CPU
+(tests/test_adjoint_cell.py).
+"""
+import numpy as np
+import pytest
+import torch
+
+from lbm_amd.torch_adjoint import ForcingProblem
+
+pytestmark = pytest.mark.gpu
+
+SMALL = dict(dims=(48, 24, 20), nsteps=40, every=10, box_i=(30, 44), nblobs=6, i_blobs=12, sigma=3.0)
+
+
+def fd_grad(prob, a, c, which, h=1e-6):
+    out = []
+    for kind, idx in which:
+        ap, am, cp, cm = a.clone(), a.clone(), c.clone(), c.clone()
+        if kind == "a":
+            ap[idx] += h
+            am[idx] -= h
+        else:
+            cp[idx] += h
+            cm[idx] -= h
+        out.append((prob.J_value(ap, cp) - prob.J_value(am, cm)) / (2 * h))
+    return np.array(out)
+
+
+def adj_grad(prob, a, c):
+    a = a.clone().requires_grad_(True)
+    c = c.clone().requires_grad_(True)
+    J = prob.J(a, c)
+    J.backward()
+    return float(J), a.grad.numpy(), c.grad.numpy()
+
+
+WHICH = [("a", 0), ("a", 3), ("a", 5), ("c", 0), ("c", 1), ("c", 4)]
+
+
+@pytest.fixture(scope="module")
+def fd64():
+    p = ForcingProblem(storage="f64", **SMALL)
+    g = fd_grad(p, p.a0, p.c0, WHICH)
+    J = p.J_value(p.a0, p.c0)
+    p.close()
+    print(f"FP64 differences: J = {J:.9e}, grad = {g}", flush=True)
+    return J, g
+
+
+@pytest.mark.parametrize("storage,tol", [("f64", 1e-6), ("f32", 1e-3)])
+def test_adjoint_gradient_matches_fp64_differences(fd64, storage, tol):
+    J64, g_fd = fd64
+    p = ForcingProblem(storage=storage, **SMALL)
+    J, ga, gc = adj_grad(p, p.a0, p.c0)
+    g = np.array([ga[i] if k == "a" else gc[i] for k, i in WHICH])
+    scale = np.abs(g_fd).max()
+    err = np.abs(g - g_fd).max() / scale
+    print(f"{storage} adjoint: J = {J:.9e}, grad = {g}, max rel err {err:.3e}", flush=True)
+    assert np.isfinite(g).all() and scale > 0
+    assert err <= tol
+    assert abs(J - J64) <= (1e-12 if storage == "f64" else 1e-5) * abs(J64)
+    p.close()
+
+
+def test_adjoint_run_leaves_the_forward_state():
+    """After the adjoint run the context holds the forward state after nsteps, bit for bit as a
+    plain run (the reverse sweep restores it), and both runs return the same J."""
+    p = ForcingProblem(storage="f32", **SMALL)
+    F = p.force_field(p.a0).numpy()
+    s = p.shear_profile(p.c0).numpy()
+    Ja, _, _ = p.evaluate(F, s, adjoint=True)
+    fa = p.dom.get_state()
+    Jo = p.evaluate(F, s, adjoint=False)
+    fo = p.dom.get_state()
+    assert Ja == Jo
+    assert np.array_equal(fa, fo)
+    p.close()
+
+
+def test_adjoint_periodic_box_force_only():
+    """Fully periodic box (no inflow, no closed face): the force gradient alone, FP64."""
+    kw = dict(SMALL, box_i=(20, 40), bnd=(0, 0, 0))
+    p = ForcingProblem(storage="f64", **kw)
+    which = [("a", 1), ("a", 2)]
+    g_fd = fd_grad(p, p.a0, p.c0, which)
+    J, ga, gc = adj_grad(p, p.a0, p.c0)
+    g = np.array([ga[1], ga[2]])
+    err = np.abs(g - g_fd).max() / np.abs(g_fd).max()
+    print(f"periodic: grad {g} vs {g_fd}, rel err {err:.3e}; shear grad max {np.abs(gc).max():.1e}", flush=True)
+    assert err <= 1e-6
+    assert np.abs(gc).max() == 0.0
+    p.close()
