@@ -6,6 +6,7 @@ library's FP64 storage run on the same input, ||u - u64||_2 / ||u64||_2 over int
 3-vector), and the same for rho - 1.  The FP64 path is pinned to the FP64 oracle by
 test_gpu_oracle.py::test_config2_physics[f64] at 1e-12.
 """
+import json
 import os
 
 import numpy as np
@@ -120,3 +121,16 @@ def test_lowprec_roundtrip_and_odd_length(storage):
     err = np.abs(f - sim.f1).max()
     print(f"{storage} odd-length grid, 30 steps: max|df| vs FP32 oracle {err:.3e}", flush=True)
     assert np.isfinite(f).all() and err <= (5e-4 if storage == "f16s" else 5e-3)
+
+
+def test_precision_study_runs_small():
+    """lbm_amd.precision_study end to end on a small grid (both physics)."""
+    from lbm_amd import precision_study
+    for phys in ("bgk", "regvre"):
+        r = precision_study.main(["--physics", phys, "--dims", "48", "32", "24", "--steps", "10", "30",
+                                  "--storages", "f32", "f16s", "--nrturb", "20"])
+        e = r["errors"]
+        print(phys, json.dumps(e), flush=True)
+        assert r["finite"]["f32"] and r["finite"]["f16s"]
+        assert e["f32"]["30"]["velocity"] < 1e-5
+        assert e["f16s"]["30"]["velocity"] < 1e-2
