@@ -230,6 +230,16 @@ def bench_adjoint(a):
     scale = np.abs(fd).max()
     e32 = float(np.abs(pick(ga32, gc32) - fd).max() / scale)
     e64 = float(np.abs(pick(ga64, gc64) - fd).max() / scale)
+    # FP64 tangent run along a seeded direction vs the FP64 adjoint's prediction
+    print("[bench] FP64 tangent run", file=sys.stderr, flush=True)
+    rng = np.random.default_rng(3)
+    da = torch.from_numpy(rng.uniform(-1e-5, 1e-5, len(ga64)))
+    dc = torch.from_numpy(rng.uniform(-0.1, 0.1, len(gc64)))
+    pt = ForcingProblem(storage="f64t", nsteps=nsteps, every=20)
+    Jt, dJt = pt.directional(pt.a0, pt.c0, da, dc)
+    pt.close()
+    pred64 = float(ga64 @ da.numpy() + gc64 @ dc.numpy())
+    pred32 = float(ga32 @ da.numpy() + gc32 @ dc.numpy())
     full64 = np.concatenate([ga64, gc64])
     full32 = np.concatenate([ga32, gc32])
     e_full = float(np.abs(full32 - full64).max() / np.abs(full64).max())
@@ -248,7 +258,10 @@ def bench_adjoint(a):
         "grad_check": {"checked": [f"{k}{i}" for k, i in which], "fd64": fd.tolist(),
                        "adj32": pick(ga32, gc32).tolist(), "adj64": pick(ga64, gc64).tolist(),
                        "max_rel_err_f32_vs_fd64": e32, "max_rel_err_f64_vs_fd64": e64,
-                       "max_rel_err_f32_vs_f64_all_72": e_full},
+                       "max_rel_err_f32_vs_f64_all_72": e_full,
+                       "tangent64_dJ": dJt, "adj64_dJ": pred64, "adj32_dJ": pred32,
+                       "rel_err_adj64_vs_tangent64": abs(pred64 - dJt) / abs(dJt),
+                       "rel_err_adj32_vs_tangent64": abs(pred32 - dJt) / abs(dJt)},
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(line))

@@ -36,6 +36,7 @@ extern "C" {
 #define LBM_STORE_F64 1
 #define LBM_STORE_F16S 2  /* (f - w_l) * 2^10 as IEEE half */
 #define LBM_STORE_BF16S 3 /* (f - w_l) as bfloat16 */
+#define LBM_STORE_F64T 4  /* FP64 with a tangent: forward-mode derivative along lbm_set_tangent */
 
 typedef struct lbm_ctx lbm_ctx;
 
@@ -183,6 +184,14 @@ int lbm_ensemble_timing_read(lbm_ens* ens, double* ms, int* nsteps);
  * accumulation in FP64.
  */
 int lbm_objective_run(lbm_ctx* ctx, int it0, int nsteps, const int32_t* box, double* J);
+
+/* Tangent run (LBM_STORE_F64T): every value carries its derivative along one input direction.
+ * lbm_set_tangent sets the direction: d external_forcing (layout of lbm_set_external_forcing;
+ * NULL: zero) and d uvel_shear (nz; NULL: zero); lbm_set_external_forcing and lbm_set_inflow
+ * reset it to zero, so call it after them.  The state's own tangent is zero after lbm_set_state.
+ * lbm_objective_tangent returns dJ of the last lbm_objective_run. */
+int lbm_set_tangent(lbm_ctx* ctx, const double* dforce, const double* dshear);
+int lbm_objective_tangent(lbm_ctx* ctx, double* dJ);
 int lbm_adjoint_run(lbm_ctx* ctx, int it0, int nsteps, int every, const int32_t* box, double* J,
                     double* grad_force, double* grad_shear);
 

@@ -15,7 +15,9 @@
 
 #include "../../include/lbm_hip.h"
 #include "ctx.h"
+#include "dual.h"
 #include "impl.h"
+#include "layout.h"
 
 using namespace lbm;
 using namespace lbm_host;
@@ -53,23 +55,64 @@ int lbm_objective_run(lbm_ctx* c, int it0, int nsteps, const int32_t* box, doubl
     AdjBox b;
     if (int rc = check_box(c, box, &b)) return rc;
     DevBuf acc;
-    if (int rc = dev_alloc(c, acc, sizeof(double))) return rc;
+    if (int rc = dev_alloc(c, acc, 2 * sizeof(double))) return rc;
     int rc = LBM_OK;
-    if (hipMemsetAsync(acc.p, 0, sizeof(double), c->stream) != hipSuccess) rc = fail(c, LBM_EHIP, "hipMemsetAsync");
+    if (hipMemsetAsync(acc.p, 0, 2 * sizeof(double), c->stream) != hipSuccess) rc = fail(c, LBM_EHIP, "hipMemsetAsync");
     for (int n = 1; n <= nsteps && rc == LBM_OK; ++n) {
         if ((rc = lbm_step(c, it0 + n - 1, 1))) break;
         const hipError_t e = c->ops->objective(c->lay, c->buf[c->cur], b, static_cast<double*>(acc.p), c->stream);
         if (e != hipSuccess) rc = hipfail(c, e, "objective kernel");
     }
-    double s = 0.0;
+    double s[2] = {0.0, 0.0};
     if (rc == LBM_OK) {
-        if (hipMemcpyAsync(&s, acc.p, sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        if (hipMemcpyAsync(s, acc.p, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
             hipStreamSynchronize(c->stream) != hipSuccess)
             rc = fail(c, LBM_EHIP, "objective read-back");
     }
     dev_free(acc);
-    if (rc == LBM_OK) *J = s / ((double)nsteps * box_cells(b));
+    if (rc == LBM_OK) {
+        const double w = 1.0 / ((double)nsteps * box_cells(b));
+        *J = s[0] * w;
+        c->obj_tangent = s[1] * w;
+    }
     return rc;
+}
+
+int lbm_objective_tangent(lbm_ctx* c, double* dJ) {
+    if (!c || !dJ) return fail(c, LBM_EINVAL, "null argument");
+    if (c->d.storage != LBM_STORE_F64T) return fail(c, LBM_EINVAL, "tangents need LBM_STORE_F64T storage");
+    *dJ = c->obj_tangent;
+    return LBM_OK;
+}
+
+int lbm_set_tangent(lbm_ctx* c, const double* dforce, const double* dshear) {
+    if (!c) return fail(c, LBM_EINVAL, "null context");
+    const lbm_desc& d = c->d;
+    if (d.storage != LBM_STORE_F64T) return fail(c, LBM_EINVAL, "tangents need LBM_STORE_F64T storage");
+    if (dforce && !c->has_force) return fail(c, LBM_ESTATE, "set the external forcing before its tangent");
+    const Layout& L = c->lay;
+    // the value parts stay; only the derivative parts are replaced
+    if (c->has_force)
+        for (int q = 0; q < 3; ++q) {
+            std::vector<Dual> img((size_t)L.plane);
+            HIPCHK(c, hipMemcpyAsync(img.data(), c->force[q].p, img.size() * sizeof(Dual), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            for (auto& x : img) x.d = 0.0;
+            if (dforce)
+                for (int k = 0; k <= L.nz + 1; ++k)
+                    for (int j = 0; j <= L.ny + 1; ++j)
+                        for (int i = 0; i <= L.nx + 1; ++i)
+                            img[(size_t)L.cell(i, j, k)].d = dforce[3 * (((size_t)k * (L.ny + 2) + j) * (L.nx + 2) + i) + q];
+            HIPCHK(c, hipMemcpyAsync(c->force[q].p, img.data(), img.size() * sizeof(Dual), hipMemcpyHostToDevice, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        }
+    std::vector<Dual> sh((size_t)d.nz);
+    HIPCHK(c, hipMemcpyAsync(sh.data(), c->shear.p, sh.size() * sizeof(Dual), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int k = 0; k < d.nz; ++k) sh[k].d = dshear ? dshear[k] : 0.0;
+    HIPCHK(c, hipMemcpyAsync(c->shear.p, sh.data(), sh.size() * sizeof(Dual), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LBM_OK;
 }
 
 int lbm_adjoint_run(lbm_ctx* c, int it0, int nsteps, int every, const int32_t* box, double* J, double* grad_force,

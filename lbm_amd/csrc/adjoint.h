@@ -395,16 +395,18 @@ __global__ void adjoint_face_kernel(Layout L, typename C::raw* lam_y, typename C
 }
 
 // Objective: sum over the box of 0.5*rho*|u|^2 of the streamed populations of buffer f (the
-// reference's macrovars after a step), added to acc[0] (FP64).
+// reference's macrovars after a step), added to acc[0] (FP64); with the tangent type its
+// directional derivative is added to acc[1].
 template <int Q, class C>
 __global__ void __launch_bounds__(256) objective_kernel(Layout L, const void* f, AdjBox box, double* acc) {
     using R = typename C::real;
     using raw = typename C::raw;
     using Lt = Lattice<Q>;
+    constexpr bool TAN = is_dual<R>::value;
     const int bx = box.hi[0] - box.lo[0] + 1, by = box.hi[1] - box.lo[1] + 1, bz = box.hi[2] - box.lo[2] + 1;
     const long long nb = (long long)bx * by * bz;
     const raw* __restrict__ p = static_cast<const raw*>(f);
-    double sum = 0.0;
+    double sum = 0.0, dsum = 0.0;
     for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < nb; e += (long long)gridDim.x * blockDim.x) {
         const int i = box.lo[0] + (int)(e % bx);
         const int j = box.lo[1] + (int)((e / bx) % by);
@@ -421,16 +423,29 @@ __global__ void __launch_bounds__(256) objective_kernel(Layout L, const void* f,
             jy += R(Lt::cy(l)) * d;
             jz += R(Lt::cz(l)) * d;
         }
-        sum += 0.5 * ((double)jx * jx + (double)jy * jy + (double)jz * jz) / (1.0 + (double)dr);
+        if constexpr (TAN) {
+            const R ke = R(0.5) * (jx * jx + jy * jy + jz * jz) / (R(1) + dr);
+            sum += ke.v;
+            dsum += ke.d;
+        } else {
+            sum += 0.5 * ((double)jx * jx + (double)jy * jy + (double)jz * jz) / (1.0 + (double)dr);
+        }
     }
-    __shared__ double red[256];
-    red[threadIdx.x] = sum;
+    __shared__ double red[2][256];
+    red[0][threadIdx.x] = sum;
+    red[1][threadIdx.x] = dsum;
     __syncthreads();
     for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        if ((int)threadIdx.x < s) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + s];
+            red[1][threadIdx.x] += red[1][threadIdx.x + s];
+        }
         __syncthreads();
     }
-    if (threadIdx.x == 0) atomicAdd(acc, red[0]);
+    if (threadIdx.x == 0) {
+        atomicAdd(acc, red[0][0]);
+        if (TAN) atomicAdd(acc + 1, red[1][0]);
+    }
 }
 
 }  // namespace lbm

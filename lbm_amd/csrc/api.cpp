@@ -14,6 +14,7 @@
 #include "ctx.h"
 #include "impl.h"
 #include "layout.h"
+#include "dual.h"
 
 using namespace lbm;
 using namespace lbm_host;
@@ -61,14 +62,26 @@ int check_kind(lbm_ctx* c, int realkind) {
     return LBM_OK;
 }
 
+// Element q of a host image of a device array of the compute type (rb = 4 float, 8 double,
+// 16 value + tangent; stored tangents are zero).
+void put_real(void* img, size_t q, double v, int rb) {
+    if (rb == 16) reinterpret_cast<Dual*>(img)[q] = Dual(v, 0.0);
+    else if (rb == 8) reinterpret_cast<double*>(img)[q] = v;
+    else reinterpret_cast<float*>(img)[q] = (float)v;
+}
+double get_real(const void* img, size_t q, int rb) {
+    if (rb == 16) return reinterpret_cast<const Dual*>(img)[q].v;
+    if (rb == 8) return reinterpret_cast<const double*>(img)[q];
+    return reinterpret_cast<const float*>(img)[q];
+}
+
 // Host reals of either kind -> device array of the compute type.
 int upload_reals(lbm_ctx* c, DevBuf& b, const void* host, size_t n, int realkind) {
     const int rb = c->ops->real_bytes;
     std::vector<unsigned char> tmp(n * rb);
     for (size_t q = 0; q < n; ++q) {
         double v = realkind == 8 ? static_cast<const double*>(host)[q] : static_cast<const float*>(host)[q];
-        if (rb == 8) reinterpret_cast<double*>(tmp.data())[q] = v;
-        else reinterpret_cast<float*>(tmp.data())[q] = (float)v;
+        put_real(tmp.data(), q, v, rb);
     }
     int rc = dev_alloc(c, b, n * rb);
     if (rc) return rc;
@@ -313,6 +326,7 @@ int lbm_create(const lbm_desc* desc, lbm_ctx** out) {
         case LBM_STORE_F64: ops = ops_f64(d.q); break;
         case LBM_STORE_F16S: ops = ops_f16s(d.q); break;
         case LBM_STORE_BF16S: ops = ops_bf16s(d.q); break;
+        case LBM_STORE_F64T: ops = ops_f64t(d.q); break;
         default: return fail(nullptr, LBM_EINVAL, "unknown storage kind");
     }
     const Layout lay = make_layout(d.nx, d.ny, d.nz, ops->raw_bytes, d.ibnd == 0, d.jbnd == 0, d.kbnd == 0);
@@ -541,8 +555,7 @@ int lbm_get_tau(lbm_ctx* c, void* tau, int realkind) {
             for (int j = 1; j <= L.ny; ++j)
                 for (int i = 1; i <= L.nx; ++i) {
                     const size_t o = (size_t)L.cell(i, j, k);
-                    const double v = rb == 8 ? reinterpret_cast<double*>(dev.data())[o] : reinterpret_cast<float*>(dev.data())[o];
-                    out[((size_t)k * (L.ny + 2) + j) * (L.nx + 2) + i] = v;
+                    out[((size_t)k * (L.ny + 2) + j) * (L.nx + 2) + i] = get_real(dev.data(), o, rb);
                 }
     }
     for (size_t q = 0; q < nc; ++q) {
@@ -653,9 +666,7 @@ int lbm_set_external_forcing(lbm_ctx* c, const void* force, int realkind) {
                     const size_t hc = ((size_t)k * (L.ny + 2) + j) * (L.nx + 2) + i;
                     const double v = realkind == 8 ? static_cast<const double*>(force)[3 * hc + q]
                                                    : static_cast<const float*>(force)[3 * hc + q];
-                    const size_t o = (size_t)L.cell(i, j, k);
-                    if (rb == 8) reinterpret_cast<double*>(plane.data())[o] = v;
-                    else reinterpret_cast<float*>(plane.data())[o] = (float)v;
+                    put_real(plane.data(), (size_t)L.cell(i, j, k), v, rb);
                 }
         if (int rc = dev_alloc(c, c->force[q], plane.size())) return rc;
         HIPCHK(c, hipMemcpyAsync(c->force[q].p, plane.data(), plane.size(), hipMemcpyHostToDevice, c->stream));

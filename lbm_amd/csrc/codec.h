@@ -16,11 +16,12 @@
 #include <hip/hip_bf16.h>
 #include <stdint.h>
 
+#include "dual.h"
 #include "lattice.h"
 
 namespace lbm {
 
-enum StorageKind : int { STORE_F32 = 0, STORE_F64 = 1, STORE_F16S = 2, STORE_BF16S = 3 };
+enum StorageKind : int { STORE_F32 = 0, STORE_F64 = 1, STORE_F16S = 2, STORE_BF16S = 3, STORE_F64T = 4 };
 
 // The FP32 and FP64 codecs are also host-callable, so that the per-cell collision can run on the
 // CPU (the adjoint's finite-difference check).
@@ -49,6 +50,24 @@ struct CodecF64 {
     using real = double;
     static constexpr int kind = STORE_F64;
     static constexpr double kDepScale = 1.0;
+    template <int Q>
+    __host__ __device__ static inline real dep(raw r, int l) { return r - weight<Q, real>(l); }
+    template <int Q>
+    __host__ __device__ static inline raw enc_dep(real dev, int l) { return weight<Q, real>(l) + dev; }
+    template <int Q>
+    __host__ __device__ static inline real decode(raw r, int) { return r; }
+    template <int Q>
+    __host__ __device__ static inline raw encode(real v, int) { return v; }
+    template <int Q>
+    __host__ __device__ static inline real shifted(raw r, int l) { return r - weight<Q, real>(l); }
+};
+
+// FP64 values with their tangent (forward-mode derivative along one input direction).
+struct CodecTan {
+    using raw = Dual;
+    using real = Dual;
+    static constexpr int kind = STORE_F64T;
+    static constexpr Dual kDepScale = Dual(1.0);
     template <int Q>
     __host__ __device__ static inline real dep(raw r, int l) { return r - weight<Q, real>(l); }
     template <int Q>

@@ -45,6 +45,7 @@ struct lbm_ctx {
     int series_lo = 0, series_hi = -1;
     std::vector<double> ut_series, ud_series;
     double tau0 = 0.0, fac0 = 0.0;
+    double obj_tangent = 0.0;      // directional derivative of the last objective (tangent storage)
     int fold_dir = -1;             // closed axis whose face pass the step kernel performs
     int variant = 1;               // step-kernel variant (LBM_STEP_VARIANT), see impl.h; non-temporal
                                    // stores measured fastest on MI355X (5.32 vs 5.11 TB/s)

@@ -118,5 +118,6 @@ const Ops* ops_f32(int q);
 const Ops* ops_f64(int q);
 const Ops* ops_f16s(int q);
 const Ops* ops_bf16s(int q);
+const Ops* ops_f64t(int q);
 
 }  // namespace lbm

@@ -179,7 +179,8 @@ __host__ __device__ __forceinline__ void collide_cell(const StepArgs<typename C:
         const R b23 = a.eps + (axy * axz + ayy * ayz + ayz * azz);
         const R bb = b11 * b22 - b12 * b12 + b11 * b33 - b13 * b13 + b22 * b33 - b23 * b23;
         const R t = bb / amag;
-        const R eddy = t > a.eps ? a.vconst * std::sqrt(t) : R(0);
+        using std::sqrt;   // the tangent type brings its own sqrt
+        const R eddy = t > a.eps ? a.vconst * sqrt(t) : R(0);
         tau = R(3) * (a.kinevisc + eddy + a.ablvisc[k - 1]) + R(0.5);
         fac = R(1) - R(1) / tau;
     }
@@ -362,7 +363,7 @@ __device__ __forceinline__ void step_cell(const StepArgs<typename C::real>& a, i
 #pragma unroll
     for (int l = 0; l < Q; ++l) {
         const raw* p = src + (size_t)l * (size_t)L.plane + (size_t)(oz[Lt::cz(l) + 1] + oy[Lt::cy(l) + 1] + ox[Lt::cx(l) + 1]);
-        if (VAR & 2) rv[l] = __builtin_nontemporal_load(p);
+        if constexpr ((VAR & 2) != 0) rv[l] = __builtin_nontemporal_load(p);
         else rv[l] = *p;
     }
 
@@ -375,7 +376,7 @@ __device__ __forceinline__ void step_cell(const StepArgs<typename C::real>& a, i
 #pragma unroll
     for (int l = 0; l < Q; ++l) {
         raw* p = dst + (size_t)l * (size_t)L.plane + (size_t)oc;
-        if (VAR & 1) __builtin_nontemporal_store(C::template enc_dep<Q>(out[l], l), p);
+        if constexpr ((VAR & 1) != 0) __builtin_nontemporal_store(C::template enc_dep<Q>(out[l], l), p);
         else *p = C::template enc_dep<Q>(out[l], l);
     }
     if (VRE && a.tau_out != nullptr) a.tau_out[oc] = tau;

@@ -22,7 +22,12 @@ struct Limits<float> {
 template <>
 struct Limits<double> {
     static double tiny() { return DBL_MIN; }
-    static double eps() { return sqrt(DBL_MIN); }
+    static double eps() { return std::sqrt(DBL_MIN); }
+};
+template <>
+struct Limits<Dual> {
+    static Dual tiny() { return Dual(DBL_MIN); }
+    static Dual eps() { return Dual(std::sqrt(DBL_MIN)); }
 };
 
 inline dim3 grid_cells(const Layout& L, int bx) { return dim3((L.nx + 2 + bx - 1) / bx, L.ny + 2, L.nz + 2); }
@@ -30,6 +35,8 @@ inline dim3 grid_cells(const Layout& L, int bx) { return dim3((L.nx + 2 + bx - 1
 template <int Q, class C>
 struct Launch {
     using R = typename C::real;
+    // non-temporal loads and stores take scalar types only (not the tangent pair)
+    static constexpr bool kNT = sizeof(typename C::raw) <= 8;
 
     template <bool B3, bool H, bool V>
     static hipError_t step_t(const Layout& L, const StepArgs<R>& a, dim3 grid, dim3 block, int var, hipStream_t s) {
@@ -50,8 +57,12 @@ struct Launch {
                 return hipGetLastError();
             }
         }
-        if (nt) hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 1>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 0>), grid, block, 0, s, a);
+        if constexpr (kNT) {
+            if (nt) hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 1>), grid, block, 0, s, a);
+            else hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 0>), grid, block, 0, s, a);
+        } else {
+            hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 0>), grid, block, 0, s, a);
+        }
         return hipGetLastError();
     }
 
@@ -225,8 +236,9 @@ struct Launch {
 
     template <bool B3, bool H, bool V>
     static void ens_step_t(const EnsLaunch& e, hipStream_t s) {
-        hipLaunchKernelGGL((step_kernel_batch<Q, C, B3, H, V>), dim3(e.grid), dim3(e.block), 0, s,
-                           static_cast<const MemberTab<R>*>(e.tab), e.start, e.nmem, e.total, e.clock);
+        if constexpr (kNT)
+            hipLaunchKernelGGL((step_kernel_batch<Q, C, B3, H, V>), dim3(e.grid), dim3(e.block), 0, s,
+                               static_cast<const MemberTab<R>*>(e.tab), e.start, e.nmem, e.total, e.clock);
     }
 
     static hipError_t ens_step(const EnsLaunch& e, hipStream_t s) {
@@ -250,7 +262,7 @@ struct Launch {
     }
 
     // ---- adjoint
-    static constexpr bool kAdj = sizeof(typename C::raw) >= 4;
+    static constexpr bool kAdj = sizeof(typename C::raw) == 4 || sizeof(typename C::raw) == 8;
 
     template <bool B3, bool H, bool V>
     static void adj_step_t(const Layout& L, const StepArgs<R>& a, const void* ly, void* lx, const AdjBox& box, R kappa,

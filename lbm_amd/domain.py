@@ -21,7 +21,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblbm_hip.so")
 
-STORAGE = {"f32": 0, "f64": 1, "f16s": 2, "bf16s": 3}
+STORAGE = {"f32": 0, "f64": 1, "f16s": 2, "bf16s": 3, "f64t": 4}
 
 EXPORTS = ["lbm_create", "lbm_destroy", "lbm_last_error", "lbm_set_state", "lbm_init_equilibrium",
            "lbm_get_state", "lbm_get_macros", "lbm_get_tau", "lbm_set_blanking", "lbm_set_ablvisc",
@@ -31,7 +31,7 @@ EXPORTS = ["lbm_create", "lbm_destroy", "lbm_last_error", "lbm_set_state", "lbm_
            "lbm_ensemble_create", "lbm_ensemble_destroy", "lbm_ensemble_last_error", "lbm_ensemble_step",
            "lbm_ensemble_synchronize", "lbm_ensemble_set_launch", "lbm_ensemble_get_launch",
            "lbm_ensemble_tune", "lbm_ensemble_set_timing", "lbm_ensemble_timing_read",
-           "lbm_objective_run", "lbm_adjoint_run"]
+           "lbm_objective_run", "lbm_adjoint_run", "lbm_set_tangent", "lbm_objective_tangent"]
 
 
 class Desc(C.Structure):
@@ -92,6 +92,8 @@ def load_library(path: str = LIB_PATH):
         "lbm_ensemble_timing_read": ([vp, C.POINTER(dbl), C.POINTER(i32)], i32),
         "lbm_objective_run": ([vp, i32, i32, C.POINTER(C.c_int32), C.POINTER(dbl)], i32),
         "lbm_adjoint_run": ([vp, i32, i32, i32, C.POINTER(C.c_int32), C.POINTER(dbl), vp, vp], i32),
+        "lbm_set_tangent": ([vp, vp, vp], i32),
+        "lbm_objective_tangent": ([vp, C.POINTER(dbl)], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -114,7 +116,7 @@ class Domain:
         self.lib = load_library()
         nx, ny, nz = dims
         self.q, self.dims, self.storage = q, (nx, ny, nz), storage
-        self.real = np.float64 if storage == "f64" else np.float32
+        self.real = np.float64 if storage in ("f64", "f64t") else np.float32
         d = Desc(nx, ny, nz, q, STORAGE[storage], ibgk, ihrr, ivreman, ibnd, jbnd, kbnd, iturb_pos,
                  int(store_tau), block[0], block[1], float(kinevisc), float(smagorinsky), float(rho0))
         h = C.c_void_p()
@@ -262,6 +264,17 @@ class Domain:
                   "lbm_adjoint_run")
         return J.value, gF, gS
 
+    def set_tangent(self, dforce=None, dshear=None):
+        """Direction of the tangent run (storage "f64t"): d forcing (forcing's shape) and d shear."""
+        F = None if dforce is None else self._arr(dforce, self.shape_cell + (3,), np.float64)
+        S = None if dshear is None else np.ascontiguousarray(dshear, dtype=np.float64)
+        self._chk(self.lib.lbm_set_tangent(self.h, _p(F), _p(S)), "lbm_set_tangent")
+
+    def objective_tangent(self):
+        dJ = C.c_double()
+        self._chk(self.lib.lbm_objective_tangent(self.h, C.byref(dJ)), "lbm_objective_tangent")
+        return dJ.value
+
     # -- time stepping -------------------------------------------------------------------
     def step(self, it0, nsteps=1):
         self._chk(self.lib.lbm_step(self.h, int(it0), int(nsteps)), "lbm_step")
@@ -336,6 +349,17 @@ class Ensemble:
         self._chk(self.lib.lbm_adjoint_run(self.h, it0, nsteps, every, self._box(box), C.byref(J), _p(gF), _p(gS)),
                   "lbm_adjoint_run")
         return J.value, gF, gS
+
+    def set_tangent(self, dforce=None, dshear=None):
+        """Direction of the tangent run (storage "f64t"): d forcing (forcing's shape) and d shear."""
+        F = None if dforce is None else self._arr(dforce, self.shape_cell + (3,), np.float64)
+        S = None if dshear is None else np.ascontiguousarray(dshear, dtype=np.float64)
+        self._chk(self.lib.lbm_set_tangent(self.h, _p(F), _p(S)), "lbm_set_tangent")
+
+    def objective_tangent(self):
+        dJ = C.c_double()
+        self._chk(self.lib.lbm_objective_tangent(self.h, C.byref(dJ)), "lbm_objective_tangent")
+        return dJ.value
 
     # -- time stepping -------------------------------------------------------------------
     def step(self, it0, nsteps=1):

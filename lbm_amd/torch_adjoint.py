@@ -49,7 +49,7 @@ class ForcingProblem:
         self.dom = Domain(19, self.dims, storage, ibgk=3, ihrr=1, ivreman=1, smagorinsky=smagorinsky,
                           kinevisc=kinevisc, ibnd=bnd[0], jbnd=bnd[1], kbnd=bnd[2])
         self.f0 = inputs.perturbed_uniform_flow(19, self.dims, seed=seed, u0=(uini, 0.0, 0.0), order=3,
-                                                dtype=np.float64 if storage == "f64" else np.float32)
+                                                dtype=np.float32 if storage == "f32" else np.float64)
         r = inputs.uniform01(seed + 77, 3 * nblobs).reshape(3, nblobs)
         self.centres = np.stack([i_blobs + 8.0 * (2 * r[0] - 1), 1 + (ny - 1) * r[1], 8 + (nz - 16) * r[2]], 1)
         self.sigma = sigma
@@ -97,6 +97,25 @@ class ForcingProblem:
         if adjoint:
             return d.adjoint(1, self.nsteps, self.every, self.box)
         return d.objective(1, self.nsteps, self.box)
+
+    def directional(self, a, coef, da, dc):
+        """Tangent run (storage "f64t"): (J, dJ) with dJ the derivative of J along (da, dc),
+        carried through the forward kernels in forward-mode FP64."""
+        with torch.no_grad():
+            F = self.force_field(a).numpy()
+            dF = self.force_field(da).numpy()
+            s = self.shear_profile(coef).numpy()
+            ds = self.shear_profile(dc).numpy()
+        d = self.dom
+        d.set_external_forcing(None)
+        d.set_inflow(self.uini, 0.0, self.shear_profile(self.c0).numpy())
+        d.set_state(self.f0)
+        d.step(0, 1)
+        d.set_external_forcing(F)
+        d.set_inflow(self.uini, 0.0, s)
+        d.set_tangent(dF, ds)
+        J = d.objective(1, self.nsteps, self.box)
+        return J, d.objective_tangent()
 
     def J(self, a, coef):
         """Objective as a differentiable function of the blob amplitudes and profile coefficients."""

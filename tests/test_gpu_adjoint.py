@@ -97,3 +97,21 @@ def test_adjoint_periodic_box_force_only():
     assert err <= 1e-6
     assert np.abs(gc).max() == 0.0
     p.close()
+
+
+def test_tangent_run_matches_fp64_adjoint():
+    """The FP64 tangent run (the forward kernels in forward-mode arithmetic) gives the
+    directional derivative the FP64 adjoint gradient predicts: <grad J, (da, dc)>."""
+    p64 = ForcingProblem(storage="f64", **SMALL)
+    J64, ga, gc = adj_grad(p64, p64.a0, p64.c0)
+    rng = np.random.default_rng(3)
+    da = torch.from_numpy(rng.uniform(-1e-5, 1e-5, len(ga)))
+    dc = torch.from_numpy(rng.uniform(-0.1, 0.1, len(gc)))
+    pred = float(ga @ da.numpy() + gc @ dc.numpy())
+    pt = ForcingProblem(storage="f64t", **SMALL)
+    Jt, dJ = pt.directional(pt.a0, pt.c0, da, dc)
+    print(f"tangent dJ {dJ:.12e}, adjoint <g, d> {pred:.12e}, J {Jt:.12e} vs {J64:.12e}", flush=True)
+    assert abs(Jt - J64) <= 1e-12 * abs(J64)
+    assert abs(dJ - pred) <= 1e-9 * abs(pred)
+    p64.close()
+    pt.close()
