@@ -104,6 +104,19 @@ int lbm_set_inflow_series(lbm_ctx* ctx, int it_lo, int it_hi, const void* uvel_t
  * uu == NULL switches it off. */
 int lbm_set_turbulence(lbm_ctx* ctx, const void* uu, const void* vv, const void* ww, int nrturb,
                        double ampl, int realkind);
+/* Inflow-turbulence fields generated on the device, replacing the reference's
+ * inflow_turbulence_compute (src/m_inflow_turbulence_compute.F90:3) over pseudo2D
+ * (src/m_pseudo2D.F90:3), which the reference runs at start-up and every nrturb steps
+ * (src/main.F90:252,381-383): per variable and slice a Gaussian random field with the analytic
+ * spectrum of pseudo2D (decorrelation length 10/sqrt(3) cells, r2 formed from rx as there,
+ * mean mode zeroed), an unnormalized inverse real 2-D transform of size ny x nz rounded up to
+ * even, then slices 1..nrturb mixed as AR(1) with 0.98 and scaled to zero mean and unit sample
+ * deviation.  The random phases come from SplitMix64(seed) where the reference calls the
+ * compiler's random_number.  The fields become the context's turbulence forcing (as
+ * lbm_set_turbulence with ampl); uu, vv, ww, rr (ny,nz,0:nrturb) receive copies when not NULL
+ * (the turbulence restart record).  FP32/FP64 compute only. */
+int lbm_generate_turbulence(lbm_ctx* ctx, int nrturb, double ampl, uint64_t seed, void* uu, void* vv,
+                            void* ww, void* rr, int realkind);
 /* external_forcing(3,0:nx+1,0:ny+1,0:nz+1) applied every step by forcings_apply
  * (src/m_forcings_apply.F90:3); NULL switches it off. */
 int lbm_set_external_forcing(lbm_ctx* ctx, const void* force, int realkind);

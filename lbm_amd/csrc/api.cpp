@@ -648,6 +648,57 @@ int lbm_set_turbulence(lbm_ctx* c, const void* uu, const void* vv, const void* w
     return LBM_OK;
 }
 
+int lbm_generate_turbulence(lbm_ctx* c, int nrturb, double ampl, uint64_t seed, void* uu, void* vv, void* ww, void* rr,
+                            int realkind) {
+    if (!c) return fail(c, LBM_EINVAL, "null context");
+    if (nrturb < 1) return fail(c, LBM_EINVAL, "nrturb must be >= 1");
+    if ((uu || vv || ww || rr) && check_kind(c, realkind)) return LBM_EINVAL;
+    const int rb = c->ops->real_bytes;
+    if (rb != 4 && rb != 8) return fail(c, LBM_EINVAL, "turbulence generation needs FP32/FP64 compute (not the tangent storage)");
+    if (c->d.iturb_pos < 1 || c->d.iturb_pos > c->d.nx) return fail(c, LBM_EINVAL, "iturb_pos outside the grid");
+    const lbm_desc& d = c->d;
+    const size_t n = (size_t)d.ny * d.nz * (nrturb + 1);
+    DevBuf scratch, out;
+    int rc = LBM_OK;
+    if ((rc = dev_alloc(c, scratch, turbulence_scratch_bytes(d.ny, d.nz, nrturb))) ||
+        (rc = dev_alloc(c, out, 4 * n * rb))) {
+        dev_free(scratch);
+        dev_free(out);
+        return rc;
+    }
+    auto run = [&]() -> int {
+        HIPCHK(c, turbulence_generate(rb, d.ny, d.nz, nrturb, seed, scratch.p, out.p, c->stream));
+        for (int q = 0; q < 3; ++q) {
+            if (int r = dev_alloc(c, c->turb[q], n * rb)) return r;
+            HIPCHK(c, hipMemcpyAsync(c->turb[q].p, static_cast<char*>(out.p) + q * n * rb, n * rb,
+                                     hipMemcpyDeviceToDevice, c->stream));
+        }
+        void* host[4] = {uu, vv, ww, rr};
+        std::vector<unsigned char> img(n * rb);
+        for (int q = 0; q < 4; ++q) {
+            if (!host[q]) continue;
+            HIPCHK(c, hipMemcpyAsync(img.data(), static_cast<char*>(out.p) + q * n * rb, n * rb, hipMemcpyDeviceToHost,
+                                     c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            for (size_t e = 0; e < n; ++e) {
+                const double v = get_real(img.data(), e, rb);
+                if (realkind == 8) static_cast<double*>(host[q])[e] = v;
+                else static_cast<float*>(host[q])[e] = (float)v;
+            }
+        }
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return LBM_OK;
+    };
+    rc = run();
+    dev_free(scratch);
+    dev_free(out);
+    if (rc) return rc;
+    c->nrturb = nrturb;
+    c->ampl = round_real(c, ampl);
+    c->has_turb = true;
+    return LBM_OK;
+}
+
 int lbm_set_external_forcing(lbm_ctx* c, const void* force, int realkind) {
     if (!c) return fail(c, LBM_EINVAL, "null context");
     if (!force) {

@@ -113,6 +113,12 @@ struct Ops {
     int raw_bytes;    // storage size
 };
 
+// Inflow-turbulence generation (turbulence.hip): scratch size and the generator writing the
+// four fields uu, vv, ww, rr (each (nrturb+1) x nz x ny, j fastest) of the compute type (rb bytes).
+size_t turbulence_scratch_bytes(int ny, int nz, int nrturb);
+hipError_t turbulence_generate(int rb, int ny, int nz, int nrturb, uint64_t seed, void* scratch, void* out,
+                               hipStream_t stream);
+
 // Defined in the per-codec translation units.
 const Ops* ops_f32(int q);
 const Ops* ops_f64(int q);

@@ -31,7 +31,8 @@ EXPORTS = ["lbm_create", "lbm_destroy", "lbm_last_error", "lbm_set_state", "lbm_
            "lbm_ensemble_create", "lbm_ensemble_destroy", "lbm_ensemble_last_error", "lbm_ensemble_step",
            "lbm_ensemble_synchronize", "lbm_ensemble_set_launch", "lbm_ensemble_get_launch",
            "lbm_ensemble_tune", "lbm_ensemble_set_timing", "lbm_ensemble_timing_read",
-           "lbm_objective_run", "lbm_adjoint_run", "lbm_set_tangent", "lbm_objective_tangent"]
+           "lbm_objective_run", "lbm_adjoint_run", "lbm_set_tangent", "lbm_objective_tangent",
+           "lbm_generate_turbulence"]
 
 
 class Desc(C.Structure):
@@ -93,6 +94,7 @@ def load_library(path: str = LIB_PATH):
         "lbm_objective_run": ([vp, i32, i32, C.POINTER(C.c_int32), C.POINTER(dbl)], i32),
         "lbm_adjoint_run": ([vp, i32, i32, i32, C.POINTER(C.c_int32), C.POINTER(dbl), vp, vp], i32),
         "lbm_set_tangent": ([vp, vp, vp], i32),
+        "lbm_generate_turbulence": ([vp, i32, dbl, C.c_uint64, vp, vp, vp, vp, i32], i32),
         "lbm_objective_tangent": ([vp, C.POINTER(dbl)], i32),
     }
     for name, (args, res) in sig.items():
@@ -232,6 +234,16 @@ class Domain:
         a = [np.ascontiguousarray(x, dtype=dt) for x in (uu, vv, ww)]
         self._chk(self.lib.lbm_set_turbulence(self.h, *[_p(x) for x in a], int(nrturb), float(ampl),
                                               a[0].itemsize), "lbm_set_turbulence")
+
+    def generate_turbulence(self, nrturb, ampl, seed, return_fields=True):
+        """Inflow-turbulence fields generated on the device (lbm_generate_turbulence) and made
+        the forcing; returns (uu, vv, ww, rr), each (nrturb+1, nz, ny), when return_fields."""
+        nx, ny, nz = self.dims
+        out = [np.empty((nrturb + 1, nz, ny), self.real) for _ in range(4)] if return_fields else [None] * 4
+        self._chk(self.lib.lbm_generate_turbulence(self.h, nrturb, float(ampl), int(seed) & (2 ** 64 - 1),
+                                                   *[_p(a) for a in out], np.dtype(self.real).itemsize),
+                  "lbm_generate_turbulence")
+        return tuple(out) if return_fields else None
 
     def set_external_forcing(self, force):
         if force is None:

@@ -6,10 +6,11 @@
 !
 ! The grid comes from the restart header when nt0 > 0 (the reference fixes it at compile time in
 ! mod_dimensions); for nt0 = 0 give it on the command line:  lbm_driver nx ny nz [nl].
+! Inflow-turbulence fields come from the turbulence restart file when it exists, otherwise (and
+! every nrturb steps, as src/main.F90:381-383) from the device generator lbm_generate_turbulence.
 ! Not carried over (outside the lattice update): turbines, buoyancy/ABL, tracers, Tecplot and
-! NetCDF output, averaging, the FFTW turbulence generator (turbulence fields come from the
-! turbulence restart file), the random perturbation of inipert (nt0 = 0 starts from the
-! inflow profile).  Geometry: 'none' or the reference's 'cylinder' (src/m_cylinder.F90).
+! NetCDF output, averaging, the random perturbation of inipert (nt0 = 0 starts from the inflow
+! profile).  Geometry: 'none' or the reference's 'cylinder' (src/m_cylinder.F90).
 program lbm_driver
    use, intrinsic :: iso_c_binding
    use lbm_hip
@@ -188,23 +189,24 @@ program lbm_driver
    endif
 
    if (inflowturbulence) then
-      ! turbulence fields from restart/turbulence_0000_<nt0>.uf (src/m_readrestart.F90:66-100)
       allocate(uu(ny,nz,0:nrturb), vv(ny,nz,0:nrturb), ww(ny,nz,0:nrturb), rr(ny,nz,0:nrturb))
       write(fname,'(a,i6.6,a)') 'restart/turbulence_0000_', nt0, '.uf'
       inquire(file=trim(fname), exist=ex)
-      if (.not. ex) then
-         print '(2a)', 'lbm_driver: no turbulence fields: ', trim(fname)
-         stop 1
+      if (nt0 == 0 .or. .not. ex) then
+         ! generated on the device (src/main.F90:252, inflow_turbulence_update)
+         call generate_turbulence(nt0)
+      else
+         ! turbulence fields from restart/turbulence_0000_<nt0>.uf (src/m_readrestart.F90:66-100)
+         open(newunit=iunit, file=trim(fname), form='unformatted', status='old')
+         read(iunit) j, k, i
+         rewind(iunit)
+         if (j /= ny .or. k /= nz .or. i /= nrturb) stop 'lbm_driver: incompatible turbulence restart file'
+         read(iunit) j, k, i, uu, vv, ww, rr
+         close(iunit)
+         rc = lbm_set_turbulence(ctx, c_loc(uu), c_loc(vv), c_loc(ww), int(nrturb, c_int), &
+                                 real(turbulence_ampl, c_double), realkind)
+         call lbm_check(ctx, rc, 'lbm_set_turbulence')
       endif
-      open(newunit=iunit, file=trim(fname), form='unformatted', status='old')
-      read(iunit) j, k, i
-      rewind(iunit)
-      if (j /= ny .or. k /= nz .or. i /= nrturb) stop 'lbm_driver: incompatible turbulence restart file'
-      read(iunit) j, k, i, uu, vv, ww, rr
-      close(iunit)
-      rc = lbm_set_turbulence(ctx, c_loc(uu), c_loc(vv), c_loc(ww), int(nrturb, c_int), &
-                              real(turbulence_ampl, c_double), realkind)
-      call lbm_check(ctx, rc, 'lbm_set_turbulence')
    endif
 
    ! ---------------------------------------------------------------- time loop
@@ -214,10 +216,14 @@ program lbm_driver
    do while (it < nt1)
       nchunk = min(10 - mod(it, 10), nt1 - it)
       if (ldump .and. irestart > 0) nchunk = min(nchunk, irestart - mod(it, irestart))
+      if (inflowturbulence .and. ibnd == 1) nchunk = min(nchunk, nrturb - mod(it, nrturb))
       rc = lbm_step(ctx, int(it, c_int), int(nchunk, c_int))
       call lbm_check(ctx, rc, 'lbm_step')
       it = it + nchunk
       if (mod(it, 10) == 0 .or. it == nt1) print '(a,i6,a,f10.2,a)', 'Iteration: ', it, ' Time: ', real(it)*p_time, ' s.'
+      ! new turbulence fields every nrturb steps (src/main.F90:381-383)
+      if (inflowturbulence .and. ibnd == 1 .and. mod(it, nrturb) == 0 .and. it > 1 .and. it < nt1) &
+         call generate_turbulence(it)
       if (ldump .and. irestart > 0 .and. mod(it, irestart) == 0 .and. it /= nt1) call save_restart(it)
    enddo
    rc = lbm_synchronize(ctx)
@@ -237,6 +243,14 @@ program lbm_driver
    rc = lbm_destroy(ctx)
 
 contains
+
+   subroutine generate_turbulence(itw)
+      ! the fields are kept on the host too, for the turbulence restart record
+      integer, intent(in) :: itw
+      rc = lbm_generate_turbulence(ctx, int(nrturb, c_int), real(turbulence_ampl, c_double), &
+                                   int(itw, c_int64_t) + 1_c_int64_t, c_loc(uu), c_loc(vv), c_loc(ww), c_loc(rr), realkind)
+      call lbm_check(ctx, rc, 'lbm_generate_turbulence')
+   end subroutine
 
    subroutine save_restart(itw)
       ! src/m_saverestart.F90:125-136 (and the turbulence fields, :66-83)

@@ -25,7 +25,7 @@ module lbm_hip
    public :: lbm_create, lbm_destroy, lbm_last_error, lbm_set_state, lbm_get_state, lbm_get_macros, &
              lbm_init_equilibrium, lbm_set_blanking, lbm_set_ablvisc, lbm_set_inflow, &
              lbm_set_inflow_series, lbm_set_turbulence, lbm_set_external_forcing, lbm_step, &
-             lbm_synchronize, lbm_get_totals, lbm_check, lbm_error_message
+             lbm_synchronize, lbm_get_totals, lbm_check, lbm_error_message, lbm_generate_turbulence
 
    interface
       integer(c_int) function lbm_create(desc, ctx) bind(C, name='lbm_create')
@@ -90,6 +90,16 @@ module lbm_hip
          type(c_ptr), value :: ctx, uu, vv, ww
          integer(c_int), value :: nrturb
          real(c_double), value :: ampl
+         integer(c_int), value :: realkind
+      end function
+      integer(c_int) function lbm_generate_turbulence(ctx, nrturb, ampl, seed, uu, vv, ww, rr, realkind) &
+            bind(C, name='lbm_generate_turbulence')
+         import :: c_int, c_ptr, c_double, c_int64_t
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: nrturb
+         real(c_double), value :: ampl
+         integer(c_int64_t), value :: seed
+         type(c_ptr), value :: uu, vv, ww, rr
          integer(c_int), value :: realkind
       end function
       integer(c_int) function lbm_set_external_forcing(ctx, force, realkind) bind(C, name='lbm_set_external_forcing')
