@@ -210,7 +210,7 @@ def bench_adjoint(a):
         cv = prob.c0.clone().requires_grad_(True)
         J = prob.J(av, cv)
         J.backward()
-        return float(J), av.grad.numpy().copy(), cv.grad.numpy().copy()
+        return float(J.detach()), av.grad.numpy().copy(), cv.grad.numpy().copy()
     J32, ga32, gc32 = grads(p32)
     p32.close()
     p64 = ForcingProblem(storage="f64", nsteps=nsteps, every=20)

@@ -38,7 +38,7 @@ def adj_grad(prob, a, c):
     c = c.clone().requires_grad_(True)
     J = prob.J(a, c)
     J.backward()
-    return float(J), a.grad.numpy(), c.grad.numpy()
+    return float(J.detach()), a.grad.numpy(), c.grad.numpy()
 
 
 WHICH = [("a", 0), ("a", 3), ("a", 5), ("c", 0), ("c", 1), ("c", 4)]
