@@ -28,29 +28,32 @@
 
 namespace lbm {
 
-// Adjoint of one cell.  D: departures g_l - w_l of the streamed (pre-collision) populations;
-// ly: adjoint of the post-collision populations; kappa: weight of the kinetic energy
+// Adjoint of one cell.  Dg(l): departure g_l - w_l of the streamed (pre-collision) population l;
+// Lg(l): adjoint of the post-collision population l; kappa: weight of the kinetic energy
 // 0.5*rho*|u|^2 of g in the objective (0: not in the objective box); coll = false: objective term
-// only.  Out: lg (adjoint of g), lF (adjoint of the body force of this cell).
-template <int Q, bool BGK3, bool HRR, bool VRE, typename R>
-__host__ __device__ inline void collide_adjoint(const StepArgs<R>& a, const R (&D)[Q], int k, int oc, bool solid,
-                                                const R (&ly_in)[Q], R kappa, bool coll, R (&lg)[Q], R (&lF)[3]) {
+// only.  The adjoint of g_l goes to sink(l, value); lF receives the adjoint of the cell's body
+// force.  The per-direction loops are rolled (the lattice constants are uniform table reads) so
+// that the cell's state stays small; Dg and Lg may be called more than once per direction.
+template <int Q, bool BGK3, bool HRR, bool VRE, typename R, class DG, class LG, class Sink>
+__host__ __device__ inline void collide_adjoint_f(const StepArgs<R>& a, DG&& Dg, int k, int oc, bool solid, LG&& Lg,
+                                                  R kappa, bool coll, Sink&& sink, R (&lF)[3]) {
     using Lt = Lattice<Q>;
     // ---- forward recompute (collide_cell's quantities, in plain rather than paired form)
     R dr = 0, jx = 0, jy = 0, jz = 0, pxx = 0, pyy = 0, pzz = 0, pxy = 0, pxz = 0, pyz = 0;
 #pragma unroll
     for (int l = 0; l < Q; ++l) {
         const R cx = R(Lt::cx(l)), cy = R(Lt::cy(l)), cz = R(Lt::cz(l));
-        dr += D[l];
-        jx += cx * D[l];
-        jy += cy * D[l];
-        jz += cz * D[l];
-        pxx += cx * cx * D[l];
-        pyy += cy * cy * D[l];
-        pzz += cz * cz * D[l];
-        pxy += cx * cy * D[l];
-        pxz += cx * cz * D[l];
-        pyz += cy * cz * D[l];
+        const R d = Dg(l);
+        dr += d;
+        jx += cx * d;
+        jy += cy * d;
+        jz += cz * d;
+        pxx += cx * cx * d;
+        pyy += cy * cy * d;
+        pzz += cz * cz * d;
+        pxy += cx * cy * d;
+        pxz += cx * cz * d;
+        pyz += cy * cz * d;
     }
     const R rho = R(1) + dr;
     const R inv = R(1) / rho;
@@ -59,9 +62,8 @@ __host__ __device__ inline void collide_adjoint(const StepArgs<R>& a, const R (&
     lF[0] = lF[1] = lF[2] = R(0);
     if (!coll) {
         // objective only: d(0.5 |j|^2 / rho)/dg_l = c.u - |u|^2/2
-#pragma unroll
-        for (int l = 0; l < Q; ++l)
-            lg[l] = kappa * (sdot<R>(Lt::cx(l), Lt::cy(l), Lt::cz(l), ux, uy, uz) - R(0.5) * usq);
+#pragma unroll 1
+        for (int l = 0; l < Q; ++l) sink(l, kappa * (sdot<R>(Lt::cx(l), Lt::cy(l), Lt::cz(l), ux, uy, uz) - R(0.5) * usq));
         return;
     }
     const R third = R(1) / R(3);
@@ -88,17 +90,8 @@ __host__ __device__ inline void collide_adjoint(const StepArgs<R>& a, const R (&
     const R gx = axx * ux + axy * uy + axz * uz, gy = axy * ux + ayy * uy + ayz * uz, gz = axz * ux + ayz * uy + azz * uz;
 
     // ---- reverse
-    R ly[Q];
-#pragma unroll
-    for (int l = 0; l < Q; ++l) ly[l] = ly_in[l];
-    if (solid) {
-#pragma unroll
-        for (int l = 1; l + 1 < Q; l += 2) {
-            const R s = ly[l];
-            ly[l] = ly[l + 1];
-            ly[l + 1] = s;
-        }
-    }
+    // adjoint of the population before the solid swap of pairs (l, l+1), l odd
+    auto ly = [&](int l) { return Lg(solid && l > 0 ? ((l & 1) ? l + 1 : l - 1) : l); };
     R lrho = 0, lux = 0, luy = 0, luz = 0;
     R lxx = 0, lyy = 0, lzz = 0, lxy = 0, lxz = 0, lyz = 0;   // adjoint of a2
     R lfac = 0;
@@ -109,33 +102,27 @@ __host__ __device__ inline void collide_adjoint(const StepArgs<R>& a, const R (&
         const R vx = ux - Fx * inv, vy = uy - Fy * inv, vz = uz - Fz * inv;
         const R vsq = vx * vx + vy * vy + vz * vz;
         R lvx = 0, lvy = 0, lvz = 0;
-#pragma unroll
+#pragma unroll 1
         for (int l = 0; l < Q; ++l) {
             const int cx = Lt::cx(l), cy = Lt::cy(l), cz = Lt::cz(l);
             const R w = weight<Q, R>(l);
             const R cv = sdot<R>(cx, cy, cz, vx, vy, vz), cu = sdot<R>(cx, cy, cz, ux, uy, uz);
             const R Pv = R(1) - R(1.5) * vsq + R(3) * cv + R(4.5) * cv * cv;
             const R Pu = R(1) - R(1.5) * usq + R(3) * cu + R(4.5) * cu * cu;
-            const R wl = w * ly[l];
+            const R wl = w * ly(l);
+            const R wr = wl * rho;
             lrho += wl * (Pv - Pu);
-            // grad P2(v) = -3 v + 3 c + 9 (c.v) c
-            const R gvx = R(-3) * vx + R(3) * R(cx) + R(9) * cv * R(cx);
-            const R gvy = R(-3) * vy + R(3) * R(cy) + R(9) * cv * R(cy);
-            const R gvz = R(-3) * vz + R(3) * R(cz) + R(9) * cv * R(cz);
-            const R gux = R(-3) * ux + R(3) * R(cx) + R(9) * cu * R(cx);
-            const R guy = R(-3) * uy + R(3) * R(cy) + R(9) * cu * R(cy);
-            const R guz = R(-3) * uz + R(3) * R(cz) + R(9) * cu * R(cz);
-            lvx += wl * rho * gvx;
-            lvy += wl * rho * gvy;
-            lvz += wl * rho * gvz;
-            lux -= wl * rho * gux;
-            luy -= wl * rho * guy;
-            luz -= wl * rho * guz;
+            // grad P2(v) - grad P2(u) = -3 (v - u) + 9 (c.v - c.u) c, and grad P2(v)
+            const R dc = R(9) * (cv - cu);
+            lux += wr * (R(-3) * (vx - ux) + dc * R(cx));
+            luy += wr * (R(-3) * (vy - uy) + dc * R(cy));
+            luz += wr * (R(-3) * (vz - uz) + dc * R(cz));
+            const R c9 = R(3) + R(9) * cv;
+            lvx += wr * (R(-3) * vx + c9 * R(cx));
+            lvy += wr * (R(-3) * vy + c9 * R(cy));
+            lvz += wr * (R(-3) * vz + c9 * R(cz));
         }
-        lux += lvx;
-        luy += lvy;
-        luz += lvz;
-        // dv/drho = F / rho^2, dv/dF = -1/rho
+        // dv/drho = F / rho^2, dv/dF = -1/rho (du/du contributions are in lu above)
         lrho += (lvx * Fx + lvy * Fy + lvz * Fz) * inv * inv;
         lF[0] = -lvx * inv;
         lF[1] = -lvy * inv;
@@ -143,25 +130,21 @@ __host__ __device__ inline void collide_adjoint(const StepArgs<R>& a, const R (&
     }
 
     // relaxation y = feq + fac * fneq
-    R lgd[Q];
-#pragma unroll
+#pragma unroll 1
     for (int l = 0; l < Q; ++l) {
         const int cx = Lt::cx(l), cy = Lt::cy(l), cz = Lt::cz(l);
+        const R fx = R(cx), fy = R(cy), fz = R(cz);
         const R w = weight<Q, R>(l);
         const R cu = sdot<R>(cx, cy, cz, ux, uy, uz);
         R P = R(1) - R(1.5) * usq + R(3) * cu + R(4.5) * cu * cu;
         // grad P = -3u + 3c + 9 cu c [+ 4.5 ((3 cu^2 - u.u) c - 2 cu u)]
-        R gpx = R(-3) * ux + R(3) * R(cx) + R(9) * cu * R(cx);
-        R gpy = R(-3) * uy + R(3) * R(cy) + R(9) * cu * R(cy);
-        R gpz = R(-3) * uz + R(3) * R(cz) + R(9) * cu * R(cz);
+        R gq = R(3) + R(9) * cu, gu = R(-3);
         if (BGK3 && l > 0) {
             P += R(4.5) * cu * (cu * cu - usq);
-            const R e = R(4.5) * (R(3) * cu * cu - usq), f2 = R(9) * cu;
-            gpx += e * R(cx) - f2 * ux;
-            gpy += e * R(cy) - f2 * uy;
-            gpz += e * R(cz) - f2 * uz;
+            gq += R(4.5) * (R(3) * cu * cu - usq);
+            gu -= R(9) * cu;
         }
-        const R feq = w * rho * P;
+        const R le = ly(l);
         if (HRR) {
             R cac = R(0);
             if (cx != 0) cac += axx;
@@ -171,37 +154,38 @@ __host__ __device__ inline void collide_adjoint(const StepArgs<R>& a, const R (&
             const R cg = sdot<R>(cx, cy, cz, gx, gy, gz);
             const R w45 = R(4.5) * w;
             const R fneq = w45 * ((cac - T * third) + (cu * (R(3) * cac - T) - R(2) * cg));
-            const R le = ly[l];
-            lrho += le * w * P;
-            lux += le * w * rho * gpx;
-            luy += le * w * rho * gpy;
-            luz += le * w * rho * gpz;
+            const R lw = le * w;
+            const R lwr = lw * rho;
+            lrho += lw * P;
+            lux += lwr * (gu * ux + gq * fx);
+            luy += lwr * (gu * uy + gq * fy);
+            luz += lwr * (gu * uz + gq * fz);
             lfac += le * fneq;
             const R phi = fac * le * w45;
-            const R fx = R(cx), fy = R(cy), fz = R(cz);
             // d fneq / d a2 (six symmetric entries)
+            const R c2 = R(2) * cu;
             lxx += phi * ((fx * fx - third) + cu * (R(3) * fx * fx - R(1)) - R(2) * fx * ux);
             lyy += phi * ((fy * fy - third) + cu * (R(3) * fy * fy - R(1)) - R(2) * fy * uy);
             lzz += phi * ((fz * fz - third) + cu * (R(3) * fz * fz - R(1)) - R(2) * fz * uz);
-            lxy += phi * (R(2) * fx * fy + cu * (R(6) * fx * fy) - R(2) * (fx * uy + fy * ux));
-            lxz += phi * (R(2) * fx * fz + cu * (R(6) * fx * fz) - R(2) * (fx * uz + fz * ux));
-            lyz += phi * (R(2) * fy * fz + cu * (R(6) * fy * fz) - R(2) * (fy * uz + fz * uy));
+            lxy += phi * (R(2) * fx * fy + R(3) * c2 * fx * fy - R(2) * (fx * uy + fy * ux));
+            lxz += phi * (R(2) * fx * fz + R(3) * c2 * fx * fz - R(2) * (fx * uz + fz * ux));
+            lyz += phi * (R(2) * fy * fz + R(3) * c2 * fy * fz - R(2) * (fy * uz + fz * uy));
             // d fneq / d u = c (3 cAc - T) - 2 a2 c
             const R s3 = R(3) * cac - T;
             lux += phi * (fx * s3 - R(2) * (axx * fx + axy * fy + axz * fz));
             luy += phi * (fy * s3 - R(2) * (axy * fx + ayy * fy + ayz * fz));
             luz += phi * (fz * s3 - R(2) * (axz * fx + ayz * fy + azz * fz));
-            lgd[l] = R(0);
         } else {
             // y = feq + fac (g - feq)
-            const R le = ly[l];
+            const R feq = w * rho * P;
             const R lfe = (R(1) - fac) * le;
-            lrho += lfe * w * P;
-            lux += lfe * w * rho * gpx;
-            luy += lfe * w * rho * gpy;
-            luz += lfe * w * rho * gpz;
-            lfac += le * ((w + D[l]) - feq);
-            lgd[l] = fac * le;
+            const R lw = lfe * w;
+            const R lwr = lw * rho;
+            lrho += lw * P;
+            lux += lwr * (gu * ux + gq * fx);
+            luy += lwr * (gu * uy + gq * fy);
+            luz += lwr * (gu * uz + gq * fz);
+            lfac += le * ((w + Dg(l)) - feq);
         }
     }
 
@@ -231,17 +215,27 @@ __host__ __device__ inline void collide_adjoint(const StepArgs<R>& a, const R (&
     // u = j / rho
     const R ljx = lux * inv, ljy = luy * inv, ljz = luz * inv;
     lrho -= (lux * ux + luy * uy + luz * uz) * inv;
-#pragma unroll
+    const R kx = kappa * ux, ky = kappa * uy, kz = kappa * uz, k0 = lrho - R(0.5) * kappa * usq;
+#pragma unroll 1
     for (int l = 0; l < Q; ++l) {
         const int cx = Lt::cx(l), cy = Lt::cy(l), cz = Lt::cz(l);
-        R v = lgd[l] + lrho + sdot<R>(cx, cy, cz, ljx, ljy, ljz);
+        R v = k0 + sdot<R>(cx, cy, cz, ljx + kx, ljy + ky, ljz + kz);
         if (cx != 0) v += lxx;
         if (cy != 0) v += lyy;
         if (cz != 0) v += lzz;
         v += sdot<R>(cx * cy, cx * cz, cy * cz, lxy, lxz, lyz);
-        if (kappa != R(0)) v += kappa * (sdot<R>(cx, cy, cz, ux, uy, uz) - R(0.5) * usq);
-        lg[l] = v;
+        if (!HRR) v += fac * ly(l);
+        sink(l, v);
     }
+}
+
+// Array form (host checks).
+template <int Q, bool BGK3, bool HRR, bool VRE, typename R>
+__host__ __device__ inline void collide_adjoint(const StepArgs<R>& a, const R (&D)[Q], int k, int oc, bool solid,
+                                                const R (&ly_in)[Q], R kappa, bool coll, R (&lg)[Q], R (&lF)[3]) {
+    collide_adjoint_f<Q, BGK3, HRR, VRE, R>(
+        a, [&](int l) { return D[l]; }, k, oc, solid, [&](int l) { return ly_in[l]; }, kappa, coll,
+        [&](int l, R v) { lg[l] = v; }, lF);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -269,29 +263,33 @@ __global__ void __launch_bounds__(256) adjoint_step_kernel(const StepArgs<typena
     const int n[3] = {L.nx, L.ny, L.nz};
     const int per[3] = {L.per_i, L.per_j, L.per_k};
     const raw* __restrict__ src = static_cast<const raw*>(a.src);
-    R D[Q];
-    int so[Q];
-#pragma unroll
-    for (int l = 0; l < Q; ++l) {
-        so[l] = L.cell(Layout::wrap(i - Lt::cx(l), L.nx, L.per_i), Layout::wrap(j - Lt::cy(l), L.ny, L.per_j),
-                       Layout::wrap(k - Lt::cz(l), L.nz, L.per_k));
-        D[l] = C::template dep<Q>(src[(size_t)l * L.plane + so[l]], l);
-    }
+    // source cell of direction l (periodic axes folded)
+    auto source = [&](int l) {
+        return L.cell(Layout::wrap(i - Lt::cx(l), L.nx, L.per_i), Layout::wrap(j - Lt::cy(l), L.ny, L.per_j),
+                      Layout::wrap(k - Lt::cz(l), L.nz, L.per_k));
+    };
+    auto Dg = [&](int l) { return C::template dep<Q>(src[(size_t)l * L.plane + source(l)], l); };
     const int oc = L.cell(i, j, k);
-    R ly[Q];
-#pragma unroll
-    for (int l = 0; l < Q; ++l) ly[l] = coll ? C::template decode<Q>(lam_y[(size_t)l * L.plane + oc], l) : R(0);
+    auto Lg = [&](int l) { return C::template decode<Q>(lam_y[(size_t)l * L.plane + oc], l); };
     bool solid = false;
     if (a.solid != nullptr) {
         const uint32_t word = a.solid[((cc - 1) * L.n(L.ax1) + (cb - 1)) * a.solid_wpr + ((ca - 1) >> 5)];
         solid = (word >> ((ca - 1) & 31)) & 1u;
     }
     const bool inbox = i >= box.lo[0] && i <= box.hi[0] && j >= box.lo[1] && j <= box.hi[1] && k >= box.lo[2] && k <= box.hi[2];
-    R lg[Q], lF[3];
-    collide_adjoint<Q, BGK3, HRR, VRE, R>(a, D, k, oc, solid, ly, inbox ? kappa : R(0), coll != 0, lg, lF);
-    raw* __restrict__ lx = lam_x;
+    // a cell next to a non-periodic face: some sources are ghosts, some own slots are unhit
+    bool edge = false;
 #pragma unroll
-    for (int l = 0; l < Q; ++l) {
+    for (int d = 0; d < 3; ++d)
+        if (!per[d] && (x[d] == 1 || x[d] == n[d])) edge = true;
+    raw* __restrict__ lx = lam_x;
+    R lF[3];
+    auto sink = [&](int l, R v) {
+        raw* p = lx + (size_t)l * L.plane + source(l);
+        if (!edge) {
+            *p = C::template encode<Q>(v, l);
+            return;
+        }
         // the source slot: interior -> assign; ghost -> accumulate
         bool ghost = false, unhit = false;
 #pragma unroll
@@ -302,12 +300,12 @@ __global__ void __launch_bounds__(256) adjoint_step_kernel(const StepArgs<typena
             if (s < 1 || s > n[d]) ghost = true;
             if (t < 1 || t > n[d]) unhit = true;
         }
-        raw* p = lx + (size_t)l * L.plane + so[l];
-        if (ghost) *p = C::template encode<Q>(C::template decode<Q>(*p, l) + lg[l], l);
-        else *p = C::template encode<Q>(lg[l], l);
+        if (ghost) *p = C::template encode<Q>(C::template decode<Q>(*p, l) + v, l);
+        else *p = C::template encode<Q>(v, l);
         // this cell's own slot of direction l is pulled by no interior cell: clear it
         if (unhit) lx[(size_t)l * L.plane + oc] = C::template encode<Q>(R(0), l);
-    }
+    };
+    collide_adjoint_f<Q, BGK3, HRR, VRE, R>(a, Dg, k, oc, solid, Lg, inbox ? kappa : R(0), coll != 0, sink, lF);
     if (gF != nullptr && a.Fx != nullptr) {
         gF[oc] += (double)lF[0];
         gF[(size_t)L.plane + oc] += (double)lF[1];
@@ -318,7 +316,7 @@ __global__ void __launch_bounds__(256) adjoint_step_kernel(const StepArgs<typena
 // Reverse of the inflow/outflow pass (inflow_cell) of step n on the adjoint buffer lam of X(n).
 // gS: FP64 accumulators of d/d uvel_shear per (j,k) (j fastest).
 template <int Q, class C>
-__global__ void adjoint_inflow_kernel(Layout L, typename C::raw* lam, typename C::real utime, typename C::real cosd,
+__global__ void __launch_bounds__(64) adjoint_inflow_kernel(Layout L, typename C::raw* lam, typename C::real utime, typename C::real cosd,
                                       typename C::real sind, typename C::real rho0, double* gS) {
     using R = typename C::real;
     using Lt = Lattice<Q>;
@@ -326,11 +324,13 @@ __global__ void adjoint_inflow_kernel(Layout L, typename C::raw* lam, typename C
     const int k = blockIdx.y + 1;
     if (j > L.ny || k > L.nz) return;
     const int o1 = L.cell(1, j, k), o0 = L.cell(0, j, k), on = L.cell(L.nx, j, k), oe = L.cell(L.nx + 1, j, k);
-    R l0[Q], le[Q], lfg[Q];
+    R l0[Q], le[Q], l1[Q], ln[Q], lfg[Q];
 #pragma unroll
     for (int l = 0; l < Q; ++l) {
         l0[l] = C::template decode<Q>(lam[(size_t)l * L.plane + o0], l);
         le[l] = C::template decode<Q>(lam[(size_t)l * L.plane + oe], l);
+        l1[l] = C::template decode<Q>(lam[(size_t)l * L.plane + o1], l);
+        ln[l] = C::template decode<Q>(lam[(size_t)l * L.plane + on], l);
         lfg[l] = R(0);
     }
 #pragma unroll
@@ -340,8 +340,8 @@ __global__ void adjoint_inflow_kernel(Layout L, typename C::raw* lam, typename C
     for (int l = 0; l < Q; ++l) {
         // fg_l = f_l - 2 w rho0 (cx cos + cy sin) uu / cs2,  uu = utime * shear(k)
         s += lfg[l] * (R(-6) * weight<Q, R>(l) * rho0 * (R(Lt::cx(l)) * cosd + R(Lt::cy(l)) * sind));
-        lam[(size_t)l * L.plane + o1] = C::template encode<Q>(C::template decode<Q>(lam[(size_t)l * L.plane + o1], l) + lfg[l], l);
-        lam[(size_t)l * L.plane + on] = C::template encode<Q>(C::template decode<Q>(lam[(size_t)l * L.plane + on], l) + le[l], l);
+        lam[(size_t)l * L.plane + o1] = C::template encode<Q>(l1[l] + lfg[l], l);
+        lam[(size_t)l * L.plane + on] = C::template encode<Q>(ln[l] + le[l], l);
         lam[(size_t)l * L.plane + o0] = C::template encode<Q>(R(0), l);
         lam[(size_t)l * L.plane + oe] = C::template encode<Q>(R(0), l);
     }
@@ -352,7 +352,7 @@ __global__ void adjoint_inflow_kernel(Layout L, typename C::raw* lam, typename C
 // lam_x adjoint of X(n-1).  One thread per ghost cell of the two faces (tangential coordinates
 // interior), t0/t1 the tangential axes.
 template <int Q, class C>
-__global__ void adjoint_face_kernel(Layout L, typename C::raw* lam_y, typename C::raw* lam_x, int D, int t0, int t1,
+__global__ void __launch_bounds__(64) adjoint_face_kernel(Layout L, typename C::raw* lam_y, typename C::raw* lam_x, int D, int t0, int t1,
                                     int opt_lo, int opt_hi) {
     using R = typename C::real;
     using Lt = Lattice<Q>;
@@ -369,26 +369,37 @@ __global__ void adjoint_face_kernel(Layout L, typename C::raw* lam_y, typename C
     const int plane = side ? n[D] : 1;
     const int opt = side ? opt_hi : opt_lo;
     const int og = L.cell(g[0], g[1], g[2]);
+    // loads first: the adjoint of the written slots (l, G), (m, G), of X(n-1)'s (l, G), and of
+    // the wall-plane sources
+    R lm[Q], ll[Q], lxl[Q], lsv[Q];
+    size_t os[Q];
 #pragma unroll
     for (int l = 0; l < Q; ++l) {
         const int cn = Lt::c(l, D);
         if (!(side ? cn > 0 : cn < 0)) continue;
         const int m = opt < 0 ? mirror<Q>(l, D, -1) : mirror<Q>(l, D, +1);
-        const R lm = C::template decode<Q>(lam_y[(size_t)m * L.plane + og], m);
-        const R ll = C::template decode<Q>(lam_y[(size_t)l * L.plane + og], l);
-        // (m, G) was copied from X(n-1)'s (l, G)
-        lam_x[(size_t)l * L.plane + og] = C::template encode<Q>(C::template decode<Q>(lam_x[(size_t)l * L.plane + og], l) + lm, l);
-        // (l, G) came from the wall-plane cell G - c_l (tangential), an interior cell or a ghost of
-        // the inflow axis
-        int s[3];
+        int sc[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            if (d == D) { s[d] = plane; continue; }
-            s[d] = g[d] - Lt::c(l, d);
-            if (per[d]) s[d] = Layout::wrap(s[d], n[d], 1);
+            if (d == D) { sc[d] = plane; continue; }
+            sc[d] = g[d] - Lt::c(l, d);
+            if (per[d]) sc[d] = Layout::wrap(sc[d], n[d], 1);
         }
-        const size_t os = (size_t)l * L.plane + L.cell(s[0], s[1], s[2]);
-        lam_y[os] = C::template encode<Q>(C::template decode<Q>(lam_y[os], l) + ll, l);
+        os[l] = (size_t)l * L.plane + L.cell(sc[0], sc[1], sc[2]);
+        lm[l] = C::template decode<Q>(lam_y[(size_t)m * L.plane + og], m);
+        ll[l] = C::template decode<Q>(lam_y[(size_t)l * L.plane + og], l);
+        lxl[l] = C::template decode<Q>(lam_x[(size_t)l * L.plane + og], l);
+        lsv[l] = C::template decode<Q>(lam_y[os[l]], l);
+    }
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+        const int cn = Lt::c(l, D);
+        if (!(side ? cn > 0 : cn < 0)) continue;
+        const int m = opt < 0 ? mirror<Q>(l, D, -1) : mirror<Q>(l, D, +1);
+        // (m, G) was copied from X(n-1)'s (l, G); (l, G) came from the wall-plane cell G - c_l
+        // (tangential): an interior cell or a ghost of the inflow axis
+        lam_x[(size_t)l * L.plane + og] = C::template encode<Q>(lxl[l] + lm[l], l);
+        lam_y[os[l]] = C::template encode<Q>(lsv[l] + ll[l], l);
         lam_y[(size_t)m * L.plane + og] = C::template encode<Q>(R(0), m);
         lam_y[(size_t)l * L.plane + og] = C::template encode<Q>(R(0), l);
     }
@@ -407,10 +418,15 @@ __global__ void __launch_bounds__(256) objective_kernel(Layout L, const void* f,
     const long long nb = (long long)bx * by * bz;
     const raw* __restrict__ p = static_cast<const raw*>(f);
     double sum = 0.0, dsum = 0.0;
+    // walk the box in storage order (contiguous axis fastest)
+    const int ext[3] = {bx, by, bz};
+    const int e0 = ext[L.ax0], e1 = ext[L.ax1];
     for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < nb; e += (long long)gridDim.x * blockDim.x) {
-        const int i = box.lo[0] + (int)(e % bx);
-        const int j = box.lo[1] + (int)((e / bx) % by);
-        const int k = box.lo[2] + (int)(e / ((long long)bx * by));
+        int c[3];
+        c[L.ax0] = box.lo[L.ax0] + (int)(e % e0);
+        c[L.ax1] = box.lo[L.ax1] + (int)((e / e0) % e1);
+        c[L.ax2] = box.lo[L.ax2] + (int)(e / ((long long)e0 * e1));
+        const int i = c[0], j = c[1], k = c[2];
         R dr = 0, jx = 0, jy = 0, jz = 0;
 #pragma unroll
         for (int l = 0; l < Q; ++l) {

@@ -306,7 +306,12 @@ struct Launch {
 
     static hipError_t adj_face(const Layout& L, void* ly, void* lx, int dir, int opt_lo, int opt_hi, hipStream_t s) {
         if constexpr (kAdj) {
-            const int t0 = dir == 0 ? 1 : 0, t1 = dir == 2 ? 1 : 2;
+            // tangential axes in storage order: threads run along the first
+            const int order[3] = {L.ax0, L.ax1, L.ax2};
+            int tt[2], q = 0;
+            for (int a = 0; a < 3; ++a)
+                if (order[a] != dir) tt[q++] = order[a];
+            const int t0 = tt[0], t1 = tt[1];
             const dim3 grid((L.n(t0) + 63) / 64, L.n(t1), 2);
             hipLaunchKernelGGL((adjoint_face_kernel<Q, C>), grid, dim3(64), 0, s, L, static_cast<typename C::raw*>(ly),
                                static_cast<typename C::raw*>(lx), dir, t0, t1, opt_lo, opt_hi);
