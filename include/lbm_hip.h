@@ -136,6 +136,16 @@ void* lbm_stream(lbm_ctx* ctx);
 int lbm_set_timing(lbm_ctx* ctx, int on);
 int lbm_timing_read(lbm_ctx* ctx, double* step_ms, int* n_step, double* bc_ms, int* n_bc);
 
+/* Time averaging, replacing averaging_full (src/m_averaging_full.F90:14-97 with its kernels
+ * src/m_averaging_full_kernel.F90:6-45 and src/m_averaging_full_kernelfin.F90:6-63): after
+ * lbm_set_averaging(ctx, 1) every step adds u, v, w, u^2, v^2, w^2 of the moments after it to
+ * per-cell sums, accumulated inside the next step's kernel, which computes those moments anyway.
+ * lbm_get_averages returns the means and Ti = sqrt((sum_c max(<c^2> - <c>^2, 0)) / 3) for the
+ * interior cells (layout of rho, ghosts 0) and the number of states averaged; averaging stays on
+ * until lbm_set_averaging(ctx, 0).  Not available inside ensembles or adjoint runs. */
+int lbm_set_averaging(lbm_ctx* ctx, int on);
+int lbm_get_averages(lbm_ctx* ctx, void* uave, void* vave, void* wave, void* ti, int* count, int realkind);
+
 /* Global mass and momentum of the interior state in FP64 (accumulated on the device). */
 int lbm_get_totals(lbm_ctx* ctx, double* mass, double* momentum3);
 

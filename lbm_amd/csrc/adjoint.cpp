@@ -124,6 +124,7 @@ int lbm_adjoint_run(lbm_ctx* c, int it0, int nsteps, int every, const int32_t* b
         return fail(c, LBM_ESTATE, "the adjoint starts from a stepped state: call lbm_step once after set_state / init_equilibrium");
     if (!c->ops->adj_step) return fail(c, LBM_EINVAL, "the adjoint needs FP32 or FP64 storage");
     if (c->has_turb) return fail(c, LBM_EINVAL, "the adjoint does not differentiate the inflow-turbulence forcing: switch it off");
+    if (c->avg_on) return fail(c, LBM_ESTATE, "switch time averaging off before an adjoint run");
     const int closed = (d.ibnd > 10) + (d.jbnd > 10) + (d.kbnd > 10);
     if (closed > 1 || (closed == 1 && c->fold_dir < 0))
         return fail(c, LBM_EINVAL, "the adjoint supports at most one closed axis (its face pass runs inside the step kernel)");

@@ -483,6 +483,9 @@ int lbm_step(lbm_ctx* c, int it0, int nsteps) {
             h.stream = 1;
         }
         for (int q = 0; q < 4; ++q) h.ext[q] = c->use_ext ? c->ext[q].p : nullptr;
+        // the step kernel adds the moments of its input (the state after the previous step)
+        h.avg = (c->avg_on && c->avg_pending) ? c->avg.p : nullptr;
+        if (h.avg) ++c->avg_count;
         if (c->has_turb) {
             int lit = it % c->nrturb;           // src/m_inflow_turbulence_forcing.F90:45-46
             if (lit == 0) lit = c->nrturb;
@@ -502,7 +505,87 @@ int lbm_step(lbm_ctx* c, int it0, int nsteps) {
         c->use_ext = false;
         if (int rc = boundary(c, c->buf[c->cur], c->buf[c->cur ^ 1], it, c->fold_dir)) return rc;
         c->ghost_raw[c->cur] = false;
+        c->avg_pending = c->avg_on;
     }
+    return LBM_OK;
+}
+
+namespace {
+// Adds the current state's moments to the averaging sums if they are not there yet.
+int average_flush(lbm_ctx* c) {
+    if (!c->avg_on || !c->avg_pending) return LBM_OK;
+    HIPCHK(c, c->ops->average(c->lay, c->buf[c->cur], c->pending ? 0 : 1, c->avg.p, c->stream));
+    ++c->avg_count;
+    c->avg_pending = false;
+    return LBM_OK;
+}
+}  // namespace
+
+int lbm_set_averaging(lbm_ctx* c, int on) {
+    if (!c) return fail(c, LBM_EINVAL, "null context");
+    if (on) {
+        const size_t n = 6 * (size_t)c->lay.plane * c->ops->real_bytes;
+        if (int rc = dev_alloc(c, c->avg, n)) return rc;
+        HIPCHK(c, hipMemsetAsync(c->avg.p, 0, n, c->stream));
+        c->avg_on = true;
+        c->avg_pending = false;
+        c->avg_count = 0;
+        return LBM_OK;
+    }
+    if (int rc = average_flush(c)) return rc;
+    c->avg_on = false;
+    return LBM_OK;
+}
+
+int lbm_get_averages(lbm_ctx* c, void* uave, void* vave, void* wave, void* ti, int* count, int realkind) {
+    if (!c) return fail(c, LBM_EINVAL, "null context");
+    if (int rc = check_kind(c, realkind)) return rc;
+    if (!c->avg.p) return fail(c, LBM_ESTATE, "no averages: call lbm_set_averaging first");
+    if (int rc = average_flush(c)) return rc;
+    const Layout& L = c->lay;
+    const int rb = c->ops->real_bytes;
+    const size_t P = (size_t)L.plane;
+    std::vector<unsigned char> img(6 * P * rb);
+    HIPCHK(c, hipMemcpyAsync(img.data(), c->avg.p, img.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (count) *count = c->avg_count;
+    const size_t nc = ncell_padded(c);
+    void* out[4] = {uave, vave, wave, ti};
+    for (int q = 0; q < 4; ++q)
+        if (out[q]) std::memset(out[q], 0, nc * realkind);
+    if (c->avg_count == 0) return LBM_OK;
+    auto put = [&](void* o, size_t idx, double v) {
+        if (!o) return;
+        if (realkind == 8) static_cast<double*>(o)[idx] = v;
+        else static_cast<float*>(o)[idx] = (float)v;
+    };
+    const bool f32 = rb == 4;
+    for (int k = 1; k <= L.nz; ++k)
+        for (int j = 1; j <= L.ny; ++j)
+            for (int i = 1; i <= L.nx; ++i) {
+                const size_t o = (size_t)L.cell(i, j, k);
+                const size_t hc = ((size_t)k * (L.ny + 2) + j) * (L.nx + 2) + i;
+                double m[3], m2[3];
+                for (int q = 0; q < 3; ++q) {
+                    m[q] = get_real(img.data(), q * P + o, rb) / c->avg_count;
+                    m2[q] = get_real(img.data(), (3 + q) * P + o, rb) / c->avg_count;
+                    if (f32) {
+                        m[q] = (float)m[q];
+                        m2[q] = (float)m2[q];
+                    }
+                }
+                // src/m_averaging_full_kernelfin.F90:46-60
+                double t = 0.0;
+                for (int q = 0; q < 3; ++q) {
+                    const double var = f32 ? (double)((float)m2[q] - (float)m[q] * (float)m[q]) : m2[q] - m[q] * m[q];
+                    t += var > 0.0 ? var : 0.0;
+                }
+                t = f32 ? (double)((float)t / 3.0f) : t / 3.0;
+                put(uave, hc, m[0]);
+                put(vave, hc, m[1]);
+                put(wave, hc, m[2]);
+                put(ti, hc, f32 ? (double)std::sqrt((float)(t > 0.0 ? t : 0.0)) : std::sqrt(t > 0.0 ? t : 0.0));
+            }
     return LBM_OK;
 }
 

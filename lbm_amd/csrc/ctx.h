@@ -46,6 +46,11 @@ struct lbm_ctx {
     std::vector<double> ut_series, ud_series;
     double tau0 = 0.0, fac0 = 0.0;
     double obj_tangent = 0.0;      // directional derivative of the last objective (tangent storage)
+    // time averaging: sums of u, v, w, u^2, v^2, w^2 (6 cell planes of the compute type)
+    lbm_host::DevBuf avg;
+    bool avg_on = false;
+    bool avg_pending = false;      // the current state's moments are not in the sums yet
+    int avg_count = 0;
     int fold_dir = -1;             // closed axis whose face pass the step kernel performs
     int variant = 1;               // step-kernel variant (LBM_STEP_VARIANT), see impl.h; non-temporal
                                    // stores measured fastest on MI355X (5.32 vs 5.11 TB/s)

@@ -98,6 +98,7 @@ int build_tables(lbm_ens* e) {
     long long total = 0, itotal = 0;
     for (int m = 0; m < n; ++m) {
         lbm_ctx* c = e->mem[m];
+        if (c->avg_on) return efail(e, LBM_ESTATE, "member " + std::to_string(m) + ": time averaging is not batched; switch it off");
         MemberHost mh{};
         mh.lay = c->lay;
         mh.h = step_host(c);

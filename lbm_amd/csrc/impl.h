@@ -44,6 +44,7 @@ struct StepHost {
     const void* turb[3];
     double ampl;
     const void* F[3];
+    void* avg;            // averaging sums (6 cell planes of the compute type) or nullptr
     void* tau_out;
     int fold_dir, fold_opt_lo, fold_opt_hi;   // closed face performed by the step kernel
     int bgk3, hrr, vre;
@@ -98,6 +99,7 @@ struct Ops {
                            int realkind, void* dst, void* mr, void* mu, void* mv, void* mw, hipStream_t);
     hipError_t (*totals)(const Layout&, const void* cur, int stream_interior, double* partial, int nblocks,
                          hipStream_t);
+    hipError_t (*average)(const Layout&, const void* cur, int stream_interior, void* avg, hipStream_t);
     // ensembles
     size_t member_bytes;   // sizeof(MemberTab<R>)
     void (*ens_pack)(const MemberHost&, int block, void* out, int* tiles, int* inflow_tiles);

@@ -47,6 +47,8 @@ struct StepArgs {
     const R* tv;
     const R* tw;
     R ampl;
+    R* avg;                // six cell planes: sums of u, v, w, u^2, v^2, w^2 of the streamed
+                           //   input (the moments after the previous step); nullptr: off
     const R* Fx;           // external forcing (cell layout); nullptr: off
     const R* Fy;
     const R* Fz;
@@ -142,6 +144,17 @@ __host__ __device__ __forceinline__ void collide_cell(const StepArgs<typename C:
     R rh = R(1) + dr;
     const R invrho = R(1) / rh;
     R ux = (jx * invS) * invrho, uy = (jy * invS) * invrho, uz = (jz * invS) * invrho;
+    if (a.avg != nullptr) {
+        // time averaging of the moments after the previous step (src/m_averaging_full_kernel.F90:33-39)
+        R* p = a.avg + oc;
+        const size_t P = (size_t)L.plane;
+        p[0] += ux;
+        p[P] += uy;
+        p[2 * P] += uz;
+        p[3 * P] += ux * ux;
+        p[4 * P] += uy * uy;
+        p[5 * P] += uz * uz;
+    }
     if (a.ext_rho) { rh = a.ext_rho[oc]; ux = a.ext_u[oc]; uy = a.ext_v[oc]; uz = a.ext_w[oc]; }
     const R usq = ux * ux + uy * uy + uz * uz;
     const R base = R(1) - R(1.5) * usq;
@@ -854,6 +867,41 @@ __global__ void feq_init_kernel(Layout L, const H* rho, const H* u, const H* v, 
 
 // Global mass and momentum of the state (interior cells) in FP64: per-block partial sums,
 // finished on the host.  partial[4*block + {0,1,2,3}] = (mass, mx, my, mz).
+// Adds the moments of the current state (streamed when stream_interior) to the averaging sums.
+template <int Q, class C>
+__global__ void average_kernel(Layout L, const void* cur, int stream_interior, typename C::real* avg) {
+    using R = typename C::real;
+    using Lt = Lattice<Q>;
+    const int ca = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int cb = blockIdx.y + 1;
+    const int cc = blockIdx.z + 1;
+    if (ca > L.n(L.ax0)) return;
+    int i, j, k;
+    L.ijk(ca, cb, cc, i, j, k);
+    const int s = stream_interior;
+    R dr = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+        const R d = C::template dep<Q>(static_cast<const typename C::raw*>(cur)[(size_t)l * L.plane +
+                                           L.cell_w(i - s * Lt::cx(l), j - s * Lt::cy(l), k - s * Lt::cz(l))], l) /
+                    C::kDepScale;
+        dr += d;
+        jx += R(Lt::cx(l)) * d;
+        jy += R(Lt::cy(l)) * d;
+        jz += R(Lt::cz(l)) * d;
+    }
+    const R inv = R(1) / (R(1) + dr);
+    const R ux = jx * inv, uy = jy * inv, uz = jz * inv;
+    R* p = avg + L.cell(i, j, k);
+    const size_t P = (size_t)L.plane;
+    p[0] += ux;
+    p[P] += uy;
+    p[2 * P] += uz;
+    p[3 * P] += ux * ux;
+    p[4 * P] += uy * uy;
+    p[5 * P] += uz * uz;
+}
+
 template <int Q, class C>
 __global__ void __launch_bounds__(256) totals_kernel(Layout L, const void* cur, int stream_interior, double* partial) {
     using Lt = Lattice<Q>;

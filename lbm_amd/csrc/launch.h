@@ -89,6 +89,7 @@ struct Launch {
         a.tv = static_cast<const R*>(h.turb[1]);
         a.tw = static_cast<const R*>(h.turb[2]);
         a.ampl = (R)h.ampl;
+        a.avg = static_cast<R*>(h.avg);
         a.Fx = static_cast<const R*>(h.F[0]);
         a.Fy = static_cast<const R*>(h.F[1]);
         a.Fz = static_cast<const R*>(h.F[2]);
@@ -210,6 +211,12 @@ struct Launch {
         return hipGetLastError();
     }
 
+    static hipError_t average(const Layout& L, const void* cur, int si, void* avg, hipStream_t s) {
+        const dim3 grid((L.n(L.ax0) + 127) / 128, L.n(L.ax1), L.n(L.ax2));
+        hipLaunchKernelGGL((average_kernel<Q, C>), grid, dim3(128), 0, s, L, cur, si, static_cast<R*>(avg));
+        return hipGetLastError();
+    }
+
     // ---- ensembles
     static void ens_pack(const MemberHost& m, int block, void* out, int* tiles, int* inflow_tiles) {
         const Layout& L = m.lay;
@@ -327,7 +334,7 @@ struct Launch {
     }
 
     static const Ops* ops() {
-        static const Ops o = {&step, &closed, &snapshot, &edges, &inflow, &aos_in, &state_out, &feq_init, &totals,
+        static const Ops o = {&step, &closed, &snapshot, &edges, &inflow, &aos_in, &state_out, &feq_init, &totals, &average,
                               sizeof(MemberTab<R>), &ens_pack, &ens_step,
                               kAdj ? &adj_step : nullptr, kAdj ? &adj_inflow : nullptr, kAdj ? &adj_face : nullptr,
                               &objective, (int)sizeof(R), (int)sizeof(typename C::raw)};

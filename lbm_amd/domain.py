@@ -32,7 +32,7 @@ EXPORTS = ["lbm_create", "lbm_destroy", "lbm_last_error", "lbm_set_state", "lbm_
            "lbm_ensemble_synchronize", "lbm_ensemble_set_launch", "lbm_ensemble_get_launch",
            "lbm_ensemble_tune", "lbm_ensemble_set_timing", "lbm_ensemble_timing_read",
            "lbm_objective_run", "lbm_adjoint_run", "lbm_set_tangent", "lbm_objective_tangent",
-           "lbm_generate_turbulence"]
+           "lbm_generate_turbulence", "lbm_set_averaging", "lbm_get_averages"]
 
 
 class Desc(C.Structure):
@@ -94,6 +94,8 @@ def load_library(path: str = LIB_PATH):
         "lbm_objective_run": ([vp, i32, i32, C.POINTER(C.c_int32), C.POINTER(dbl)], i32),
         "lbm_adjoint_run": ([vp, i32, i32, i32, C.POINTER(C.c_int32), C.POINTER(dbl), vp, vp], i32),
         "lbm_set_tangent": ([vp, vp, vp], i32),
+        "lbm_set_averaging": ([vp, i32], i32),
+        "lbm_get_averages": ([vp, vp, vp, vp, vp, C.POINTER(i32), i32], i32),
         "lbm_generate_turbulence": ([vp, i32, dbl, C.c_uint64, vp, vp, vp, vp, i32], i32),
         "lbm_objective_tangent": ([vp, C.POINTER(dbl)], i32),
     }
@@ -190,6 +192,19 @@ class Domain:
         t = np.empty(self.shape_cell, self.real)
         self._chk(self.lib.lbm_get_tau(self.h, _p(t), t.itemsize), "lbm_get_tau")
         return t
+
+    def set_averaging(self, on=True):
+        """Time averaging of the moments after each following step (lbm_set_averaging)."""
+        self._chk(self.lib.lbm_set_averaging(self.h, int(on)), "lbm_set_averaging")
+
+    def averages(self, dtype=None):
+        """(uave, vave, wave, Ti, count) with cell-field shape (nz+2, ny+2, nx+2) (lbm_get_averages)."""
+        dt = dtype or self.real
+        out = [np.empty(self.shape_cell, dt) for _ in range(4)]
+        n = C.c_int()
+        self._chk(self.lib.lbm_get_averages(self.h, *[_p(a) for a in out], C.byref(n), np.dtype(dt).itemsize),
+                  "lbm_get_averages")
+        return (*out, n.value)
 
     def totals(self):
         m = C.c_double()
