@@ -55,9 +55,11 @@ int lbm_objective_run(lbm_ctx* c, int it0, int nsteps, const int32_t* box, doubl
     AdjBox b;
     if (int rc = check_box(c, box, &b)) return rc;
     DevBuf acc;
-    if (int rc = dev_alloc(c, acc, 2 * sizeof(double))) return rc;
+    const int nblk = objective_blocks(b);
+    const size_t accb = 2 * (size_t)nblk * sizeof(double);
+    if (int rc = dev_alloc(c, acc, accb)) return rc;
     int rc = LBM_OK;
-    if (hipMemsetAsync(acc.p, 0, 2 * sizeof(double), c->stream) != hipSuccess) rc = fail(c, LBM_EHIP, "hipMemsetAsync");
+    if (hipMemsetAsync(acc.p, 0, accb, c->stream) != hipSuccess) rc = fail(c, LBM_EHIP, "hipMemsetAsync");
     for (int n = 1; n <= nsteps && rc == LBM_OK; ++n) {
         if ((rc = lbm_step(c, it0 + n - 1, 1))) break;
         const hipError_t e = c->ops->objective(c->lay, c->buf[c->cur], b, static_cast<double*>(acc.p), c->stream);
@@ -65,9 +67,14 @@ int lbm_objective_run(lbm_ctx* c, int it0, int nsteps, const int32_t* box, doubl
     }
     double s[2] = {0.0, 0.0};
     if (rc == LBM_OK) {
-        if (hipMemcpyAsync(s, acc.p, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        std::vector<double> h(2 * (size_t)nblk);
+        if (hipMemcpyAsync(h.data(), acc.p, accb, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
             hipStreamSynchronize(c->stream) != hipSuccess)
             rc = fail(c, LBM_EHIP, "objective read-back");
+        for (int q = 0; q < nblk; ++q) {
+            s[0] += h[2 * q];
+            s[1] += h[2 * q + 1];
+        }
     }
     dev_free(acc);
     if (rc == LBM_OK) {
@@ -142,7 +149,8 @@ int lbm_adjoint_run(lbm_ctx* c, int it0, int nsteps, int every, const int32_t* b
     int rc = LBM_OK;
     if ((rc = dev_alloc(c, ck, (size_t)nck * 2 * bb)) || (rc = dev_alloc(c, snaps, (size_t)every * bb)) ||
         (rc = dev_alloc(c, lam, 2 * bb)) || (rc = dev_alloc(c, fin, 2 * bb)) ||
-        (rc = dev_alloc(c, acc, sizeof(double))) || (rc = dev_alloc(c, gS, (size_t)d.ny * d.nz * sizeof(double))) ||
+        (rc = dev_alloc(c, acc, 2 * (size_t)objective_blocks(b) * sizeof(double))) ||
+        (rc = dev_alloc(c, gS, (size_t)d.ny * d.nz * sizeof(double))) ||
         (grad_force && c->has_force && (rc = dev_alloc(c, gF, 3 * plane * sizeof(double))))) {
         cleanup();
         return rc;
@@ -152,7 +160,8 @@ int lbm_adjoint_run(lbm_ctx* c, int it0, int nsteps, int every, const int32_t* b
     char* lmp = static_cast<char*>(lam.p);
     std::vector<int> ck_cur(nck, 0);
     auto run = [&]() -> int {
-        HIPCHK(c, hipMemsetAsync(acc.p, 0, sizeof(double), c->stream));
+        const int nblk = objective_blocks(b);
+        HIPCHK(c, hipMemsetAsync(acc.p, 0, 2 * (size_t)nblk * sizeof(double), c->stream));
         // ---- forward sweep with checkpoints and the objective
         for (int n = 1; n <= nsteps; ++n) {
             if ((n - 1) % every == 0) {
@@ -223,14 +232,15 @@ int lbm_adjoint_run(lbm_ctx* c, int it0, int nsteps, int every, const int32_t* b
         c->ghost_raw[1] = fin_raw1;
         // results
         double s = 0.0;
-        std::vector<double> hs((size_t)d.ny * d.nz), hf;
-        HIPCHK(c, hipMemcpyAsync(&s, acc.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        std::vector<double> hs((size_t)d.ny * d.nz), hf, ha(2 * (size_t)nblk);
+        HIPCHK(c, hipMemcpyAsync(ha.data(), acc.p, ha.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipMemcpyAsync(hs.data(), gS.p, hs.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         if (gF.p) {
             hf.resize(3 * plane);
             HIPCHK(c, hipMemcpyAsync(hf.data(), gF.p, hf.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         }
         HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (int q = 0; q < nblk; ++q) s += ha[2 * q];
         *J = s * kappa;
         if (grad_shear)
             for (int k = 0; k < d.nz; ++k) {

@@ -406,8 +406,9 @@ __global__ void __launch_bounds__(64) adjoint_face_kernel(Layout L, typename C::
 }
 
 // Objective: sum over the box of 0.5*rho*|u|^2 of the streamed populations of buffer f (the
-// reference's macrovars after a step), added to acc[0] (FP64); with the tangent type its
-// directional derivative is added to acc[1].
+// reference's macrovars after a step), added per block b to acc[2b] (FP64); with the tangent
+// type its directional derivative is added to acc[2b+1].  The grid size is fixed per box
+// (objective_blocks), so the host's ordered sum over the slots is reproducible.
 template <int Q, class C>
 __global__ void __launch_bounds__(256) objective_kernel(Layout L, const void* f, AdjBox box, double* acc) {
     using R = typename C::real;
@@ -458,9 +459,10 @@ __global__ void __launch_bounds__(256) objective_kernel(Layout L, const void* f,
         }
         __syncthreads();
     }
+    // every block owns its pair of slots: the sum is the same in every run
     if (threadIdx.x == 0) {
-        atomicAdd(acc, red[0][0]);
-        if (TAN) atomicAdd(acc + 1, red[1][0]);
+        acc[2 * blockIdx.x] += red[0][0];
+        acc[2 * blockIdx.x + 1] += red[1][0];
     }
 }
 

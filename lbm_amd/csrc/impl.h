@@ -83,6 +83,12 @@ struct EnsLaunch {
 struct AdjBox {
     int lo[3], hi[3];
 };
+// Blocks of the objective kernel for a box; each owns two FP64 accumulator slots.
+inline int objective_blocks(const AdjBox& b) {
+    const long long nb = (long long)(b.hi[0] - b.lo[0] + 1) * (b.hi[1] - b.lo[1] + 1) * (b.hi[2] - b.lo[2] + 1);
+    const long long g = (nb + 255) / 256;
+    return (int)(g < 1024 ? g : 1024);
+}
 
 struct Ops {
     hipError_t (*step)(const Layout&, const StepHost&, hipStream_t);

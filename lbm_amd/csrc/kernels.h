@@ -371,11 +371,16 @@ __device__ __forceinline__ void step_cell(const StepArgs<typename C::real>& a, i
         oy[d] = Layout::wrap(j - s * c, L.ny, L.per_j) * L.sj;
         oz[d] = Layout::wrap(k - s * c, L.nz, L.per_k) * L.sk;
     }
-    const raw* __restrict__ src = static_cast<const raw*>(a.src);
+    // Plane bases are uniform (scalar registers); the per-lane part is a 32-bit byte offset
+    // within a plane (planes are below 4 GiB), so loads and stores take the scalar-base form
+    // without 64-bit address arithmetic per population.
+    const char* __restrict__ srcb = static_cast<const char*>(a.src);
+    const size_t pbytes = (size_t)L.plane * sizeof(raw);
     raw rv[Q];
 #pragma unroll
     for (int l = 0; l < Q; ++l) {
-        const raw* p = src + (size_t)l * (size_t)L.plane + (size_t)(oz[Lt::cz(l) + 1] + oy[Lt::cy(l) + 1] + ox[Lt::cx(l) + 1]);
+        const uint32_t off = (uint32_t)(oz[Lt::cz(l) + 1] + oy[Lt::cy(l) + 1] + ox[Lt::cx(l) + 1]) * (uint32_t)sizeof(raw);
+        const raw* p = reinterpret_cast<const raw*>(srcb + (size_t)l * pbytes + off);
         if constexpr ((VAR & 2) != 0) rv[l] = __builtin_nontemporal_load(p);
         else rv[l] = *p;
     }
@@ -385,10 +390,11 @@ __device__ __forceinline__ void step_cell(const StepArgs<typename C::real>& a, i
     const int oc = L.cell(i, j, k);
     collide_cell<Q, C, BGK3, HRR, VRE>(a, rv, i, j, k, ca, cb, cc, oc, out, tau);
 
-    raw* __restrict__ dst = static_cast<raw*>(a.dst);
+    char* __restrict__ dstb = static_cast<char*>(a.dst);
+    const uint32_t ob = (uint32_t)oc * (uint32_t)sizeof(raw);
 #pragma unroll
     for (int l = 0; l < Q; ++l) {
-        raw* p = dst + (size_t)l * (size_t)L.plane + (size_t)oc;
+        raw* p = reinterpret_cast<raw*>(dstb + (size_t)l * pbytes + ob);
         if constexpr ((VAR & 1) != 0) __builtin_nontemporal_store(C::template enc_dep<Q>(out[l], l), p);
         else *p = C::template enc_dep<Q>(out[l], l);
     }

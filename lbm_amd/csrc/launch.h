@@ -327,9 +327,7 @@ struct Launch {
     }
 
     static hipError_t objective(const Layout& L, const void* f, const AdjBox& box, double* acc, hipStream_t s) {
-        const long long nb = (long long)(box.hi[0] - box.lo[0] + 1) * (box.hi[1] - box.lo[1] + 1) * (box.hi[2] - box.lo[2] + 1);
-        const int blocks = (int)std::min<long long>((nb + 255) / 256, 1024);
-        hipLaunchKernelGGL((objective_kernel<Q, C>), dim3(blocks), dim3(256), 0, s, L, f, box, acc);
+        hipLaunchKernelGGL((objective_kernel<Q, C>), dim3(objective_blocks(box)), dim3(256), 0, s, L, f, box, acc);
         return hipGetLastError();
     }
 
