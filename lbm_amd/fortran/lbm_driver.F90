@@ -8,9 +8,10 @@
 ! mod_dimensions); for nt0 = 0 give it on the command line:  lbm_driver nx ny nz [nl].
 ! Inflow-turbulence fields come from the turbulence restart file when it exists, otherwise (and
 ! every nrturb steps, as src/main.F90:381-383) from the device generator lbm_generate_turbulence.
-! Not carried over (outside the lattice update): turbines, buoyancy/ABL, tracers, Tecplot and
-! NetCDF output, averaging, the random perturbation of inipert (nt0 = 0 starts from the inflow
-! profile).  Geometry: 'none' or the reference's 'cylinder' (src/m_cylinder.F90).
+! Time averages (laveraging, avestart, avesave) are accumulated on the device and written to
+! averages/averages_0000_<avesave>.uf.  Not carried over (outside the lattice update): turbines,
+! buoyancy/ABL, tracers, Tecplot and NetCDF output, the random perturbation of inipert (nt0 = 0
+! starts from the inflow profile).  Geometry: 'none' or the reference's 'cylinder' (src/m_cylinder.F90).
 program lbm_driver
    use, intrinsic :: iso_c_binding
    use lbm_hip
@@ -217,10 +218,18 @@ program lbm_driver
       nchunk = min(10 - mod(it, 10), nt1 - it)
       if (ldump .and. irestart > 0) nchunk = min(nchunk, irestart - mod(it, irestart))
       if (inflowturbulence .and. ibnd == 1) nchunk = min(nchunk, nrturb - mod(it, nrturb))
+      if (laveraging .and. it < avestart) nchunk = min(nchunk, avestart - it)
+      if (laveraging .and. it < avesave) nchunk = min(nchunk, avesave - it)
       rc = lbm_step(ctx, int(it, c_int), int(nchunk, c_int))
       call lbm_check(ctx, rc, 'lbm_step')
       it = it + nchunk
       if (mod(it, 10) == 0 .or. it == nt1) print '(a,i6,a,f10.2,a)', 'Iteration: ', it, ' Time: ', real(it)*p_time, ' s.'
+      ! time averaging over steps avestart+1 .. avesave (src/main.F90:374-377)
+      if (laveraging .and. it == avestart) then
+         rc = lbm_set_averaging(ctx, 1_c_int)
+         call lbm_check(ctx, rc, 'lbm_set_averaging')
+      endif
+      if (laveraging .and. it == avesave) call save_averages(it)
       ! new turbulence fields every nrturb steps (src/main.F90:381-383)
       if (inflowturbulence .and. ibnd == 1 .and. mod(it, nrturb) == 0 .and. it > 1 .and. it < nt1) &
          call generate_turbulence(it)
@@ -243,6 +252,26 @@ program lbm_driver
    rc = lbm_destroy(ctx)
 
 contains
+
+   subroutine save_averages(itw)
+      ! uave, vave, wave and Ti of averaging_full (src/m_averaging_full.F90:82-95), written as an
+      ! unformatted record averages/averages_0000_<it>.uf instead of the Tecplot diag file
+      integer, intent(in) :: itw
+      integer :: iu, navg
+      character(len=100) :: fn
+      real(rk), allocatable, target :: ua(:,:,:), va(:,:,:), wa(:,:,:), ti(:,:,:)
+      allocate(ua(0:nx+1,0:ny+1,0:nz+1), va(0:nx+1,0:ny+1,0:nz+1), wa(0:nx+1,0:ny+1,0:nz+1), ti(0:nx+1,0:ny+1,0:nz+1))
+      rc = lbm_get_averages(ctx, c_loc(ua), c_loc(va), c_loc(wa), c_loc(ti), navg, realkind)
+      call lbm_check(ctx, rc, 'lbm_get_averages')
+      rc = lbm_set_averaging(ctx, 0_c_int)
+      call lbm_check(ctx, rc, 'lbm_set_averaging')
+      call execute_command_line('mkdir -p averages')
+      write(fn,'(a,i6.6,a)') 'averages/averages_0000_', itw, '.uf'
+      open(newunit=iu, file=trim(fn), form='unformatted', status='replace')
+      write(iu) nx, ny, nz, navg, ua, va, wa, ti
+      close(iu)
+      print '(a,i6,a,a)', 'Averages of ', navg, ' steps written to ', trim(fn)
+   end subroutine
 
    subroutine generate_turbulence(itw)
       ! the fields are kept on the host too, for the turbulence restart record

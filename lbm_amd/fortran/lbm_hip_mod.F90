@@ -25,7 +25,8 @@ module lbm_hip
    public :: lbm_create, lbm_destroy, lbm_last_error, lbm_set_state, lbm_get_state, lbm_get_macros, &
              lbm_init_equilibrium, lbm_set_blanking, lbm_set_ablvisc, lbm_set_inflow, &
              lbm_set_inflow_series, lbm_set_turbulence, lbm_set_external_forcing, lbm_step, &
-             lbm_synchronize, lbm_get_totals, lbm_check, lbm_error_message, lbm_generate_turbulence
+             lbm_synchronize, lbm_get_totals, lbm_check, lbm_error_message, lbm_generate_turbulence, &
+             lbm_set_averaging, lbm_get_averages
 
    interface
       integer(c_int) function lbm_create(desc, ctx) bind(C, name='lbm_create')
@@ -111,6 +112,17 @@ module lbm_hip
          import :: c_int, c_ptr
          type(c_ptr), value :: ctx
          integer(c_int), value :: it0, nsteps
+      end function
+      integer(c_int) function lbm_set_averaging(ctx, on) bind(C, name='lbm_set_averaging')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: on
+      end function
+      integer(c_int) function lbm_get_averages(ctx, uave, vave, wave, ti, count, realkind) bind(C, name='lbm_get_averages')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx, uave, vave, wave, ti
+         integer(c_int), intent(out) :: count
+         integer(c_int), value :: realkind
       end function
       integer(c_int) function lbm_synchronize(ctx) bind(C, name='lbm_synchronize')
          import :: c_int, c_ptr

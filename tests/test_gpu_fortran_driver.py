@@ -52,3 +52,29 @@ def test_fortran_driver_matches_reference_build(name, tmp_path):
     assert np.isfinite(f).all() and d.max() <= tol
     # the testing file (src/m_testing.F90) is written on the first run and compared on a second
     assert (tmp_path / "testing" / fio.restart_name(nt1, prefix="testing")).exists()
+
+
+def test_fortran_driver_cold_start_generated_turbulence_and_averages(tmp_path):
+    """nt0 = 0: equilibrium start from the inflow profile, turbulence fields generated on the
+    device at start-up and every nrturb steps, time averages over avestart < it <= avesave."""
+    exe = os.path.join(BUILD, "lbm_driver_f32")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran driver not built (no amdflang)")
+    nx, ny, nz, nrturb = 40, 16, 12, 8
+    txt = oracle.infile_text(nt0=0, nt1=20, turb=True, ampl=1e-3, nrturb=nrturb, ibgk=3, ihrr=1, ivreman=1,
+                             smagorinsky=0.16733, ibnd=1, jbnd=0, kbnd=12, uini=0.05, visc=1e-3)
+    assert " F F\n 20000\n 40000\n" in txt
+    (tmp_path / "infile.in").write_text(txt.replace(" F F\n 20000\n 40000\n", " T F\n 5\n 15\n"))
+    print(f"driver cold start {nx}x{ny}x{nz}, nrturb={nrturb}", flush=True)
+    p = subprocess.run([exe, str(nx), str(ny), str(nz), "27"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=120)
+    print(p.stdout[-2000:], p.stderr[-1000:], flush=True)
+    assert p.returncode == 0
+    f = fio.read_restart(str(tmp_path / "restart" / fio.restart_name(20)))
+    assert f.shape == (nz + 2, ny + 2, nx + 2, 27) and np.isfinite(f).all()
+    uu, vv, ww, rr = fio.read_turbulence(str(tmp_path / "restart" / fio.restart_name(20, prefix="turbulence")))
+    assert uu.shape == (nrturb + 1, nz, ny)
+    assert np.abs(uu[1:].mean(axis=(1, 2))).max() < 1e-5
+    assert np.abs(uu[1:].std(axis=(1, 2), ddof=1) - 1).max() < 1e-4
+    assert "Averages of     10 steps" in p.stdout
+    assert (tmp_path / "averages" / "averages_0000_000015.uf").exists()
