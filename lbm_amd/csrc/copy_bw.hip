@@ -1,7 +1,10 @@
 // Plain device-to-device copy of a known size: the calibration kernel for HBM byte counters
 // (FETCH_SIZE / WRITE_SIZE) and the achievable-bandwidth reference next to the lattice kernels.
 //   copy_bw [MiB] [iterations] [blocks] [mode]   prints one JSON line (mode 0 copy, 1 non-temporal
-//   copy, 2 read only, 3 write only)
+//   copy, 2 read only, 3 write only, 4 one float4 per thread, 5 27 planes per thread as the
+//   lattice kernels move them, 6/7/8 the same with 1/2/4 cells per thread and the third argument
+//   as dynamic LDS bytes per block to cap occupancy, 9/10 the row-blocked layout with rows of
+//   256/64 cells; modes 4-10 size their own grid)
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -54,6 +57,55 @@ __global__ void __launch_bounds__(256) write_kernel(const float4* __restrict__ s
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) dst[t] = make_float4(1.f, 2.f, 3.f, 4.f);
 }
 
+// One float4 per thread, no loop (the one-shot grid shape of the lattice kernels).
+__global__ void __launch_bounds__(256) copy_once_kernel(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) dst[t] = src[t];
+}
+
+// The lattice kernels' stream shape: 27 planes of 4-byte values read and 27 written per thread,
+// one cell per thread (loads issued before stores), non-temporal stores.
+__global__ void __launch_bounds__(256) planes_kernel(const float* __restrict__ src, float* __restrict__ dst, size_t ncell) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    float v[27];
+#pragma unroll
+    for (int l = 0; l < 27; ++l) v[l] = src[(size_t)l * ncell + c];
+#pragma unroll
+    for (int l = 0; l < 27; ++l) __builtin_nontemporal_store(v[l], dst + (size_t)l * ncell + c);
+}
+
+// The same with W consecutive cells per thread (W-wide vector per plane), and dynamic LDS to
+// cap the number of resident waves.
+template <int W>
+__global__ void __launch_bounds__(256) planes_vec_kernel(const float* __restrict__ src, float* __restrict__ dst, size_t ncell) {
+    typedef float vec __attribute__((ext_vector_type(W)));
+    extern __shared__ float pad[];
+    const size_t c = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * W;
+    if (c + W > ncell) return;
+    vec v[27];
+#pragma unroll
+    for (int l = 0; l < 27; ++l) v[l] = *reinterpret_cast<const vec*>(src + (size_t)l * ncell + c);
+#pragma unroll
+    for (int l = 0; l < 27; ++l) __builtin_nontemporal_store(v[l], reinterpret_cast<vec*>(dst + (size_t)l * ncell + c));
+    if (c == (size_t)-1) pad[0] = 0.f;
+}
+
+// Row-blocked layout [row][population][R cells]: the 27 values of a thread's cell lie R*4 bytes
+// apart inside one 27*R*4-byte block.
+template <int R>
+__global__ void __launch_bounds__(256) rows_kernel(const float* __restrict__ src, float* __restrict__ dst, size_t ncell) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    const size_t row = c / R, w = c % R;
+    const size_t base = row * 27 * R + w;
+    float v[27];
+#pragma unroll
+    for (int l = 0; l < 27; ++l) v[l] = src[base + (size_t)l * R];
+#pragma unroll
+    for (int l = 0; l < 27; ++l) __builtin_nontemporal_store(v[l], dst + base + (size_t)l * R);
+}
+
 #define CHK(x)                                                                      \
     do {                                                                            \
         hipError_t e = (x);                                                         \
@@ -75,26 +127,58 @@ int main(int argc, char** argv) {
     CHK(hipMemset(b, 0, bytes));
     hipDeviceProp_t prop;
     CHK(hipGetDeviceProperties(&prop, 0));
-    const int blocks = argc > 3 ? atoi(argv[3]) : prop.multiProcessorCount * 16;
-    const int mode = argc > 4 ? atoi(argv[4]) : 0;   // 0 copy, 1 non-temporal copy, 2 read, 3 write
-    void (*kern)(const float4*, float4*, size_t) = mode == 1 ? copy_nt_kernel : mode == 2 ? read_kernel : mode == 3 ? write_kernel : copy_kernel;
-    const char* names[4] = {"copy_kernel", "copy_nt_kernel", "read_kernel", "write_kernel"};
+    const int mode = argc > 4 ? atoi(argv[4]) : 0;
+    // modes 4 and 5 cover the data with one grid of one-shot threads
+    const size_t ncell = bytes / (27 * sizeof(float));
+    // modes 6, 7, 8: planes with 1, 2, 4 cells per thread; argument 3 is then the dynamic LDS
+    // bytes per block (occupancy cap)
+    const int W = mode == 7 ? 2 : mode == 8 ? 4 : 1;
+    const size_t nrow = ncell / 256 * 256;   // rows_kernel: whole rows only
+    const int lds = mode >= 6 && argc > 3 ? atoi(argv[3]) : 0;
+    const int blocks = mode == 4 ? (int)((n + 255) / 256)
+                       : mode == 5 ? (int)((ncell + 255) / 256)
+                       : mode >= 9 ? (int)(nrow / 256)
+                       : mode >= 6 ? (int)((ncell / W + 255) / 256)
+                                   : (argc > 3 ? atoi(argv[3]) : prop.multiProcessorCount * 16);
+    const char* names[11] = {"copy_kernel", "copy_nt_kernel", "read_kernel", "write_kernel", "copy_once_kernel",
+                             "planes_kernel", "planes_vec1", "planes_vec2", "planes_vec4", "rows_256", "rows_64"};
     const double rd = (mode == 3) ? 0.0 : 1.0, wr = (mode == 2) ? 0.0 : 1.0;
+    const double moved = mode >= 9 ? (double)nrow * 27 * sizeof(float)
+                         : mode >= 5 ? (double)(ncell / W * W) * 27 * sizeof(float) : (double)bytes;
+    auto launch = [&]() {
+        const float* fa = reinterpret_cast<const float*>(a);
+        float* fb = reinterpret_cast<float*>(b);
+        if (mode == 9) hipLaunchKernelGGL(rows_kernel<256>, dim3(blocks), dim3(256), 0, 0, fa, fb, nrow);
+        else if (mode == 10) hipLaunchKernelGGL(rows_kernel<64>, dim3(blocks), dim3(256), 0, 0, fa, fb, nrow);
+        else if (mode == 6) hipLaunchKernelGGL(planes_vec_kernel<1>, dim3(blocks), dim3(256), lds, 0, fa, fb, ncell);
+        else if (mode == 7) hipLaunchKernelGGL(planes_vec_kernel<2>, dim3(blocks), dim3(256), lds, 0, fa, fb, ncell);
+        else if (mode == 8) hipLaunchKernelGGL(planes_vec_kernel<4>, dim3(blocks), dim3(256), lds, 0, fa, fb, ncell);
+        else if (mode == 5)
+            hipLaunchKernelGGL(planes_kernel, dim3(blocks), dim3(256), 0, 0, reinterpret_cast<const float*>(a),
+                               reinterpret_cast<float*>(b), ncell);
+        else if (mode == 4)
+            hipLaunchKernelGGL(copy_once_kernel, dim3(blocks), dim3(256), 0, 0, a, b, n);
+        else {
+            void (*kern)(const float4*, float4*, size_t) =
+                mode == 1 ? copy_nt_kernel : mode == 2 ? read_kernel : mode == 3 ? write_kernel : copy_kernel;
+            hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, 0, a, b, n);
+        }
+    };
     hipEvent_t e0, e1;
     CHK(hipEventCreate(&e0));
     CHK(hipEventCreate(&e1));
-    hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, 0, a, b, n);
+    launch();
     CHK(hipDeviceSynchronize());
     CHK(hipEventRecord(e0, 0));
-    for (int it = 0; it < iters; ++it) hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, 0, a, b, n);
+    for (int it = 0; it < iters; ++it) launch();
     CHK(hipEventRecord(e1, 0));
     CHK(hipEventSynchronize(e1));
     float ms = 0;
     CHK(hipEventElapsedTime(&ms, e0, e1));
     const double per = ms / iters;
     printf("{\"kernel\": \"%s\", \"bytes_read\": %.0f, \"bytes_written\": %.0f, \"ms_per_launch\": %.5f, "
-           "\"GBps\": %.1f, \"blocks\": %d, \"cus\": %d}\n", names[mode], rd * bytes, wr * bytes, per,
-           (rd + wr) * bytes / (per * 1e-3) / 1e9, blocks, prop.multiProcessorCount);
+           "\"GBps\": %.1f, \"blocks\": %d, \"cus\": %d}\n", names[mode], rd * moved, wr * moved, per,
+           (rd + wr) * moved / (per * 1e-3) / 1e9, blocks, prop.multiProcessorCount);
     CHK(hipFree(a));
     CHK(hipFree(b));
     return 0;
