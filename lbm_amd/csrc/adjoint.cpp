@@ -183,7 +183,7 @@ int lbm_adjoint_run(lbm_ctx* c, int it0, int nsteps, int every, const int32_t* b
         HIPCHK(c, hipMemsetAsync(gS.p, 0, (size_t)d.ny * d.nz * sizeof(double), c->stream));
         if (gF.p) HIPCHK(c, hipMemsetAsync(gF.p, 0, 3 * plane * sizeof(double), c->stream));
         const double kappa = 1.0 / ((double)nsteps * box_cells(b));
-        StepHost h = step_host(c);
+        StepHost h = step_host(c, false);
         h.stream = 1;
         h.tau_out = nullptr;
         for (int q = 0; q < 4; ++q) h.ext[q] = nullptr;

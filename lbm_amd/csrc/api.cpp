@@ -234,7 +234,7 @@ void inflow_values(const lbm_ctx* c, int it, double* ut, double* cs, double* sn)
     }
 }
 
-StepHost step_host(const lbm_ctx* c) {
+StepHost step_host(const lbm_ctx* c, bool fold) {
     const lbm_desc& d = c->d;
     StepHost h{};
     h.fac0 = c->fac0;
@@ -253,9 +253,9 @@ StepHost step_host(const lbm_ctx* c) {
     h.block_x = d.block_x;
     h.block_y = d.block_y;
     h.variant = c->variant;
-    h.fold_dir = c->fold_dir;
+    h.fold_dir = fold ? c->fold_dir : -1;
     h.fold_opt_lo = h.fold_opt_hi = 0;
-    if (c->fold_dir >= 0) {
+    if (h.fold_dir >= 0) {
         const int code = c->fold_dir == 0 ? d.ibnd : c->fold_dir == 1 ? d.jbnd : d.kbnd;
         opts_of(code, &h.fold_opt_lo, &h.fold_opt_hi);
     }
@@ -353,14 +353,19 @@ int lbm_create(const lbm_desc* desc, lbm_ctx** out) {
     if (const char* v = getenv("LBM_STEP_VARIANT")) c->variant = atoi(v) & 5;
     if ((c->variant & 4) && (lay.n(lay.ax0) % 2 != 0 || ops->raw_bytes > 4)) c->variant &= ~4;
     {
-        // The step kernel performs the face pass of the closed axis when it is the only closed
-        // one (no edges, corners or snapshots then; the inflow planes of i are written after).
+        // With one closed axis (no edges, corners or snapshots then; the inflow planes of i are
+        // written after) the step kernel can perform that axis's face pass itself.  Ensembles
+        // always do (a member's face must not need a launch of its own).  A single domain uses
+        // the separate face kernel unless LBM_FOLD=1: measured on MI355X, the folded pass makes
+        // the step kernel's last (wall-plane) blocks a tail, 10-15 us per step against the 7 us
+        // launch it saves.
         const int codes[3] = {d.ibnd, d.jbnd, d.kbnd};
         int nclosed = 0, cl = -1;
         for (int a = 0; a < 3; ++a)
             if (codes[a] > 10) { ++nclosed; cl = a; }
-        const char* nf = getenv("LBM_NO_FOLD");
-        if (nclosed == 1 && !(nf && atoi(nf))) c->fold_dir = cl;
+        if (nclosed == 1) c->fold_dir = cl;
+        const char* fv = getenv("LBM_FOLD");
+        c->fold_on = c->fold_dir >= 0 && fv && atoi(fv);
     }
     hipGetDevice(&c->device);
     c->lay = lay;
@@ -469,7 +474,7 @@ int lbm_step(lbm_ctx* c, int it0, int nsteps) {
     if (!c->have_state) return fail(c, LBM_ESTATE, "no state: call lbm_set_state or lbm_init_equilibrium first");
     if (nsteps < 0) return fail(c, LBM_EINVAL, "nsteps < 0");
     const lbm_desc& d = c->d;
-    StepHost h = step_host(c);
+    StepHost h = step_host(c, c->fold_on);
     const size_t slice = (size_t)d.ny * d.nz * c->ops->real_bytes;
     for (int it = it0 + 1; it <= it0 + nsteps; ++it) {
         if (c->pending) {
@@ -503,7 +508,7 @@ int lbm_step(lbm_ctx* c, int it0, int nsteps) {
         if (!c->pending) c->cur ^= 1;
         c->pending = false;
         c->use_ext = false;
-        if (int rc = boundary(c, c->buf[c->cur], c->buf[c->cur ^ 1], it, c->fold_dir)) return rc;
+        if (int rc = boundary(c, c->buf[c->cur], c->buf[c->cur ^ 1], it, c->fold_on ? c->fold_dir : -1)) return rc;
         c->ghost_raw[c->cur] = false;
         c->avg_pending = c->avg_on;
     }

@@ -51,7 +51,9 @@ struct lbm_ctx {
     bool avg_on = false;
     bool avg_pending = false;      // the current state's moments are not in the sums yet
     int avg_count = 0;
-    int fold_dir = -1;             // closed axis whose face pass the step kernel performs
+    int fold_dir = -1;             // the only closed axis (-1: none or several): its face pass can
+                                   // run inside the step kernel (always in ensembles)
+    bool fold_on = false;          // single-domain steps fold it (LBM_FOLD=1)
     int variant = 1;               // step-kernel variant (LBM_STEP_VARIANT), see impl.h; non-temporal
                                    // stores measured fastest on MI355X (5.32 vs 5.11 TB/s)
     // timing
@@ -73,8 +75,9 @@ double utime_at(const lbm_ctx* c, int it);
 double udir_at(const lbm_ctx* c, int it);
 // (utime, cos(udir), sin(udir)) of step it in the compute type's rounding
 void inflow_values(const lbm_ctx* c, int it, double* ut, double* cs, double* sn);
-// the step-kernel parameters of c that do not change from step to step
-lbm::StepHost step_host(const lbm_ctx* c);
+// the step-kernel parameters of c that do not change from step to step; fold: the step kernel
+// performs the face pass of c->fold_dir
+lbm::StepHost step_host(const lbm_ctx* c, bool fold);
 // boundarycond(f1, f2) for the non-periodic directions; skip_dir: face done by the step kernel
 int boundary(lbm_ctx* c, void* f1, const void* f2, int it, int skip_dir = -1);
 

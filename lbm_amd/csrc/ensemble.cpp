@@ -101,7 +101,7 @@ int build_tables(lbm_ens* e) {
         if (c->avg_on) return efail(e, LBM_ESTATE, "member " + std::to_string(m) + ": time averaging is not batched; switch it off");
         MemberHost mh{};
         mh.lay = c->lay;
-        mh.h = step_host(c);
+        mh.h = step_host(c, true);
         mh.buf[0] = c->buf[0];
         mh.buf[1] = c->buf[1];
         mh.cur0 = c->cur;

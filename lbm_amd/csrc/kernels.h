@@ -68,6 +68,17 @@ __host__ __device__ __forceinline__ R sdot(int sx, int sy, int sz, R a, R b, R c
     return s;
 }
 
+// Reciprocal: the hardware's (1 ulp) in FP32 device code, the IEEE quotient otherwise.  Used for
+// 1/rho, 1/tau and the Vreman ratio, where the FP32 quotient's correction sequence costs about
+// ten instructions per use in an issue-bound kernel.
+template <typename R>
+__host__ __device__ __forceinline__ R recip(R x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (std::is_same<R, float>::value) return __builtin_amdgcn_rcpf(x);
+#endif
+    return R(1) / x;
+}
+
 // Hermite equilibrium of direction l (src/m_postcoll_kernel.F90:136-183 in closed form):
 //   feq = w rho (1 - 1.5 u.u + cu (3 + 4.5 cu [+ 4.5 (cu^2 - u.u)]))   with cu = c.u
 // The bracket is the reference's full H3:A3 contraction, 4.5 rho cu (cu^2 - u.u), applied for
@@ -142,7 +153,7 @@ __host__ __device__ __forceinline__ void collide_cell(const StepArgs<typename C:
     }
     dr *= invS;
     R rh = R(1) + dr;
-    const R invrho = R(1) / rh;
+    const R invrho = recip(rh);
     R ux = (jx * invS) * invrho, uy = (jy * invS) * invrho, uz = (jz * invS) * invrho;
     if (a.avg != nullptr) {
         // time averaging of the moments after the previous step (src/m_averaging_full_kernel.F90:33-39)
@@ -191,11 +202,11 @@ __host__ __device__ __forceinline__ void collide_cell(const StepArgs<typename C:
         const R b13 = a.eps + (axx * axz + axy * ayz + axz * azz);
         const R b23 = a.eps + (axy * axz + ayy * ayz + ayz * azz);
         const R bb = b11 * b22 - b12 * b12 + b11 * b33 - b13 * b13 + b22 * b33 - b23 * b23;
-        const R t = bb / amag;
+        const R t = bb * recip(amag);
         using std::sqrt;   // the tangent type brings its own sqrt
         const R eddy = t > a.eps ? a.vconst * sqrt(t) : R(0);
         tau = R(3) * (a.kinevisc + eddy + a.ablvisc[k - 1]) + R(0.5);
-        fac = R(1) - R(1) / tau;
+        fac = R(1) - recip(tau);
     }
 
     const R T = axx + ayy + azz;
@@ -268,9 +279,10 @@ __host__ __device__ __forceinline__ void collide_cell(const StepArgs<typename C:
         add_feq_diff(ux + a.ampl * a.tu[sl], uy + a.ampl * a.tv[sl], uz + a.ampl * a.tw[sl],
                      std::integral_constant<bool, BGK3>());
     }
-    if (a.Fx != nullptr && rh > R(0))
-        add_feq_diff(ux + (-a.Fx[oc] / rh), uy + (-a.Fy[oc] / rh), uz + (-a.Fz[oc] / rh),
-                     std::integral_constant<bool, false>());
+    if (a.Fx != nullptr && rh > R(0)) {
+        const R ir = recip(rh);
+        add_feq_diff(ux - a.Fx[oc] * ir, uy - a.Fy[oc] * ir, uz - a.Fz[oc] * ir, std::integral_constant<bool, false>());
+    }
 
     // Solid cells: swap the opposite pairs (2,3),(4,5),... (src/m_solids.F90:19-32).
     if (a.solid != nullptr) {
@@ -347,9 +359,30 @@ __device__ __forceinline__ void fold_closed_face(const StepArgs<typename C::real
     }
 }
 
+#if defined(__HIP_DEVICE_COMPILE__)
+// Raw buffer access of 2- or 4-byte values: base (scalar), byte offset voff (vector), byte offset
+// soff (scalar), bounds nbytes.  The descriptor word 3 is the gfx9-family raw-buffer format.
+template <typename raw>
+__device__ __forceinline__ raw buffer_load(const void* base, int nbytes, uint32_t voff, uint32_t soff) {
+    __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, nbytes, 0x00020000);
+    if constexpr (sizeof(raw) == 2) return __builtin_bit_cast(raw, __builtin_amdgcn_raw_buffer_load_b16(r, (int)voff, (int)soff, 0));
+    else return __builtin_bit_cast(raw, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+// NT: the non-temporal cache policy (aux bit 1), as the pointer path's non-temporal stores.
+template <typename raw, bool NT>
+__device__ __forceinline__ void buffer_store(void* base, int nbytes, uint32_t voff, uint32_t soff, raw v) {
+    __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(base, 0, nbytes, 0x00020000);
+    if constexpr (sizeof(raw) == 2)
+        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v), r, (int)voff, (int)soff, NT ? 2 : 0);
+    else
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, (int)voff, (int)soff, NT ? 2 : 0);
+}
+#endif
+
 // One cell of the step at storage coordinates (ca, cb, cc) (1-based along ax0, ax1, ax2):
 // gather the streamed populations, collide, store, and the folded face pass.
-// VAR bit 0: non-temporal stores, bit 1: non-temporal loads (measured slower; kept for tuning).
+// VAR bit 0: non-temporal stores, bit 1: non-temporal loads (measured slower; kept for tuning),
+// bit 3: the buffer is below 4 GiB (32-bit offsets from one base).
 template <int Q, class C, bool BGK3, bool HRR, bool VRE, int VAR>
 __device__ __forceinline__ void step_cell(const StepArgs<typename C::real>& a, int ca, int cb, int cc) {
     using R = typename C::real;
@@ -362,39 +395,79 @@ __device__ __forceinline__ void step_cell(const StepArgs<typename C::real>& a, i
 
     // Source offsets: population l comes from (i - cx, j - cy, k - cz), periodic directions
     // folded onto the interior.  Index d = c + 1.
+    // The cell's own offset is formed once; the neighbours' by adding the uniform stride, and
+    // for a periodic axis at its first/last cell by the uniform wrap distance (no per-offset
+    // multiplies).
     const int s = a.stream;
+    const int oc0 = L.cell(i, j, k);
     int ox[3], oy[3], oz[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        const int c = d - 1;
-        ox[d] = L.base + Layout::wrap(i - s * c, L.nx, L.per_i) * L.si;
-        oy[d] = Layout::wrap(j - s * c, L.ny, L.per_j) * L.sj;
-        oz[d] = Layout::wrap(k - s * c, L.nz, L.per_k) * L.sk;
-    }
+    auto offsets = [&](int x, int n, int per, int st, int (&o)[3]) {
+        // o[d]: offset change of the source x - s*(d-1) along this axis
+        const int wrapd = per ? n * st : 0;
+        const int lo = (per && x == n) ? -wrapd : 0;   // x + 1 wraps to 1
+        const int hi = (per && x == 1) ? wrapd : 0;    // x - 1 wraps to n
+        o[0] = s * (st + lo);                          // source x + 1 (c = -1)
+        o[1] = 0;
+        o[2] = -s * (st - hi);                         // source x - 1 (c = +1)
+    };
+    offsets(i, L.nx, L.per_i, L.si, ox);
+    offsets(j, L.ny, L.per_j, L.sj, oy);
+    offsets(k, L.nz, L.per_k, L.sk, oz);
+    ox[0] += oc0;
+    ox[1] += oc0;
+    ox[2] += oc0;
     // Plane bases are uniform (scalar registers); the per-lane part is a 32-bit byte offset
     // within a plane (planes are below 4 GiB), so loads and stores take the scalar-base form
     // without 64-bit address arithmetic per population.
+    // With VAR bit 3 the whole buffer is below 4 GiB: one scalar base and 32-bit offsets that
+    // include the plane (two scalar registers instead of one pair per plane).
     const char* __restrict__ srcb = static_cast<const char*>(a.src);
     const size_t pbytes = (size_t)L.plane * sizeof(raw);
+    const uint32_t pb32 = (uint32_t)pbytes;
     raw rv[Q];
+    // 2- and 4-byte storage in a buffer below 4 GiB: raw buffer loads/stores, the plane offset in
+    // the scalar offset operand, so an access costs no vector address arithmetic at all.
+#if defined(__HIP_DEVICE_COMPILE__)
+    constexpr bool BUF = (VAR & 8) != 0 && (sizeof(raw) == 2 || sizeof(raw) == 4);
+#else
+    constexpr bool BUF = false;
+#endif
+    // descriptor byte count: an unsigned 32-bit field (the variant runs only below 4 GiB)
+    const int nbytes = (int)(uint32_t)((size_t)Q * pbytes < 0xffffffffull ? (size_t)Q * pbytes : 0xffffffffull);
 #pragma unroll
     for (int l = 0; l < Q; ++l) {
         const uint32_t off = (uint32_t)(oz[Lt::cz(l) + 1] + oy[Lt::cy(l) + 1] + ox[Lt::cx(l) + 1]) * (uint32_t)sizeof(raw);
-        const raw* p = reinterpret_cast<const raw*>(srcb + (size_t)l * pbytes + off);
+        if constexpr (BUF) {
+#if defined(__HIP_DEVICE_COMPILE__)
+            rv[l] = buffer_load<raw>(srcb, nbytes, off, (uint32_t)l * pb32);
+#endif
+            continue;
+        }
+        const raw* p;
+        if constexpr ((VAR & 8) != 0) p = reinterpret_cast<const raw*>(srcb + (off + (uint32_t)l * pb32));
+        else p = reinterpret_cast<const raw*>(srcb + (size_t)l * pbytes + off);
         if constexpr ((VAR & 2) != 0) rv[l] = __builtin_nontemporal_load(p);
         else rv[l] = *p;
     }
 
     R out[Q];
     R tau = 0;
-    const int oc = L.cell(i, j, k);
+    const int oc = oc0;
     collide_cell<Q, C, BGK3, HRR, VRE>(a, rv, i, j, k, ca, cb, cc, oc, out, tau);
 
     char* __restrict__ dstb = static_cast<char*>(a.dst);
     const uint32_t ob = (uint32_t)oc * (uint32_t)sizeof(raw);
 #pragma unroll
     for (int l = 0; l < Q; ++l) {
-        raw* p = reinterpret_cast<raw*>(dstb + (size_t)l * pbytes + ob);
+        if constexpr (BUF) {
+#if defined(__HIP_DEVICE_COMPILE__)
+            buffer_store<raw, (VAR & 1) != 0>(dstb, nbytes, ob, (uint32_t)l * pb32, C::template enc_dep<Q>(out[l], l));
+#endif
+            continue;
+        }
+        raw* p;
+        if constexpr ((VAR & 8) != 0) p = reinterpret_cast<raw*>(dstb + (ob + (uint32_t)l * pb32));
+        else p = reinterpret_cast<raw*>(dstb + (size_t)l * pbytes + ob);
         if constexpr ((VAR & 1) != 0) __builtin_nontemporal_store(C::template enc_dep<Q>(out[l], l), p);
         else *p = C::template enc_dep<Q>(out[l], l);
     }

@@ -275,14 +275,14 @@ def test_storage_axis_choices(codes):
 
 @pytest.mark.parametrize("codes", [(1, 0, 12), (0, 21, 0), (11, 0, 0), (1, 0, 22)])
 def test_folded_face_pass_is_bit_identical(codes, monkeypatch):
-    """With one closed axis the step kernel performs its face pass; the separate face kernel
-    (LBM_NO_FOLD=1) must give the same bits."""
+    """With one closed axis the step kernel can perform its face pass (LBM_FOLD=1, and always in
+    ensembles); the separate face kernel (the single-domain default) must give the same bits."""
     q, dims, steps = 27, (20, 14, 12), 25
     o = dict(ibgk=3, ihrr=1, ivreman=1, smagorinsky=0.16733, kinevisc=2e-3, ibnd=codes[0], jbnd=codes[1], kbnd=codes[2])
     f0 = inputs.perturbed_uniform_flow(q, dims, seed=13, u0=(0.04, 0.01, -0.01), order=3)
     out = []
-    for nofold in ("0", "1"):
-        monkeypatch.setenv("LBM_NO_FOLD", nofold)
+    for fold in ("1", "0"):
+        monkeypatch.setenv("LBM_FOLD", fold)
         dom = Domain(q, dims, "f32", **o)
         dom.set_state(f0)
         dom.set_inflow(0.04, 10.0)
