@@ -130,8 +130,11 @@ __host__ __device__ __forceinline__ void collide_cell(const StepArgs<typename C:
     R D[Q];
 #pragma unroll
     for (int l = 0; l < Q; ++l) D[l] = C::template dep<Q>(rv[l], l);
-    R dr = D[0], jx = 0, jy = 0, jz = 0;
-    R pxx = 0, pyy = 0, pzz = 0, pxy = 0, pxz = 0, pyz = 0;
+    // accumulators start at -0, the additive identity the compiler may fold (0 + x is not x for
+    // x = -0)
+    const R z0 = -R(0);
+    R dr = D[0], jx = z0, jy = z0, jz = z0;
+    R pxx = z0, pyy = z0, pzz = z0, pxy = z0, pxz = z0, pyz = z0;
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
         const int l = 2 * p + 1;
@@ -220,31 +223,47 @@ __host__ __device__ __forceinline__ void collide_cell(const StepArgs<typename C:
         const R ne = HRR ? -(R(1.5) * wS) * T : D[0] - e;
         dev[0] = e + fac * ne;
     }
+    // Factors shared by every direction of a weight class (|c|^2 = 1, 2, 3; directions 1, 7, 19
+    // are the first of each class), hoisted out of the pair loop:
+    //   even part  ev = kE0 + kE2 cu^2 + kF cAc
+    //   odd part   od = kO cu pol + kF (cu (3 cAc - T) + c.(-2 g))
+    // with kE2 = wS*4.5rho, kO = wS*rho, kF = fac*4.5wS, kE0 = wS(rho base - 1) - kF T/3.
+    constexpr int NC = Q == 27 ? 3 : 2;
+    R kE0[NC + 1], kE2[NC + 1], kO[NC + 1], kF[NC + 1];
+    const R T3 = T * (R(1) / R(3));
+#pragma unroll
+    for (int n = 1; n <= NC; ++n) {
+        const R wS = weight<Q, R>(n == 1 ? 1 : n == 2 ? 7 : 19) * S;
+        kE2[n] = wS * r45;
+        kO[n] = wS * rh;
+        kF[n] = fac * (R(4.5) * wS);
+        kE0[n] = HRR ? wS * rb1 - kF[n] * T3 : wS * rb1;
+    }
+    const R a2xy = R(2) * axy, a2xz = R(2) * axz, a2yz = R(2) * ayz;
+    const R gmx = R(-2) * gx, gmy = R(-2) * gy, gmz = R(-2) * gz;
+    const R P0 = R(3) - R(4.5) * usq;
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
         const int l = 2 * p + 1;
         const int cx = Lt::cx(l), cy = Lt::cy(l), cz = Lt::cz(l);
-        const R wS = weight<Q, R>(l) * S;
+        const int n2 = Lt::norm2(l);
         const R cu = sdot<R>(cx, cy, cz, ux, uy, uz);
         const R cu2 = cu * cu;
-        R pol = R(3);
-        if (BGK3) pol += R(4.5) * (cu2 - usq);
-        const R e = wS * (rb1 + r45 * cu2);       // S (E - w)
-        const R o = wS * (rh * (cu * pol));       // S O
+        const R pol = BGK3 ? P0 + R(4.5) * cu2 : R(3);
+        const R cp = cu * pol;
         if (HRR) {
-            R cac = R(0);
+            R cac = sdot<R>(cx * cy, cx * cz, cy * cz, a2xy, a2xz, a2yz);
             if (cx != 0) cac += axx;
             if (cy != 0) cac += ayy;
             if (cz != 0) cac += azz;
-            cac += R(2) * sdot<R>(cx * cy, cx * cz, cy * cz, axy, axz, ayz);
-            const R cg = sdot<R>(cx, cy, cz, gx, gy, gz);
-            const R w45 = R(4.5) * wS;
-            const R ne = w45 * (cac - T * (R(1) / R(3)));
-            const R no = w45 * (cu * (R(3) * cac - T) - R(2) * cg);
-            const R ev = e + fac * ne, od = o + fac * no;
+            const R ev = kE0[n2] + kE2[n2] * cu2 + kF[n2] * cac;
+            const R t2 = cu * (R(3) * cac - T) + sdot<R>(cx, cy, cz, gmx, gmy, gmz);
+            const R od = kO[n2] * cp + kF[n2] * t2;
             dev[l] = ev + od;
             dev[l + 1] = ev - od;
         } else {
+            const R e = kE0[n2] + kE2[n2] * cu2;   // S (E - w)
+            const R o = kO[n2] * cp;               // S O
             dev[l] = (e + o) + fac * (D[l] - (e + o));
             dev[l + 1] = (e - o) + fac * (D[l + 1] - (e - o));
         }
