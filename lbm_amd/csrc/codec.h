@@ -102,14 +102,10 @@ struct CodecF16S {
     }
 };
 
-// bfloat16 = the upper half of an IEEE float; encode rounds to nearest even.
+// bfloat16 = the upper half of an IEEE float; encode rounds to nearest even (gfx950 converts in
+// one instruction, v_cvt_pk_bf16_f32).
 __device__ inline float bf16_bits_to_float(unsigned short r) { return __uint_as_float((unsigned)r << 16); }
-__device__ inline unsigned short float_to_bf16_bits(float x) {
-    unsigned u = __float_as_uint(x);
-    if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (unsigned short)(u >> 16);
-}
+__device__ inline unsigned short float_to_bf16_bits(float x) { return __builtin_bit_cast(unsigned short, (__bf16)x); }
 
 struct CodecBF16S {
     using raw = unsigned short;
