@@ -130,9 +130,12 @@ int lbm_step(lbm_ctx* ctx, int it0, int nsteps);
 int lbm_synchronize(lbm_ctx* ctx);
 void* lbm_stream(lbm_ctx* ctx);
 
-/* Kernel timing with HIP events on the context's stream.  While on, every launch of the
- * step kernel and every group of boundary kernels is bracketed by events; lbm_timing_read
- * synchronises and returns the summed milliseconds and launch counts since the last read. */
+/* Kernel timing with HIP events on the context's stream.  on = 0: off; on = n >= 1: the step
+ * kernel of every n-th step (the first after the call, then every n) and that step's group of
+ * boundary kernels, if it launches any, are bracketed by events (each event record costs the
+ * stream a few microseconds, so n > 1 keeps the overhead off most steps).  lbm_timing_read
+ * synchronises and returns the summed milliseconds and bracketed launch counts since the last
+ * read. */
 int lbm_set_timing(lbm_ctx* ctx, int on);
 int lbm_timing_read(lbm_ctx* ctx, double* step_ms, int* n_step, double* bc_ms, int* n_bc);
 

@@ -3,6 +3,7 @@
 // (src/main.F90:290-329).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -259,16 +260,24 @@ StepHost step_host(const lbm_ctx* c, bool fold) {
         const int code = c->fold_dir == 0 ? d.ibnd : c->fold_dir == 1 ? d.jbnd : d.kbnd;
         opts_of(code, &h.fold_opt_lo, &h.fold_opt_hi);
     }
+    h.heavy_first = h.fold_dir >= 0 && h.fold_dir == c->lay.ax2;
     return h;
 }
 
 // boundarycond(f1, f2) for the non-periodic directions (src/m_boundarycond.F90:166-365).
 // skip_dir: a closed axis whose face pass the step kernel has already performed.
-int boundary(lbm_ctx* c, void* f1, const void* f2, int it, int skip_dir) {
+int boundary(lbm_ctx* c, void* f1, const void* f2, int it, int skip_dir, bool skip_inflow, bool timed) {
     const lbm_desc& d = c->d;
     const Layout& L = c->lay;
+    int a1, a2, b1, b2, k1, k2;
+    opts_of(d.ibnd, &a1, &a2);
+    opts_of(d.jbnd, &b1, &b2);
+    opts_of(d.kbnd, &k1, &k2);
+    // timed: bracket the boundary kernels with events, if this step launches any
+    timed = timed && (c->snap_k.n || c->snap_j.n || (a1 && skip_dir != 0) || (b1 && skip_dir != 1) ||
+                      (k1 && skip_dir != 2) || (d.ibnd == 1 && !skip_inflow));
     hipEvent_t e0 = nullptr;
-    if (c->timing) {
+    if (timed) {
         e0 = next_event(c, c->ev_bc, c->n_bc_ev);
         if (e0) hipEventRecord(e0, c->stream);
     }
@@ -277,21 +286,17 @@ int boundary(lbm_ctx* c, void* f1, const void* f2, int it, int skip_dir) {
     const size_t half = (size_t)8 * d.q * c->ops->raw_bytes;
     if (c->snap_k.n) HIPCHK(c, c->ops->snapshot(L, f1, snap, c->snap_k, c->stream));
     if (c->snap_j.n) HIPCHK(c, c->ops->snapshot(L, f1, snap + half, c->snap_j, c->stream));
-    int o1, oN;
-    opts_of(d.ibnd, &o1, &oN);
-    if (o1 && skip_dir != 0) HIPCHK(c, c->ops->closed(L, f1, f2, 0, o1, oN, nullptr, none, c->stream));
-    opts_of(d.jbnd, &o1, &oN);
-    if (o1 && skip_dir != 1) HIPCHK(c, c->ops->closed(L, f1, f2, 1, o1, oN, snap + half, c->snap_j, c->stream));
-    opts_of(d.kbnd, &o1, &oN);
-    if (o1 && skip_dir != 2) HIPCHK(c, c->ops->closed(L, f1, f2, 2, o1, oN, snap, c->snap_k, c->stream));
+    if (a1 && skip_dir != 0) HIPCHK(c, c->ops->closed(L, f1, f2, 0, a1, a2, nullptr, none, c->stream));
+    if (b1 && skip_dir != 1) HIPCHK(c, c->ops->closed(L, f1, f2, 1, b1, b2, snap + half, c->snap_j, c->stream));
+    if (k1 && skip_dir != 2) HIPCHK(c, c->ops->closed(L, f1, f2, 2, k1, k2, snap, c->snap_k, c->stream));
     const EdgeList e = edge_list(d);
     if (e.n) HIPCHK(c, c->ops->edges(L, f1, f2, e, c->stream));
-    if (d.ibnd == 1) {
+    if (d.ibnd == 1 && !skip_inflow) {
         double ut, cs, sn;
         inflow_values(c, it, &ut, &cs, &sn);
         HIPCHK(c, c->ops->inflow(L, f1, c->shear.p, ut, cs, sn, d.rho0, c->stream));
     }
-    if (c->timing && e0) {
+    if (timed && e0) {
         hipEvent_t e1 = next_event(c, c->ev_bc, c->n_bc_ev);
         if (e1) hipEventRecord(e1, c->stream);
     }
@@ -330,13 +335,18 @@ int lbm_create(const lbm_desc* desc, lbm_ctx** out) {
         default: return fail(nullptr, LBM_EINVAL, "unknown storage kind");
     }
     const Layout lay = make_layout(d.nx, d.ny, d.nz, ops->raw_bytes, d.ibnd == 0, d.jbnd == 0, d.kbnd == 0);
+    // the step kernel of 8-byte (and tangent) storage is compiled for at most 256 threads per block
+    // (more registers per lane, no spills)
+    const int max_threads = ops->raw_bytes >= 8 ? 256 : 1024;
     int bx = d.block_x, by = d.block_y;
     if (bx == 0) {
         const int rows = (lay.n(lay.ax0) + 63) / 64;
-        bx = rows * 64 <= 512 ? rows * 64 : 256;
+        bx = rows * 64 <= std::min(512, max_threads) ? rows * 64 : 256;
     }
     if (by == 0) by = bx >= 256 ? 1 : 256 / bx;
-    if (bx % 64 != 0 || bx * by > 1024 || by < 1) return fail(nullptr, LBM_EINVAL, "block_x must be a multiple of 64, block_x*block_y <= 1024");
+    if (bx % 64 != 0 || bx * by > max_threads || by < 1)
+        return fail(nullptr, LBM_EINVAL, "block_x must be a multiple of 64, block_x*block_y <= " +
+                                             std::to_string(max_threads) + " for this storage");
     int ndev = 0;
     hipError_t e = hipGetDeviceCount(&ndev);
     if (e != hipSuccess || ndev == 0) return fail(nullptr, LBM_EHIP, std::string("no HIP device: ") + hipGetErrorString(e));
@@ -353,19 +363,20 @@ int lbm_create(const lbm_desc* desc, lbm_ctx** out) {
     if (const char* v = getenv("LBM_STEP_VARIANT")) c->variant = atoi(v) & 5;
     if ((c->variant & 4) && (lay.n(lay.ax0) % 2 != 0 || ops->raw_bytes > 4)) c->variant &= ~4;
     {
-        // With one closed axis (no edges, corners or snapshots then; the inflow planes of i are
-        // written after) the step kernel can perform that axis's face pass itself.  Ensembles
-        // always do (a member's face must not need a launch of its own).  A single domain uses
-        // the separate face kernel unless LBM_FOLD=1: measured on MI355X, the folded pass makes
-        // the step kernel's last (wall-plane) blocks a tail, 10-15 us per step against the 7 us
-        // launch it saves.
+        // With one closed axis (no edges, corners or snapshots then) the step kernel can perform
+        // that axis's face pass itself, and with at most that one closed axis the inflow/outflow
+        // planes of i as well: a step is then one launch.  Ensembles always fold the face (a
+        // member's face must not need a launch of its own).  LBM_FOLD=0 selects the separate
+        // boundary kernels for a single domain.
         const int codes[3] = {d.ibnd, d.jbnd, d.kbnd};
         int nclosed = 0, cl = -1;
         for (int a = 0; a < 3; ++a)
             if (codes[a] > 10) { ++nclosed; cl = a; }
         if (nclosed == 1) c->fold_dir = cl;
         const char* fv = getenv("LBM_FOLD");
-        c->fold_on = c->fold_dir >= 0 && fv && atoi(fv);
+        const bool fold = fv ? atoi(fv) != 0 : kFoldDefault;
+        c->fold_on = c->fold_dir >= 0 && fold;
+        c->inflow_fold_on = d.ibnd == 1 && fold && (nclosed == 0 || c->fold_on) && !(c->variant & 4);
     }
     hipGetDevice(&c->device);
     c->lay = lay;
@@ -475,6 +486,11 @@ int lbm_step(lbm_ctx* c, int it0, int nsteps) {
     if (nsteps < 0) return fail(c, LBM_EINVAL, "nsteps < 0");
     const lbm_desc& d = c->d;
     StepHost h = step_host(c, c->fold_on);
+    if (c->inflow_fold_on) {
+        h.inflow_fold = 1;
+        h.shear = c->shear.p;
+        h.rho0 = d.rho0;
+    }
     const size_t slice = (size_t)d.ny * d.nz * c->ops->real_bytes;
     for (int it = it0 + 1; it <= it0 + nsteps; ++it) {
         if (c->pending) {
@@ -498,17 +514,22 @@ int lbm_step(lbm_ctx* c, int it0, int nsteps) {
         } else {
             h.turb[0] = h.turb[1] = h.turb[2] = nullptr;
         }
+        if (h.inflow_fold) inflow_values(c, it, &h.utime, &h.cosd, &h.sind);
+        // events on every timing_every-th step only: each record costs the stream a few us
+        const bool timed = c->timing && c->timing_seq++ % c->timing_every == 0;
         hipEvent_t e0 = nullptr;
-        if (c->timing && (e0 = next_event(c, c->ev_step, c->n_step_ev))) hipEventRecord(e0, c->stream);
+        if (timed && (e0 = next_event(c, c->ev_step, c->n_step_ev))) hipEventRecord(e0, c->stream);
         HIPCHK(c, c->ops->step(c->lay, h, c->stream));
-        if (c->timing && e0) {
+        if (timed && e0) {
             hipEvent_t e1 = next_event(c, c->ev_step, c->n_step_ev);
             if (e1) hipEventRecord(e1, c->stream);
         }
         if (!c->pending) c->cur ^= 1;
         c->pending = false;
         c->use_ext = false;
-        if (int rc = boundary(c, c->buf[c->cur], c->buf[c->cur ^ 1], it, c->fold_on ? c->fold_dir : -1)) return rc;
+        if (int rc = boundary(c, c->buf[c->cur], c->buf[c->cur ^ 1], it, c->fold_on ? c->fold_dir : -1,
+                              c->inflow_fold_on, timed))
+            return rc;
         c->ghost_raw[c->cur] = false;
         c->avg_pending = c->avg_on;
     }
@@ -826,7 +847,10 @@ void* lbm_stream(lbm_ctx* c) { return c ? (void*)c->stream : nullptr; }
 
 int lbm_set_timing(lbm_ctx* c, int on) {
     if (!c) return fail(c, LBM_EINVAL, "null context");
+    if (on < 0) return fail(c, LBM_EINVAL, "timing interval < 0");
     c->timing = on != 0;
+    c->timing_every = on > 1 ? on : 1;
+    c->timing_seq = 0;
     c->n_step_ev = c->n_bc_ev = 0;
     return LBM_OK;
 }

@@ -53,11 +53,15 @@ struct lbm_ctx {
     int avg_count = 0;
     int fold_dir = -1;             // the only closed axis (-1: none or several): its face pass can
                                    // run inside the step kernel (always in ensembles)
-    bool fold_on = false;          // single-domain steps fold it (LBM_FOLD=1)
+    bool fold_on = false;          // single-domain steps fold it
+    bool inflow_fold_on = false;   // single-domain steps write the inflow/outflow planes of i in the
+                                   // step kernel (with fold_on, or with no closed axis)
     int variant = 1;               // step-kernel variant (LBM_STEP_VARIANT), see impl.h; non-temporal
                                    // stores measured fastest on MI355X (5.32 vs 5.11 TB/s)
     // timing
     bool timing = false;
+    int timing_every = 1;          // events around every timing_every-th step
+    long long timing_seq = 0;
     std::vector<hipEvent_t> ev_step, ev_bc;   // pairs
     size_t n_step_ev = 0, n_bc_ev = 0;
     std::string err;
@@ -78,8 +82,14 @@ void inflow_values(const lbm_ctx* c, int it, double* ut, double* cs, double* sn)
 // the step-kernel parameters of c that do not change from step to step; fold: the step kernel
 // performs the face pass of c->fold_dir
 lbm::StepHost step_host(const lbm_ctx* c, bool fold);
-// boundarycond(f1, f2) for the non-periodic directions; skip_dir: face done by the step kernel
-int boundary(lbm_ctx* c, void* f1, const void* f2, int it, int skip_dir = -1);
+// boundarycond(f1, f2) for the non-periodic directions; skip_dir: face done by the step kernel;
+// skip_inflow: inflow/outflow planes done by the step kernel
+int boundary(lbm_ctx* c, void* f1, const void* f2, int it, int skip_dir = -1, bool skip_inflow = false,
+             bool timed = false);
+// single domains fold the closed face and the inflow planes into the step kernel unless LBM_FOLD=0
+// (measured on MI355X, D3Q27 384x192x128: one launch per step instead of three, +3 % for FP16
+// storage, +0.5 % for FP32)
+constexpr bool kFoldDefault = true;
 
 }  // namespace lbm_host
 

@@ -47,6 +47,10 @@ struct StepHost {
     void* avg;            // averaging sums (6 cell planes of the compute type) or nullptr
     void* tau_out;
     int fold_dir, fold_opt_lo, fold_opt_hi;   // closed face performed by the step kernel
+    int inflow_fold;                          // inflow/outflow planes performed by the step kernel
+    const void* shear;                        // uvel_shear(nz) (compute type)
+    double utime, cosd, sind, rho0;           // inflow values of the step
+    int heavy_first;                          // slowest storage axis: planes 1 and n first
     int bgk3, hrr, vre;
     int variant;      // step-kernel variant: bit 0 non-temporal stores, bit 2 two cells per lane
     int block_x, block_y;

@@ -55,6 +55,11 @@ struct StepArgs {
     R* tau_out;            // cell layout; nullptr: tau not stored
     int fold_dir;          // closed axis whose face pass this kernel performs (-1: none)
     int fold_opt_lo, fold_opt_hi;
+    // inflow/outflow planes of i written by the cells at i = 1 and i = nx (0: separate pass)
+    int inflow_fold;
+    const R* shear;        // uvel_shear(nz)
+    R utime, cosd, sind, rho0;
+    int heavy_first;       // 1: the blocks of the slowest storage axis run its planes 1 and n first
 };
 
 // Signed sum  sx*a + sy*b + sz*c  for compile-time signs in {-1,0,1}: adds only.
@@ -373,8 +378,77 @@ __device__ __forceinline__ void fold_closed_face(const StepArgs<typename C::real
                 }
             }
             if (push) f1[(size_t)l * L.plane + L.cell(t[0], t[1], t[2])] = C::template enc_dep<Q>(dev[l], l);
-            if (from_ghost) f1[(size_t)l * L.plane + og] = f1[(size_t)l * L.plane + L.cell(s[0], s[1], s[2])];
+            // with the inflow planes folded too, the cell that rewrites that ghost copies it first
+            if (from_ghost && !a.inflow_fold)
+                f1[(size_t)l * L.plane + og] = f1[(size_t)l * L.plane + L.cell(s[0], s[1], s[2])];
         }
+    }
+}
+
+// Inflow ghost values (src/m_boundary_i_inflow_kernel.F90:61-74) in place over the decoded
+// populations of cell (1, j, k): fg_l - 2 w_l rho0 (c_x u cos(udir) + c_y u sin(udir)) / cs^2.
+// The ghost takes fg_l for c_x <= 0 and fg of the opposite direction otherwise (:89-104).
+template <int Q, typename R>
+__host__ __device__ __forceinline__ void inflow_ghost_values(R (&fg)[Q], R uu, R cosd, R sind, R rho0) {
+    using Lt = Lattice<Q>;
+    const R cs2 = R(1) / R(3);
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+        const R wl = weight<Q, R>(l);
+        const R cxl = R(Lt::cx(l)), cyl = R(Lt::cy(l));
+        fg[l] = fg[l] - R(2) * wl * rho0 * (cxl * uu * cosd + cyl * uu * sind) / cs2;
+    }
+}
+
+// The inflow/outflow pass for the ghost row (j, k) of i, performed by the cell (i, j, k) with
+// i = 1 (inflow ghost i = 0) or i = nx (outflow ghost i = nx+1, a copy of the cell) from the
+// values it has just stored.  When the closed face of fold_dir is folded as well, the face pass
+// reads this ghost's value from before the step (f1(l, G - c_l) on the wall plane, G - c_l on
+// the inflow axis): those copies are made here, before the ghost is rewritten.
+template <int Q, class C>
+__device__ __forceinline__ void fold_inflow(const StepArgs<typename C::real>& a, int i, int j, int k,
+                                            const typename C::real (&dev)[Q]) {
+    using R = typename C::real;
+    using raw = typename C::raw;
+    using Lt = Lattice<Q>;
+    const Layout& L = a.lay;
+    raw* __restrict__ f1 = static_cast<raw*>(a.dst);
+    const int gi = i == 1 ? 0 : L.nx + 1;
+    const int og = L.cell(gi, j, k);
+    const int D = a.fold_dir;
+    if (D > 0) {
+        const int x[3] = {i, j, k};
+        const int n[3] = {L.nx, L.ny, L.nz};
+        const int O = 3 - D;   // the other tangential axis (periodic: only D is closed)
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            const int plane = side ? n[D] : 1;
+            if (x[D] != plane) continue;
+#pragma unroll
+            for (int l = 0; l < Q; ++l) {
+                const int cn = Lt::c(l, D);
+                if (!(side ? cn > 0 : cn < 0)) continue;
+                // the target ghost G = (gi + c_x, x_O + c_O, ghost of D) lies in the face's range
+                if (Lt::cx(l) != (i == 1 ? 1 : -1)) continue;
+                int t[3];
+                t[0] = i;
+                t[D] = side ? n[D] + 1 : 0;
+                t[O] = Layout::wrap(x[O] + Lt::c(l, O), n[O], 1);
+                f1[(size_t)l * L.plane + L.cell(t[0], t[1], t[2])] = f1[(size_t)l * L.plane + og];
+            }
+        }
+    }
+    if (i == 1) {
+        R fg[Q];
+#pragma unroll
+        for (int l = 0; l < Q; ++l) fg[l] = C::template decode<Q>(C::template enc_dep<Q>(dev[l], l), l);
+        inflow_ghost_values<Q, R>(fg, a.utime * a.shear[k - 1], a.cosd, a.sind, a.rho0);
+#pragma unroll
+        for (int l = 0; l < Q; ++l)
+            f1[(size_t)l * L.plane + og] = C::template encode<Q>(Lt::cx(l) <= 0 ? fg[l] : fg[Lt::opposite(l)], l);
+    } else {
+#pragma unroll
+        for (int l = 0; l < Q; ++l) f1[(size_t)l * L.plane + og] = C::template enc_dep<Q>(dev[l], l);
     }
 }
 
@@ -492,13 +566,19 @@ __device__ __forceinline__ void step_cell(const StepArgs<typename C::real>& a, i
     }
     if (VRE && a.tau_out != nullptr) a.tau_out[oc] = tau;
     if (a.fold_dir >= 0) fold_closed_face<Q, C>(a, i, j, k, out);
+    if (a.inflow_fold && (i == 1 || i == L.nx)) fold_inflow<Q, C>(a, i, j, k, out);
 }
 
 // Threads run along the storage axes: x along the contiguous axis ax0, y along ax1, z along ax2.
+// With heavy_first the slowest axis's planes are taken in the order 1, n, 2, 3, .., n-1: the planes
+// of a folded face (and of folded inflow/outflow planes) carry the extra stores and start first
+// instead of forming the launch's tail.
 template <int Q, class C, bool BGK3, bool HRR, bool VRE, int VAR>
-__global__ void __launch_bounds__(1024) step_kernel(const StepArgs<typename C::real> a) {
+__global__ void __launch_bounds__(sizeof(typename C::raw) >= 8 ? 256 : 1024, sizeof(typename C::raw) >= 8 ? 3 : 1) step_kernel(const StepArgs<typename C::real> a) {
+    const int z = blockIdx.z;
+    const int cc = !a.heavy_first ? z + 1 : z == 0 ? 1 : z == 1 ? a.lay.n(a.lay.ax2) : z;
     step_cell<Q, C, BGK3, HRR, VRE, VAR>(a, blockIdx.x * blockDim.x + threadIdx.x + 1,
-                                         blockIdx.y * blockDim.y + threadIdx.y + 1, blockIdx.z + 1);
+                                         blockIdx.y * blockDim.y + threadIdx.y + 1, cc);
 }
 
 // Two raw values packed in one aligned word: 32 bits for 16-bit storage, 64 bits for FP32.
@@ -749,7 +829,6 @@ __device__ __forceinline__ void inflow_cell(const Layout& L, void* f1, const typ
     using Lt = Lattice<Q>;
     raw* __restrict__ p = static_cast<raw*>(f1);
     const R uu = utime * shear[k - 1];        // uvel(k) = uvel_time(it)*uvel_shear(k), taper = 1
-    const R cs2 = R(1) / R(3);
     const int o1 = L.cell(1, j, k), o0 = L.cell(0, j, k), on = L.cell(L.nx, j, k), oe = L.cell(L.nx + 1, j, k);
     R fg[Q];
     raw fo[Q];
@@ -758,12 +837,7 @@ __device__ __forceinline__ void inflow_cell(const Layout& L, void* f1, const typ
         fg[l] = C::template decode<Q>(p[(size_t)l * L.plane + o1], l);
         fo[l] = p[(size_t)l * L.plane + on];
     }
-#pragma unroll
-    for (int l = 0; l < Q; ++l) {
-        const R wl = weight<Q, R>(l);
-        const R cxl = R(Lt::cx(l)), cyl = R(Lt::cy(l));
-        fg[l] = fg[l] - R(2) * wl * rho0 * (cxl * uu * cosd + cyl * uu * sind) / cs2;
-    }
+    inflow_ghost_values<Q, R>(fg, uu, cosd, sind, rho0);
 #pragma unroll
     for (int l = 0; l < Q; ++l) {
         p[(size_t)l * L.plane + o0] = C::template encode<Q>(Lt::cx(l) <= 0 ? fg[l] : fg[Lt::opposite(l)], l);

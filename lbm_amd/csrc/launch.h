@@ -42,7 +42,7 @@ struct Launch {
     static hipError_t step_t(const Layout& L, const StepArgs<R>& a, dim3 grid, dim3 block, int var, hipStream_t s) {
         const bool nt = var & 1;
         if constexpr (sizeof(typename C::raw) <= 4) {
-            if ((var & 4) && a.fold_dir < 0) {
+            if ((var & 4) && a.fold_dir < 0 && !a.inflow_fold) {
                 // two cells per lane along the contiguous axis
                 const long long nt_slab = (long long)(L.n(L.ax0) / 2) * L.n(L.ax1);
                 const dim3 g((unsigned)((nt_slab + 255) / 256), L.n(L.ax2), 1);
@@ -100,6 +100,13 @@ struct Launch {
         a.fold_dir = h.fold_dir;
         a.fold_opt_lo = h.fold_opt_lo;
         a.fold_opt_hi = h.fold_opt_hi;
+        a.inflow_fold = h.inflow_fold;
+        a.shear = static_cast<const R*>(h.shear);
+        a.utime = (R)h.utime;
+        a.cosd = (R)h.cosd;
+        a.sind = (R)h.sind;
+        a.rho0 = (R)h.rho0;
+        a.heavy_first = h.heavy_first;
         return a;
     }
 

@@ -309,8 +309,9 @@ class Domain:
     def synchronize(self):
         self._chk(self.lib.lbm_synchronize(self.h), "lbm_synchronize")
 
-    def set_timing(self, on=True):
-        self._chk(self.lib.lbm_set_timing(self.h, int(on)), "lbm_set_timing")
+    def set_timing(self, on=True, every=1):
+        """HIP-event timing of the step kernel (and boundary kernels) of every ``every``-th step."""
+        self._chk(self.lib.lbm_set_timing(self.h, int(every) if on else 0), "lbm_set_timing")
 
     def timing_read(self):
         s, b = C.c_double(), C.c_double()

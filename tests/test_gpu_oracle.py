@@ -273,24 +273,37 @@ def test_storage_axis_choices(codes):
     assert np.abs(rho - r0).max() <= 1e-5 and max(np.abs(u - u0).max(), np.abs(v - v0).max()) <= 1e-6
 
 
-@pytest.mark.parametrize("codes", [(1, 0, 12), (0, 21, 0), (11, 0, 0), (1, 0, 22)])
-def test_folded_face_pass_is_bit_identical(codes, monkeypatch):
-    """With one closed axis the step kernel can perform its face pass (LBM_FOLD=1, and always in
-    ensembles); the separate face kernel (the single-domain default) must give the same bits."""
+@pytest.mark.parametrize("codes,storage", [((1, 0, 12), "f32"), ((0, 21, 0), "f32"), ((11, 0, 0), "f32"),
+                                           ((1, 0, 22), "f32"), ((1, 12, 0), "f32"), ((1, 0, 0), "f32"),
+                                           ((1, 21, 11), "f32"), ((1, 0, 12), "f16s"), ((1, 12, 0), "f64")])
+def test_folded_boundary_passes_are_bit_identical(codes, storage, monkeypatch):
+    """With at most one closed axis the step kernel performs that axis's face pass and the
+    inflow/outflow planes of i itself (ensembles always fold the face); the separate boundary
+    kernels (LBM_FOLD=0) must give the same bits, with solids on the inflow and wall planes and
+    inflow turbulence.  Two closed axes (edges) keep the separate kernels either way."""
     q, dims, steps = 27, (20, 14, 12), 25
     o = dict(ibgk=3, ihrr=1, ivreman=1, smagorinsky=0.16733, kinevisc=2e-3, ibnd=codes[0], jbnd=codes[1], kbnd=codes[2])
     f0 = inputs.perturbed_uniform_flow(q, dims, seed=13, u0=(0.04, 0.01, -0.01), order=3)
+    mask = inputs.box_mask(dims, (1, 5, 1), (3, 8, 3))
+    tu = inputs.turbulence_fields(13, dims[1], dims[2], 10)
+    print(f"fold codes={codes} storage={storage}", flush=True)
     out = []
     for fold in ("1", "0"):
         monkeypatch.setenv("LBM_FOLD", fold)
-        dom = Domain(q, dims, "f32", **o)
+        dom = Domain(q, dims, storage, **o)
         dom.set_state(f0)
         dom.set_inflow(0.04, 10.0)
+        dom.set_blanking(mask)
+        if codes[0] == 1:
+            dom.set_turbulence(tu[0], tu[1], tu[2], 10, 1e-3)
         dom.step(1, steps)
         out.append(dom.get_state())
         dom.close()
     assert np.array_equal(out[0], out[1])
+    if storage != "f32":
+        return
     sim = oracle.OracleSim(q, dims, "f32", **{k: (f32(v) if isinstance(v, float) else v) for k, v in o.items()})
     sim.restart(f0)
-    sim.run(1, 1 + steps, uini=f32(0.04), udir=f32(10.0))
+    sim.run(1, 1 + steps, uini=f32(0.04), udir=f32(10.0), blank=mask,
+            turb=(tu[0], tu[1], tu[2], 10, f32(1e-3)) if codes[0] == 1 else None)
     check(out[0], sim.f1, q, "f32")
