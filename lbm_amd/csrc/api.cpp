@@ -260,7 +260,7 @@ StepHost step_host(const lbm_ctx* c, bool fold) {
         const int code = c->fold_dir == 0 ? d.ibnd : c->fold_dir == 1 ? d.jbnd : d.kbnd;
         opts_of(code, &h.fold_opt_lo, &h.fold_opt_hi);
     }
-    h.heavy_first = h.fold_dir >= 0 && h.fold_dir == c->lay.ax2;
+    h.heavy_first = h.fold_dir < 0 ? 0 : h.fold_dir == c->lay.ax1 ? 1 : h.fold_dir == c->lay.ax2 ? 2 : 0;
     return h;
 }
 
@@ -490,6 +490,7 @@ int lbm_step(lbm_ctx* c, int it0, int nsteps) {
         h.inflow_fold = 1;
         h.shear = c->shear.p;
         h.rho0 = d.rho0;
+        h.heavy_first |= c->lay.ax1 == 0 ? 1 : c->lay.ax2 == 0 ? 2 : 0;
     }
     const size_t slice = (size_t)d.ny * d.nz * c->ops->real_bytes;
     for (int it = it0 + 1; it <= it0 + nsteps; ++it) {

@@ -4,7 +4,9 @@
 //   copy, 2 read only, 3 write only, 4 one float4 per thread, 5 27 planes per thread as the
 //   lattice kernels move them, 6/7/8 the same with 1/2/4 cells per thread and the third argument
 //   as dynamic LDS bytes per block to cap occupancy, 9/10 the row-blocked layout with rows of
-//   256/64 cells; modes 4-10 size their own grid)
+//   256/64 cells; modes 4-10 size their own grid; 11/12 the D3Q27 pull-stream copy of 4-byte
+//   values over the 384x192x128 workload in the plane / row-blocked layout, 13/14 the same with
+//   2-byte values)
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -106,6 +108,44 @@ __global__ void __launch_bounds__(256) rows_kernel(const float* __restrict__ src
     for (int l = 0; l < 27; ++l) __builtin_nontemporal_store(v[l], dst + base + (size_t)l * R);
 }
 
+// The step kernel's memory traffic without its arithmetic: a D3Q27 pull-stream copy of FP32
+// populations over the 384x192x128 workload (contiguous axis a0 = 192 cells, padded rows of 256
+// with the first interior cell 128-byte aligned), one cell per thread, blocks of one row, raw
+// buffer loads with the population offset in the scalar operand, non-temporal stores.  The
+// element of population l at (a0, a1, a2) is at  l*PL + a2*S2 + a1*S1 + a0 + 31:
+//   planes (the library's layout)          PL = plane, S1 = 256,      S2 = 256*(n1+2)
+//   row-blocked [row][population][256]     PL = 256,   S1 = 27*256,   S2 = 27*256*(n1+2)
+struct LatShape {
+    int n0, n1, n2;
+    unsigned PL, S1, S2;
+    unsigned elems;   // per buffer
+};
+// B: bytes per value (4: FP32 storage, 2: 16-bit storage)
+template <int B>
+__global__ void __launch_bounds__(1024) lattice_copy_kernel(const void* __restrict__ src, void* __restrict__ dst, LatShape g) {
+    extern __shared__ float pad[];
+    const int a0 = threadIdx.x + 1, a1 = blockIdx.y + 1, a2 = blockIdx.z + 1;
+    if (a0 > g.n0) return;
+    if (a0 == 0) pad[0] = 0.f;   // never true: keeps the dynamic LDS allocation
+    const unsigned own = (unsigned)a2 * g.S2 + (unsigned)a1 * g.S1 + (unsigned)a0 + 31u;
+    const int nbytes = (int)(g.elems * (unsigned)B);
+    __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(src), 0, nbytes, 0x00020000);
+    __amdgpu_buffer_rsrc_t rd = __builtin_amdgcn_make_buffer_rsrc(dst, 0, nbytes, 0x00020000);
+    unsigned v[27];
+#pragma unroll
+    for (int l = 0; l < 27; ++l) {
+        const int c0 = l % 3 - 1, c1 = (l / 3) % 3 - 1, c2 = l / 9 - 1;
+        const int off = -(c2 * (int)g.S2 + c1 * (int)g.S1 + c0);
+        if constexpr (B == 4) v[l] = __builtin_amdgcn_raw_buffer_load_b32(rs, (int)((own + off) * 4u), (int)(l * g.PL * 4u), 0);
+        else v[l] = __builtin_amdgcn_raw_buffer_load_b16(rs, (int)((own + off) * 2u), (int)(l * g.PL * 2u), 0);
+    }
+#pragma unroll
+    for (int l = 0; l < 27; ++l) {
+        if constexpr (B == 4) __builtin_amdgcn_raw_buffer_store_b32(v[l], rd, (int)(own * 4u), (int)(l * g.PL * 4u), 2);
+        else __builtin_amdgcn_raw_buffer_store_b16((unsigned short)v[l], rd, (int)(own * 2u), (int)(l * g.PL * 2u), 2);
+    }
+}
+
 #define CHK(x)                                                                      \
     do {                                                                            \
         hipError_t e = (x);                                                         \
@@ -134,12 +174,51 @@ int main(int argc, char** argv) {
     // bytes per block (occupancy cap)
     const int W = mode == 7 ? 2 : mode == 8 ? 4 : 1;
     const size_t nrow = ncell / 256 * 256;   // rows_kernel: whole rows only
+    // modes 6-8 and 11-14: argument 3 is dynamic LDS bytes per block (occupancy cap)
     const int lds = mode >= 6 && argc > 3 ? atoi(argv[3]) : 0;
     const int blocks = mode == 4 ? (int)((n + 255) / 256)
                        : mode == 5 ? (int)((ncell + 255) / 256)
                        : mode >= 9 ? (int)(nrow / 256)
                        : mode >= 6 ? (int)((ncell / W + 255) / 256)
                                    : (argc > 3 ? atoi(argv[3]) : prop.multiProcessorCount * 16);
+    if (mode >= 11 && mode <= 14) {
+        const int B = mode <= 12 ? 4 : 2;
+        // lattice_copy_kernel over two buffers of its own (the first argument is ignored)
+        LatShape g;
+        g.n0 = 192; g.n1 = 384; g.n2 = 128;
+        const unsigned rowl = 256, nrows = (unsigned)(g.n1 + 2) * (g.n2 + 2);
+        g.elems = 27u * rowl * nrows + 256u;
+        if (mode == 11 || mode == 13) { g.PL = rowl * nrows; g.S1 = rowl; g.S2 = rowl * (g.n1 + 2); }
+        else { g.PL = rowl; g.S1 = 27u * rowl; g.S2 = 27u * rowl * (g.n1 + 2); }
+        float *la = nullptr, *lb = nullptr;
+        CHK(hipMalloc(&la, (size_t)g.elems * B));
+        CHK(hipMalloc(&lb, (size_t)g.elems * B));
+        CHK(hipMemset(la, 0, (size_t)g.elems * B));
+        CHK(hipMemset(lb, 0, (size_t)g.elems * B));
+        auto lat = [&](const float* s_, float* d_) {
+            if (B == 4) hipLaunchKernelGGL(lattice_copy_kernel<4>, dim3(1, g.n1, g.n2), dim3(192), lds, 0, s_, d_, g);
+            else hipLaunchKernelGGL(lattice_copy_kernel<2>, dim3(1, g.n1, g.n2), dim3(192), lds, 0, s_, d_, g);
+        };
+        hipEvent_t e0, e1;
+        CHK(hipEventCreate(&e0));
+        CHK(hipEventCreate(&e1));
+        lat(la, lb);
+        CHK(hipDeviceSynchronize());
+        CHK(hipEventRecord(e0, 0));
+        for (int it = 0; it < iters; ++it) lat((it & 1) ? lb : la, (it & 1) ? la : lb);
+        CHK(hipEventRecord(e1, 0));
+        CHK(hipEventSynchronize(e1));
+        float ms = 0;
+        CHK(hipEventElapsedTime(&ms, e0, e1));
+        const double per = ms / iters, moved = 2.0 * 27 * B * (double)g.n0 * g.n1 * g.n2;
+        const char* ln[4] = {"lattice_copy_planes_b32", "lattice_copy_rows_b32", "lattice_copy_planes_b16",
+                             "lattice_copy_rows_b16"};
+        printf("{\"kernel\": \"%s\", \"bytes_moved\": %.0f, \"ms_per_launch\": %.5f, \"GBps\": %.1f}\n",
+               ln[mode - 11], moved, per, moved / (per * 1e-3) / 1e9);
+        CHK(hipFree(la));
+        CHK(hipFree(lb));
+        return 0;
+    }
     const char* names[11] = {"copy_kernel", "copy_nt_kernel", "read_kernel", "write_kernel", "copy_once_kernel",
                              "planes_kernel", "planes_vec1", "planes_vec2", "planes_vec4", "rows_256", "rows_64"};
     const double rd = (mode == 3) ? 0.0 : 1.0, wr = (mode == 2) ? 0.0 : 1.0;

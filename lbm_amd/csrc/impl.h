@@ -50,7 +50,7 @@ struct StepHost {
     int inflow_fold;                          // inflow/outflow planes performed by the step kernel
     const void* shear;                        // uvel_shear(nz) (compute type)
     double utime, cosd, sind, rho0;           // inflow values of the step
-    int heavy_first;                          // slowest storage axis: planes 1 and n first
+    int heavy_first;                          // bit 0 / 1: cells 1 and n of ax1 / ax2 first
     int bgk3, hrr, vre;
     int variant;      // step-kernel variant: bit 0 non-temporal stores, bit 2 two cells per lane
     int block_x, block_y;

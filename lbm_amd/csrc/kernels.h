@@ -59,7 +59,7 @@ struct StepArgs {
     int inflow_fold;
     const R* shear;        // uvel_shear(nz)
     R utime, cosd, sind, rho0;
-    int heavy_first;       // 1: the blocks of the slowest storage axis run its planes 1 and n first
+    int heavy_first;       // bit 0 / bit 1: cells 1 and n of ax1 / ax2 are dispatched first
 };
 
 // Signed sum  sx*a + sy*b + sz*c  for compile-time signs in {-1,0,1}: adds only.
@@ -570,15 +570,19 @@ __device__ __forceinline__ void step_cell(const StepArgs<typename C::real>& a, i
 }
 
 // Threads run along the storage axes: x along the contiguous axis ax0, y along ax1, z along ax2.
-// With heavy_first the slowest axis's planes are taken in the order 1, n, 2, 3, .., n-1: the planes
-// of a folded face (and of folded inflow/outflow planes) carry the extra stores and start first
-// instead of forming the launch's tail.
+// heavy_first bit 0 (ax1) / bit 1 (ax2): that axis's cells are taken in the order 1, n, 2, 3, ..,
+// n-1, so that the rows and planes of a folded face or of the folded inflow/outflow planes, which
+// carry extra stores, are dispatched first (blocks start in x, y, z order) instead of forming the
+// launch's tail.
+__device__ __forceinline__ int heavy_order(int x, int n) { return x == 0 ? 1 : x == 1 ? n : x; }
+
 template <int Q, class C, bool BGK3, bool HRR, bool VRE, int VAR>
 __global__ void __launch_bounds__(sizeof(typename C::raw) >= 8 ? 256 : 1024, sizeof(typename C::raw) >= 8 ? 3 : 1) step_kernel(const StepArgs<typename C::real> a) {
-    const int z = blockIdx.z;
-    const int cc = !a.heavy_first ? z + 1 : z == 0 ? 1 : z == 1 ? a.lay.n(a.lay.ax2) : z;
-    step_cell<Q, C, BGK3, HRR, VRE, VAR>(a, blockIdx.x * blockDim.x + threadIdx.x + 1,
-                                         blockIdx.y * blockDim.y + threadIdx.y + 1, cc);
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int n1 = a.lay.n(a.lay.ax1);
+    const int cb = (a.heavy_first & 1) && y < n1 ? heavy_order(y, n1) : y + 1;
+    const int cc = (a.heavy_first & 2) ? heavy_order(blockIdx.z, a.lay.n(a.lay.ax2)) : blockIdx.z + 1;
+    step_cell<Q, C, BGK3, HRR, VRE, VAR>(a, blockIdx.x * blockDim.x + threadIdx.x + 1, cb, cc);
 }
 
 // Two raw values packed in one aligned word: 32 bits for 16-bit storage, 64 bits for FP32.
