@@ -267,6 +267,76 @@ def bench_adjoint(a):
     print(json.dumps(line))
 
 
+def bench_tile(a):
+    """One GPU's share of the headline workload tiled along j (tile 4 of 8: j = 73..96, through the
+    cylinder), with its halo exchange every step: pack both boundary planes, hand each buffer over
+    (a device-to-device copy on the context's stream where RCCL would send/recv), unpack."""
+    import torch
+    from lbm_amd.domain import BND_HALO, Domain
+    w = workload(a.storage)
+    nx, ny, nz = w["dims"]
+    ntile = a.tiles
+    if ny % ntile:
+        raise SystemExit(f"ny = {ny} is not a multiple of --tiles {ntile}")
+    nt = ny // ntile
+    t = min(3, ntile - 1)
+    j0, j1 = t * nt + 1, (t + 1) * nt
+    nrturb = max(1001, a.steps + a.warmup + 1)
+    print(f"[bench] tile j = {j0}..{j1} of {w['dims']} storage={a.storage}", file=sys.stderr, flush=True)
+    f0, turb, mask = make_inputs(w, nrturb)
+    idx = [(j - 1) % ny + 1 for j in range(j0 - 1, j1 + 2)]
+    f0t = np.ascontiguousarray(f0[:, idx])
+    maskt = np.ascontiguousarray(mask[:, idx])
+    turbt = [np.ascontiguousarray(x[:, :, j0 - 1:j1]) for x in turb[:3]]
+    del f0, mask
+    dom = Domain(w["q"], (nx, nt, nz), a.storage, ibgk=w["ibgk"], ihrr=w["ihrr"], ivreman=w["ivreman"],
+                 smagorinsky=w["smagorinsky"], kinevisc=w["kinevisc"], ibnd=w["ibnd"], jbnd=BND_HALO,
+                 kbnd=w["kbnd"], iturb_pos=w["ip"])
+    dom.set_state(f0t)
+    dom.set_inflow(w["uini"], w["udir"])
+    dom.set_blanking(maskt)
+    dom.set_turbulence(turbt[0], turbt[1], turbt[2], nrturb, w["ampl"])
+    nb = dom.halo_bytes()
+    snd = torch.empty(2 * nb, dtype=torch.uint8, device="cuda")   # side 0's halo, then side 1's
+    rcv = torch.empty(2 * nb, dtype=torch.uint8, device="cuda")
+    ext = torch.cuda.ExternalStream(dom.stream_ptr())
+
+    def step(it):
+        dom.step(it, 1)
+        dom.halo_pack(2, snd.data_ptr())
+        with torch.cuda.stream(ext):   # the transfer (RCCL send/recv); this tile is its own neighbour
+            rcv[:nb].copy_(snd[nb:])
+            rcv[nb:].copy_(snd[:nb])
+        dom.halo_unpack(2, rcv.data_ptr())
+
+    for it in range(1, a.warmup + 1):
+        step(it)
+    dom.synchronize()
+    dom.set_timing(True, every=10)
+    t0 = time.perf_counter()
+    for it in range(a.warmup + 1, a.warmup + a.steps + 1):
+        step(it)
+    dom.synchronize()
+    secs = time.perf_counter() - t0
+    tm = dom.timing_read()
+    rho, u, v, ww = dom.macrovars()
+    finite = bool(np.isfinite(rho).all() and np.isfinite(u).all())
+    dom.close()
+    cells = nx * nt * nz
+    kern_ms = tm["step_ms"] / max(tm["n_step"], 1)
+    bpc = bytes_per_cell(w["q"], a.storage)
+    ms = secs / a.steps * 1e3
+    print(json.dumps({
+        "metric": "MLUPS of one GPU's j-tile (1/%d of 384x192x128) with its halo exchange" % ntile,
+        "value": round(cells * a.steps / secs / 1e6, 1), "unit": "MLUPS", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(ms, 5), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": a.storage, "data": "synthetic (the headline workload's inputs, tile j=%d..%d)" % (j0, j1),
+        "config": {"workload": "tile %dx%dx%d + 2 halo planes of %d bytes" % (nx, nt, nz, nb), "storage": a.storage,
+                   "parallelism": "single GPU (one tile of a j-decomposition)"},
+        "step_kernel_ms": round(kern_ms, 5), "step_kernel_GBps": round(bpc * cells / (kern_ms * 1e-3) / 1e9, 1),
+        "halo_ms_per_step": round(ms - kern_ms, 5), "finite": finite}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -277,7 +347,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
     ap.add_argument("--block", type=int, nargs=2, default=(0, 0))
-    ap.add_argument("--workload", default="headline", choices=["headline", "ensemble", "adjoint"])
+    ap.add_argument("--workload", default="headline", choices=["headline", "ensemble", "adjoint", "tile"])
+    ap.add_argument("--tiles", type=int, default=8)
     ap.add_argument("--members", type=int, default=64)
     a = ap.parse_args()
     if a.gpus != 1:
@@ -289,6 +360,9 @@ def main():
         return
     if a.workload == "adjoint":
         bench_adjoint(a)
+        return
+    if a.workload == "tile":
+        bench_tile(a)
         return
 
     from lbm_amd.domain import Domain

@@ -38,6 +38,10 @@ extern "C" {
 #define LBM_STORE_BF16S 3 /* (f - w_l) as bfloat16 */
 #define LBM_STORE_F64T 4  /* FP64 with a tangent: forward-mode derivative along lbm_set_tangent */
 
+/* Boundary code for jbnd only: the j ghost planes are halos that the caller fills between steps
+ * with lbm_halo_pack / lbm_halo_unpack (a domain tiled along j, one tile per GPU). */
+#define LBM_BND_HALO 3
+
 typedef struct lbm_ctx lbm_ctx;
 
 /* Run-time description of one domain.  Field meanings follow the reference's infile.in
@@ -49,7 +53,8 @@ typedef struct {
     int32_t ibgk;              /* 2 or 3: order of the Hermite equilibrium                    */
     int32_t ihrr;              /* 1: regularized non-equilibrium (Malaspinas recursion)       */
     int32_t ivreman;           /* 1: Vreman eddy viscosity                                    */
-    int32_t ibnd, jbnd, kbnd;  /* 0 periodic, 1 inflow/outflow (i only), 11/12/21/22 closed   */
+    int32_t ibnd, jbnd, kbnd;  /* 0 periodic, 1 inflow/outflow (i only), 11/12/21/22 closed,
+                                  LBM_BND_HALO (j only)                                       */
     int32_t iturb_pos;         /* inflow-turbulence slice (reference: 10)                     */
     int32_t store_tau;         /* 1: keep the per-cell tau of the Vreman closure              */
     int32_t block_x, block_y;  /* step-kernel block shape; 0 = default                        */
@@ -129,6 +134,24 @@ int lbm_step(lbm_ctx* ctx, int it0, int nsteps);
  * their own work after the library's. */
 int lbm_synchronize(lbm_ctx* ctx);
 void* lbm_stream(lbm_ctx* ctx);
+
+/* Halo exchange of a domain tiled along j (jbnd = LBM_BND_HALO).  Replaces
+ * mpi_halo_exchange_j (src/m_mpi_halo_exchange_j.F90:55-120) with its pack and unpack kernels
+ * (src/m_mpi_pack_jplane.F90, src/m_mpi_unpack_jplane.F90); the transfer between them is the
+ * caller's (RCCL send/recv, peer copies).  After each lbm_step of one step:
+ *   lbm_halo_pack(ctx, 0, s) copies plane j = 1 (for the neighbour below), side 1 plane j = ny
+ *   (for the neighbour above), into the device buffer s;
+ *   lbm_halo_unpack(ctx, 0, r) fills ghost plane j = 0 from the lower neighbour's side-1 pack,
+ *   side 1 ghost plane j = ny+1 from the upper neighbour's side-0 pack.
+ * side = 2 packs (unpacks) both in one launch: the buffer then holds side 0's halo followed by
+ * side 1's.  Both calls are enqueued on lbm_stream(ctx).  A halo holds only the populations
+ * crossing the plane (9 of 27, 5 of 19) over i = 0..nx+1, k = 0..nz+1, in the storage encoding:
+ * lbm_halo_bytes gives the size of one side's halo.  Unlike the reference's MPI path, the unpack also completes the closed face's
+ * copies whose source lies on the halo plane, so a tiled run reproduces the untiled periodic run
+ * bit for bit. */
+int lbm_halo_bytes(lbm_ctx* ctx, size_t* bytes);
+int lbm_halo_pack(lbm_ctx* ctx, int side, void* dev_buf);
+int lbm_halo_unpack(lbm_ctx* ctx, int side, const void* dev_buf);
 
 /* Kernel timing with HIP events on the context's stream.  on = 0: off; on = n >= 1: the step
  * kernel of every n-th step (the first after the call, then every n) and that step's group of

@@ -133,6 +133,7 @@ int lbm_adjoint_run(lbm_ctx* c, int it0, int nsteps, int every, const int32_t* b
     if (c->has_turb) return fail(c, LBM_EINVAL, "the adjoint does not differentiate the inflow-turbulence forcing: switch it off");
     if (c->avg_on) return fail(c, LBM_ESTATE, "switch time averaging off before an adjoint run");
     const int closed = (d.ibnd > 10) + (d.jbnd > 10) + (d.kbnd > 10);
+    if (c->d.jbnd == LBM_BND_HALO) return fail(c, LBM_EINVAL, "the adjoint takes untiled domains (no halo planes)");
     if (closed > 1 || (closed == 1 && c->fold_dir < 0))
         return fail(c, LBM_EINVAL, "the adjoint supports at most one closed axis (its face pass runs inside the step kernel)");
     if (nsteps < 1 || every < 1) return fail(c, LBM_EINVAL, "nsteps and every must be >= 1");

@@ -261,6 +261,7 @@ StepHost step_host(const lbm_ctx* c, bool fold) {
         opts_of(code, &h.fold_opt_lo, &h.fold_opt_hi);
     }
     h.heavy_first = h.fold_dir < 0 ? 0 : h.fold_dir == c->lay.ax1 ? 1 : h.fold_dir == c->lay.ax2 ? 2 : 0;
+    h.halo_dir = d.jbnd == LBM_BND_HALO ? 1 : -1;
     return h;
 }
 
@@ -321,8 +322,10 @@ int lbm_create(const lbm_desc* desc, lbm_ctx** out) {
     if (d.q != 19 && d.q != 27) return fail(nullptr, LBM_EINVAL, "q must be 19 or 27");
     if (d.ibgk != 2 && d.ibgk != 3) return fail(nullptr, LBM_EINVAL, "ibgk must be 2 or 3");
     auto okb = [](int b) { return b == 0 || b == 11 || b == 12 || b == 21 || b == 22; };
-    if (!(okb(d.ibnd) || d.ibnd == 1) || !okb(d.jbnd) || !okb(d.kbnd))
-        return fail(nullptr, LBM_EINVAL, "boundary codes: 0, 11, 12, 21, 22 (and 1 for ibnd)");
+    if (!(okb(d.ibnd) || d.ibnd == 1) || !(okb(d.jbnd) || d.jbnd == LBM_BND_HALO) || !okb(d.kbnd))
+        return fail(nullptr, LBM_EINVAL, "boundary codes: 0, 11, 12, 21, 22 (and 1 for ibnd, LBM_BND_HALO for jbnd)");
+    if (d.jbnd == LBM_BND_HALO && d.ibnd > 10 && d.kbnd > 10)
+        return fail(nullptr, LBM_EINVAL, "a halo domain takes at most one closed axis");
     if (d.ibnd == 1 && (d.iturb_pos < 1 || d.iturb_pos > d.nx) && d.iturb_pos != 0)
         return fail(nullptr, LBM_EINVAL, "iturb_pos outside 1..nx");
     const Ops* ops = nullptr;
@@ -374,8 +377,10 @@ int lbm_create(const lbm_desc* desc, lbm_ctx** out) {
             if (codes[a] > 10) { ++nclosed; cl = a; }
         if (nclosed == 1) c->fold_dir = cl;
         const char* fv = getenv("LBM_FOLD");
-        const bool fold = fv ? atoi(fv) != 0 : kFoldDefault;
+        // halo domains always fold: the separate boundary kernels do not know halo planes
+        const bool fold = d.jbnd == LBM_BND_HALO || (fv ? atoi(fv) != 0 : kFoldDefault);
         c->fold_on = c->fold_dir >= 0 && fold;
+        if (d.jbnd == LBM_BND_HALO) c->variant &= ~4;
         c->inflow_fold_on = d.ibnd == 1 && fold && (nclosed == 0 || c->fold_on) && !(c->variant & 4);
     }
     hipGetDevice(&c->device);
@@ -453,6 +458,8 @@ int lbm_init_equilibrium(lbm_ctx* c, const void* rho, const void* u, const void*
                          int it0) {
     if (!c || !rho || !u || !v || !w) return fail(c, LBM_EINVAL, "null argument");
     if (int rc = check_kind(c, realkind)) return rc;
+    if (c->d.jbnd == LBM_BND_HALO)
+        return fail(c, LBM_EINVAL, "a halo domain starts from lbm_set_state (its ghost planes are the neighbours')");
     const size_t nc = ncell_padded(c);
     const size_t cb = nc * realkind;
     if (int rc = dev_alloc(c, c->stage, 4 * cb)) return rc;
@@ -845,6 +852,36 @@ int lbm_synchronize(lbm_ctx* c) {
 }
 
 void* lbm_stream(lbm_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int lbm_halo_bytes(lbm_ctx* c, size_t* bytes) {
+    if (!c || !bytes) return fail(c, LBM_EINVAL, "null argument");
+    if (c->d.jbnd != LBM_BND_HALO) return fail(c, LBM_EINVAL, "not a halo domain (jbnd != LBM_BND_HALO)");
+    *bytes = (size_t)(c->d.q == 27 ? 9 : 5) * (c->d.nx + 2) * (c->d.nz + 2) * c->ops->raw_bytes;
+    return LBM_OK;
+}
+
+namespace {
+int halo_check(lbm_ctx* c, int side, const void* buf) {
+    if (!c || !buf) return fail(c, LBM_EINVAL, "null argument");
+    if (c->d.jbnd != LBM_BND_HALO) return fail(c, LBM_EINVAL, "not a halo domain (jbnd != LBM_BND_HALO)");
+    if (side < 0 || side > 2) return fail(c, LBM_EINVAL, "side must be 0, 1 or 2 (both)");
+    if (!c->have_state) return fail(c, LBM_ESTATE, "no state: call lbm_set_state first");
+    return LBM_OK;
+}
+}  // namespace
+
+int lbm_halo_pack(lbm_ctx* c, int side, void* dev_buf) {
+    if (int rc = halo_check(c, side, dev_buf)) return rc;
+    // the state of the last step (or the uploaded state, which both buffers hold)
+    HIPCHK(c, c->ops->halo_pack(c->lay, c->buf[c->cur], side, dev_buf, c->stream));
+    return LBM_OK;
+}
+
+int lbm_halo_unpack(lbm_ctx* c, int side, const void* dev_buf) {
+    if (int rc = halo_check(c, side, dev_buf)) return rc;
+    HIPCHK(c, c->ops->halo_unpack(c->lay, c->buf[c->cur], side, dev_buf, c->fold_on ? c->fold_dir : -1, c->stream));
+    return LBM_OK;
+}
 
 int lbm_set_timing(lbm_ctx* c, int on) {
     if (!c) return fail(c, LBM_EINVAL, "null context");

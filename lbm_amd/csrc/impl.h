@@ -51,6 +51,7 @@ struct StepHost {
     const void* shear;                        // uvel_shear(nz) (compute type)
     double utime, cosd, sind, rho0;           // inflow values of the step
     int heavy_first;                          // bit 0 / 1: cells 1 and n of ax1 / ax2 first
+    int halo_dir;                             // halo axis (-1: none)
     int bgk3, hrr, vre;
     int variant;      // step-kernel variant: bit 0 non-temporal stores, bit 2 two cells per lane
     int block_x, block_y;
@@ -121,6 +122,9 @@ struct Ops {
                              hipStream_t);
     hipError_t (*adj_face)(const Layout&, void* lam_y, void* lam_x, int dir, int opt_lo, int opt_hi, hipStream_t);
     hipError_t (*objective)(const Layout&, const void* f, const AdjBox&, double* acc, hipStream_t);
+    // halo planes of j (side 0: j = 1 / ghost 0, side 1: j = ny / ghost ny+1)
+    hipError_t (*halo_pack)(const Layout&, const void* f, int side, void* buf, hipStream_t);
+    hipError_t (*halo_unpack)(const Layout&, void* f, int side, const void* buf, int fold_dir, hipStream_t);
     int real_bytes;   // compute type size
     int raw_bytes;    // storage size
 };

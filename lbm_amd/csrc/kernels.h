@@ -60,6 +60,7 @@ struct StepArgs {
     const R* shear;        // uvel_shear(nz)
     R utime, cosd, sind, rho0;
     int heavy_first;       // bit 0 / bit 1: cells 1 and n of ax1 / ax2 are dispatched first
+    int halo_dir;          // axis whose ghost planes are halos filled by lbm_halo_unpack (-1: none)
 };
 
 // Signed sum  sx*a + sy*b + sz*c  for compile-time signs in {-1,0,1}: adds only.
@@ -362,7 +363,7 @@ __device__ __forceinline__ void fold_closed_face(const StepArgs<typename C::real
             f1[(size_t)m * L.plane + og] = f2[(size_t)l * L.plane + og];
             // push: the ghost this population streams to
             int t[3], s[3];
-            bool push = true, from_ghost = false;
+            bool push = true, from_inflow = false, from_halo = false;
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
                 if (d == D) { t[d] = ghost; s[d] = plane; continue; }
@@ -374,12 +375,19 @@ __device__ __forceinline__ void fold_closed_face(const StepArgs<typename C::real
                     s[d] = Layout::wrap(s[d], n[d], 1);
                 } else {
                     if (t[d] < 1 || t[d] > n[d]) push = false;
-                    if (s[d] < 1 || s[d] > n[d]) from_ghost = true;
+                    if (s[d] < 1 || s[d] > n[d]) {
+                        if (d == a.halo_dir) from_halo = true;
+                        else from_inflow = true;
+                    }
                 }
             }
             if (push) f1[(size_t)l * L.plane + L.cell(t[0], t[1], t[2])] = C::template enc_dep<Q>(dev[l], l);
-            // with the inflow planes folded too, the cell that rewrites that ghost copies it first
-            if (from_ghost && !a.inflow_fold)
+            // G - c_l outside the face's range.  On a halo plane only: the neighbour's value of this
+            // step, pushed by that halo's unpack.  On the inflow axis only: the ghost's value from
+            // before this step, copied by the cell that rewrites the ghost when the inflow planes are
+            // folded too, else here.  On both (the inflow ghost of a halo plane): the value that
+            // ghost holds now, the neighbour's inflow ghost from before this step.
+            if (from_inflow && (from_halo || !a.inflow_fold))
                 f1[(size_t)l * L.plane + og] = f1[(size_t)l * L.plane + L.cell(s[0], s[1], s[2])];
         }
     }
@@ -419,7 +427,8 @@ __device__ __forceinline__ void fold_inflow(const StepArgs<typename C::real>& a,
     if (D > 0) {
         const int x[3] = {i, j, k};
         const int n[3] = {L.nx, L.ny, L.nz};
-        const int O = 3 - D;   // the other tangential axis (periodic: only D is closed)
+        const int O = 3 - D;   // the other tangential axis: periodic or a halo axis
+        const int perO = O == 1 ? L.per_j : L.per_k;
 #pragma unroll
         for (int side = 0; side < 2; ++side) {
             const int plane = side ? n[D] : 1;
@@ -433,7 +442,9 @@ __device__ __forceinline__ void fold_inflow(const StepArgs<typename C::real>& a,
                 int t[3];
                 t[0] = i;
                 t[D] = side ? n[D] + 1 : 0;
-                t[O] = Layout::wrap(x[O] + Lt::c(l, O), n[O], 1);
+                t[O] = x[O] + Lt::c(l, O);
+                if (perO) t[O] = Layout::wrap(t[O], n[O], 1);
+                else if (t[O] < 1 || t[O] > n[O]) continue;   // a ghost of the halo axis
                 f1[(size_t)l * L.plane + L.cell(t[0], t[1], t[2])] = f1[(size_t)l * L.plane + og];
             }
         }
@@ -859,6 +870,103 @@ __global__ void bc_inflow_kernel(Layout L, void* f1, const typename C::real* she
     const int k = jfast ? y : x;
     if (j > L.ny || k > L.nz) return;
     inflow_cell<Q, C>(L, f1, shear, utime, cosd, sind, rho0, j, k);
+}
+
+// ------------------------------------------------------------------------------------------
+// Halo planes of j for a domain tiled along j (src/m_mpi_pack_jplane.F90,
+// src/m_mpi_unpack_jplane.F90, exchanged in src/m_mpi_halo_exchange_j.F90:55-120).  A halo holds
+// only the populations that cross the plane: pack side 0 sends plane j = 1's populations with
+// c_y = -1 (to the neighbour below), side 1 plane j = ny's with c_y = +1; unpack side 0 fills
+// ghost j = 0 with c_y = +1 populations, side 1 ghost j = ny+1 with c_y = -1.  Buffer order
+// [q][k = 0..nz+1][i = 0..nx+1], q counting those populations in lattice order (both sides: side
+// 0's halo, then side 1's); values in the storage encoding.
+// ------------------------------------------------------------------------------------------
+template <int Q>
+__host__ __device__ constexpr int halo_count() { return Q == 27 ? 9 : 5; }
+
+// the q-th population with c_y = cy
+template <int Q>
+__device__ __forceinline__ int halo_pop(int q, int cy) {
+    int n = 0;
+#pragma unroll
+    for (int l = 0; l < Q; ++l)
+        if (Lattice<Q>::cy(l) == cy) {
+            if (n == q) return l;
+            ++n;
+        }
+    return -1;
+}
+
+// One thread per buffer element e = ((h * count + q) * (nz+2) + k) * (nx+2) + i over the nside
+// sides side0 .. side0 + nside - 1 (h = side - side0); with both sides the buffer holds side 0's
+// halo and then side 1's.
+struct HaloIdx {
+    int i, k, q, h;
+};
+template <int Q>
+__device__ __forceinline__ bool halo_index(const Layout& L, int nside, HaloIdx& x) {
+    const unsigned e = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned ni = L.nx + 2, nk = L.nz + 2;
+    if (e >= (unsigned)nside * halo_count<Q>() * nk * ni) return false;
+    x.i = e % ni;
+    const unsigned r = e / ni;
+    x.k = r % nk;
+    const unsigned qh = r / nk;
+    x.q = qh % halo_count<Q>();
+    x.h = qh / halo_count<Q>();
+    return true;
+}
+
+template <int Q, class C>
+__global__ void __launch_bounds__(256) halo_pack_kernel(Layout L, const void* f, int side0, int nside, void* buf) {
+    using raw = typename C::raw;
+    HaloIdx x;
+    if (!halo_index<Q>(L, nside, x)) return;
+    const int side = side0 + x.h;
+    const int l = halo_pop<Q>(x.q, side ? 1 : -1);
+    const int j = side ? L.ny : 1;
+    static_cast<raw*>(buf)[blockIdx.x * blockDim.x + threadIdx.x] =
+        static_cast<const raw*>(f)[(size_t)l * L.plane + L.cell(x.i, j, x.k)];
+}
+
+// Unpack, and the closed-face pass's copies whose source lies on this halo plane: for ghost G of
+// the face of fold_dir and population l pointing out of the fluid, f(l, G) = f(l, G - c_l) on the
+// wall plane (src/m_boundary_k_closed_kernel.F90:39-41); when G - c_l is on the halo plane it is
+// the neighbour's value of this step, which only the halo brings.  Sources on a ghost of another
+// non-periodic axis (the inflow ghost of the halo plane) are left to the step kernel, which reads
+// them before this unpack replaces them.
+template <int Q, class C>
+__global__ void __launch_bounds__(256) halo_unpack_kernel(Layout L, void* f, int side0, int nside, const void* buf,
+                                                          int fold_dir) {
+    using raw = typename C::raw;
+    using Lt = Lattice<Q>;
+    HaloIdx hx;
+    if (!halo_index<Q>(L, nside, hx)) return;
+    const int i = hx.i, k = hx.k;
+    const int side = side0 + hx.h;
+    const int cy = side ? -1 : 1;
+    const int l = halo_pop<Q>(hx.q, cy);
+    const int j = side ? L.ny + 1 : 0;
+    raw* __restrict__ p = static_cast<raw*>(f);
+    const raw v = static_cast<const raw*>(buf)[blockIdx.x * blockDim.x + threadIdx.x];
+    p[(size_t)l * L.plane + L.cell(i, j, k)] = v;
+    const int D = fold_dir;
+    if (D != 0 && D != 2) return;
+    const int x[3] = {i, j, k};
+    const int n[3] = {L.nx, L.ny, L.nz};
+    const int O = 2 - D;   // the face's tangential axis other than j
+    if (x[O] < 1 || x[O] > n[O]) return;
+    const int cD = Lt::c(l, D);
+    const int side_d = x[D] == 1 && cD < 0 ? 0 : x[D] == n[D] && cD > 0 ? 1 : -1;
+    if (side_d < 0) return;
+    int t[3];
+    t[1] = j + cy;
+    t[D] = side_d ? n[D] + 1 : 0;
+    t[O] = x[O] + Lt::c(l, O);
+    const int perO = O == 0 ? L.per_i : L.per_k;
+    if (perO) t[O] = Layout::wrap(t[O], n[O], 1);
+    else if (t[O] < 1 || t[O] > n[O]) return;
+    p[(size_t)l * L.plane + L.cell(t[0], t[1], t[2])] = v;
 }
 
 // ------------------------------------------------------------------------------------------

@@ -107,6 +107,7 @@ struct Launch {
         a.sind = (R)h.sind;
         a.rho0 = (R)h.rho0;
         a.heavy_first = h.heavy_first;
+        a.halo_dir = h.halo_dir;
         return a;
     }
 
@@ -130,6 +131,25 @@ struct Launch {
     static hipError_t snapshot(const Layout& L, const void* f1, void* snap, const SnapList& sl, hipStream_t s) {
         if (sl.n == 0) return hipSuccess;
         hipLaunchKernelGGL((bc_snapshot_kernel<Q, C>), dim3(1), dim3(256), 0, s, L, f1, snap, sl);
+        return hipGetLastError();
+    }
+
+    // side 0 or 1, or 2: both (side 0's halo first in buf)
+    static dim3 halo_grid(const Layout& L, int nside) {
+        const long long n = (long long)nside * halo_count<Q>() * (L.nz + 2) * (L.nx + 2);
+        return dim3((unsigned)((n + 255) / 256));
+    }
+    static hipError_t halo_pack(const Layout& L, const void* f, int side, void* buf, hipStream_t s) {
+        const int nside = side == 2 ? 2 : 1;
+        hipLaunchKernelGGL((halo_pack_kernel<Q, C>), halo_grid(L, nside), dim3(256), 0, s, L, f, side == 2 ? 0 : side,
+                           nside, buf);
+        return hipGetLastError();
+    }
+
+    static hipError_t halo_unpack(const Layout& L, void* f, int side, const void* buf, int fold_dir, hipStream_t s) {
+        const int nside = side == 2 ? 2 : 1;
+        hipLaunchKernelGGL((halo_unpack_kernel<Q, C>), halo_grid(L, nside), dim3(256), 0, s, L, f,
+                           side == 2 ? 0 : side, nside, buf, fold_dir);
         return hipGetLastError();
     }
 
@@ -345,7 +365,7 @@ struct Launch {
         static const Ops o = {&step, &closed, &snapshot, &edges, &inflow, &aos_in, &state_out, &feq_init, &totals, &average,
                               sizeof(MemberTab<R>), &ens_pack, &ens_step,
                               kAdj ? &adj_step : nullptr, kAdj ? &adj_inflow : nullptr, kAdj ? &adj_face : nullptr,
-                              &objective, (int)sizeof(R), (int)sizeof(typename C::raw)};
+                              &objective, &halo_pack, &halo_unpack, (int)sizeof(R), (int)sizeof(typename C::raw)};
         return &o;
     }
 };

@@ -22,6 +22,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblbm_hip.so")
 
 STORAGE = {"f32": 0, "f64": 1, "f16s": 2, "bf16s": 3, "f64t": 4}
+BND_HALO = 3   # jbnd: ghost planes of j filled by halo_unpack (a domain tiled along j)
 
 EXPORTS = ["lbm_create", "lbm_destroy", "lbm_last_error", "lbm_set_state", "lbm_init_equilibrium",
            "lbm_get_state", "lbm_get_macros", "lbm_get_tau", "lbm_set_blanking", "lbm_set_ablvisc",
@@ -32,7 +33,8 @@ EXPORTS = ["lbm_create", "lbm_destroy", "lbm_last_error", "lbm_set_state", "lbm_
            "lbm_ensemble_synchronize", "lbm_ensemble_set_launch", "lbm_ensemble_get_launch",
            "lbm_ensemble_tune", "lbm_ensemble_set_timing", "lbm_ensemble_timing_read",
            "lbm_objective_run", "lbm_adjoint_run", "lbm_set_tangent", "lbm_objective_tangent",
-           "lbm_generate_turbulence", "lbm_set_averaging", "lbm_get_averages"]
+           "lbm_generate_turbulence", "lbm_set_averaging", "lbm_get_averages",
+           "lbm_halo_bytes", "lbm_halo_pack", "lbm_halo_unpack"]
 
 
 class Desc(C.Structure):
@@ -77,6 +79,9 @@ def load_library(path: str = LIB_PATH):
         "lbm_step": ([vp, i32, i32], i32),
         "lbm_synchronize": ([vp], i32),
         "lbm_stream": ([vp], vp),
+        "lbm_halo_bytes": ([vp, C.POINTER(C.c_size_t)], i32),
+        "lbm_halo_pack": ([vp, i32, vp], i32),
+        "lbm_halo_unpack": ([vp, i32, vp], i32),
         "lbm_set_timing": ([vp, i32], i32),
         "lbm_timing_read": ([vp, C.POINTER(dbl), C.POINTER(i32), C.POINTER(dbl), C.POINTER(i32)], i32),
         "lbm_get_totals": ([vp, C.POINTER(dbl), C.POINTER(dbl)], i32),
@@ -308,6 +313,24 @@ class Domain:
 
     def synchronize(self):
         self._chk(self.lib.lbm_synchronize(self.h), "lbm_synchronize")
+
+    def stream_ptr(self):
+        """The context's hipStream_t (as an integer), e.g. for torch.cuda.ExternalStream."""
+        return self.lib.lbm_stream(self.h) or 0
+
+    def halo_bytes(self):
+        """Size of one halo buffer (jbnd = BND_HALO domains)."""
+        n = C.c_size_t()
+        self._chk(self.lib.lbm_halo_bytes(self.h, C.byref(n)), "lbm_halo_bytes")
+        return n.value
+
+    def halo_pack(self, side, dev_ptr):
+        """Enqueue the pack of plane j = 1 (side 0) or j = ny (side 1) into device memory dev_ptr."""
+        self._chk(self.lib.lbm_halo_pack(self.h, int(side), C.c_void_p(dev_ptr)), "lbm_halo_pack")
+
+    def halo_unpack(self, side, dev_ptr):
+        """Enqueue the unpack of a neighbour's halo into ghost plane j = 0 (side 0) or ny+1 (side 1)."""
+        self._chk(self.lib.lbm_halo_unpack(self.h, int(side), C.c_void_p(dev_ptr)), "lbm_halo_unpack")
 
     def set_timing(self, on=True, every=1):
         """HIP-event timing of the step kernel (and boundary kernels) of every ``every``-th step."""
