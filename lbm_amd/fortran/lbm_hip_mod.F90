@@ -13,6 +13,7 @@ module lbm_hip
 
    integer(c_int), parameter, public :: LBM_STORE_F32 = 0, LBM_STORE_F64 = 1, &
                                         LBM_STORE_F16S = 2, LBM_STORE_BF16S = 3
+   integer(c_int), parameter, public :: LBM_BND_HALO = 3   ! jbnd of a domain tiled along j
 
    type, bind(C), public :: lbm_desc
       integer(c_int32_t) :: nx, ny, nz, q, storage
@@ -26,7 +27,7 @@ module lbm_hip
              lbm_init_equilibrium, lbm_set_blanking, lbm_set_ablvisc, lbm_set_inflow, &
              lbm_set_inflow_series, lbm_set_turbulence, lbm_set_external_forcing, lbm_step, &
              lbm_synchronize, lbm_get_totals, lbm_check, lbm_error_message, lbm_generate_turbulence, &
-             lbm_set_averaging, lbm_get_averages
+             lbm_set_averaging, lbm_get_averages, lbm_stream, lbm_halo_bytes, lbm_halo_pack, lbm_halo_unpack
 
    interface
       integer(c_int) function lbm_create(desc, ctx) bind(C, name='lbm_create')
@@ -112,6 +113,29 @@ module lbm_hip
          import :: c_int, c_ptr
          type(c_ptr), value :: ctx
          integer(c_int), value :: it0, nsteps
+      end function
+      ! halo planes of a domain tiled along j (jbnd = LBM_BND_HALO): after each one-step lbm_step,
+      ! pack (side 0: plane 1, side 1: plane ny, 2: both) into device memory, send/receive it
+      ! (GPU-aware MPI on the device buffers, or RCCL on lbm_stream), unpack into the ghost planes;
+      ! replaces mpi_halo_exchange_j (src/m_mpi_halo_exchange_j.F90:55-120)
+      type(c_ptr) function lbm_stream(ctx) bind(C, name='lbm_stream')
+         import :: c_ptr
+         type(c_ptr), value :: ctx
+      end function
+      integer(c_int) function lbm_halo_bytes(ctx, nbytes) bind(C, name='lbm_halo_bytes')
+         import :: c_int, c_ptr, c_size_t
+         type(c_ptr), value :: ctx
+         integer(c_size_t), intent(out) :: nbytes
+      end function
+      integer(c_int) function lbm_halo_pack(ctx, side, dev_buf) bind(C, name='lbm_halo_pack')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx, dev_buf
+         integer(c_int), value :: side
+      end function
+      integer(c_int) function lbm_halo_unpack(ctx, side, dev_buf) bind(C, name='lbm_halo_unpack')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx, dev_buf
+         integer(c_int), value :: side
       end function
       integer(c_int) function lbm_set_averaging(ctx, on) bind(C, name='lbm_set_averaging')
          import :: c_int, c_ptr
