@@ -138,6 +138,14 @@ def test_two_tiles_both_sides_per_launch(name):
     run_case(name, [(1, 6), (7, 13)], both=True)
 
 
+@pytest.mark.parametrize("name", ["inflow_closedk_solids_turb", "inflow_periodic_k"])
+def test_two_tiles_separate_inflow_pass(name, monkeypatch):
+    """The inflow/outflow planes written by their own kernel after the step (LBM_INFLOW_FOLD=0):
+    the face's copies from inflow ghosts are then made by the step kernel itself."""
+    monkeypatch.setenv("LBM_INFLOW_FOLD", "0")
+    run_case(name, [(1, 5), (6, 13)])
+
+
 def test_three_tiles_fp16():
     run_case("inflow_closedk_f16s", [(1, 4), (5, 9), (10, 13)])
 
