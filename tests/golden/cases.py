@@ -26,6 +26,12 @@ CASES = {
     "q27_regvre_inflow_cyl": (27, "sp", (60, 14, 5), 20, 2, "cylinder", 25,
                               dict(ibgk=3, ihrr=1, ivreman=1, smagorinsky=0.16733, ibnd=1, jbnd=0,
                                    kbnd=12, uini=0.05, udir=0.0, visc=1e-4, ampl=1e-3)),
+    # the same case over 100 steps: the moments' 1e-5 bar after 100 steps (nrturb above the
+    # step count: the reference regenerates the fields with FFTW every nrturb steps,
+    # src/main.F90:381-383, and FFTW is absent here, so the injected fields serve throughout)
+    "q27_regvre_inflow_cyl_100": (27, "sp", (60, 14, 5), 100, 2, "cylinder", 110,
+                                  dict(ibgk=3, ihrr=1, ivreman=1, smagorinsky=0.16733, ibnd=1, jbnd=0,
+                                       kbnd=12, uini=0.05, udir=0.0, visc=1e-4, ampl=1e-3)),
     # the same physics in the FP64 build (no cylinder: grid too small for it)
     "q27_regvre_inflow_dp": (27, "dp", (16, 8, 6), 20, 3, "none", 25,
                              dict(ibgk=3, ihrr=1, ivreman=1, smagorinsky=0.16733, ibnd=1, jbnd=0,

@@ -2,7 +2,7 @@
 
 Tolerance (FP32): the library evaluates the collision in closed form and with fused
 multiply-adds, so its rounding differs from the reference's dense Hermite contractions.  Over
-the 12-20 steps of a fixture the populations must agree to 2e-6 absolute (f is O(0.3)) and the
+the 12-100 steps of a fixture the populations must agree to 2e-6 absolute (f is O(0.3)) and the
 moments to a relative 1e-5 (density by max|rho|, velocities by the largest speed).  FP64: 1e-12.
 """
 import numpy as np
