@@ -342,7 +342,9 @@ def bench_tile(a):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: the workload's configuration: headline and tile 1000, "
+                         "ensemble 500, adjoint 400)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--storage", default="f32", choices=["f32", "f64", "f16s", "bf16s"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -353,6 +355,8 @@ def main():
     ap.add_argument("--tiles", type=int, default=8)
     ap.add_argument("--members", type=int, default=64)
     a = ap.parse_args()
+    if a.steps is None:
+        a.steps = {"ensemble": 500, "adjoint": 400}.get(a.workload, 1000)
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
                                    "described in DESIGN.md, not built)"}))

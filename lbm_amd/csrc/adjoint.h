@@ -32,12 +32,17 @@ namespace lbm {
 // Lg(l): adjoint of the post-collision population l; kappa: weight of the kinetic energy
 // 0.5*rho*|u|^2 of g in the objective (0: not in the objective box); coll = false: objective term
 // only.  The adjoint of g_l goes to sink(l, value); lF receives the adjoint of the cell's body
-// force.  The per-direction loops are rolled (the lattice constants are uniform table reads) so
-// that the cell's state stays small; Dg and Lg may be called more than once per direction.
+// force.  The per-direction loops are rolled or partly unrolled (the lattice constants of a
+// run-time direction are scalar shifts of packed constants) so that the cell's state stays small;
+// Dg and Lg may be called more than once per direction.
 template <int Q, bool BGK3, bool HRR, bool VRE, typename R, class DG, class LG, class Sink>
 __host__ __device__ inline void collide_adjoint_f(const StepArgs<R>& a, DG&& Dg, int k, int oc, bool solid, LG&& Lg,
                                                   R kappa, bool coll, Sink&& sink, R (&lF)[3]) {
     using Lt = Lattice<Q>;
+    // The loops that read the adjoint of the outputs run 4 directions per iteration in FP32, so
+    // that four loads are in flight together (reverse kernel 344 -> 325 us on 192x96x96 D3Q19);
+    // in FP64 that costs occupancy (134 -> 174 VGPRs) and is slower, so they stay rolled.
+    constexpr int kLoadUnroll = sizeof(R) == 4 ? 4 : 1;
     // ---- forward recompute (collide_cell's quantities, in plain rather than paired form)
     R dr = 0, jx = 0, jy = 0, jz = 0, pxx = 0, pyy = 0, pzz = 0, pxy = 0, pxz = 0, pyz = 0;
 #pragma unroll
@@ -102,7 +107,7 @@ __host__ __device__ inline void collide_adjoint_f(const StepArgs<R>& a, DG&& Dg,
         const R vx = ux - Fx * inv, vy = uy - Fy * inv, vz = uz - Fz * inv;
         const R vsq = vx * vx + vy * vy + vz * vz;
         R lvx = 0, lvy = 0, lvz = 0;
-#pragma unroll 1
+#pragma unroll kLoadUnroll
         for (int l = 0; l < Q; ++l) {
             const int cx = Lt::cx(l), cy = Lt::cy(l), cz = Lt::cz(l);
             const R w = weight<Q, R>(l);
@@ -130,7 +135,7 @@ __host__ __device__ inline void collide_adjoint_f(const StepArgs<R>& a, DG&& Dg,
     }
 
     // relaxation y = feq + fac * fneq
-#pragma unroll 1
+#pragma unroll kLoadUnroll
     for (int l = 0; l < Q; ++l) {
         const int cx = Lt::cx(l), cy = Lt::cy(l), cz = Lt::cz(l);
         const R fx = R(cx), fy = R(cy), fz = R(cz);

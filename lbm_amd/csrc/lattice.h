@@ -10,18 +10,26 @@
 
 namespace lbm {
 
-__host__ __device__ constexpr int cx27(int l) {
-    constexpr int t[27] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, -1, 1, 0, 0, -1, 1, 0, 0, -1, 1, -1, 1, 1, -1, -1, 1};
-    return t[l];
+// Each velocity component of the 27 directions, packed two bits per direction (c + 1) into one
+// 64-bit constant: a lookup with a run-time (wave-uniform) index is then a scalar shift and mask,
+// not a load from a constant table (the rolled direction loops of the adjoint kernel).
+namespace detail {
+struct Dir27 {
+    int t[27];
+};
+constexpr unsigned long long pack2(const Dir27& d) {
+    unsigned long long p = 0;
+    for (int l = 0; l < 27; ++l) p |= (unsigned long long)(d.t[l] + 1) << (2 * l);
+    return p;
 }
-__host__ __device__ constexpr int cy27(int l) {
-    constexpr int t[27] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 1, -1, 0, 0, -1, 1, 1, -1, -1, 1, 1, -1, 1, -1};
-    return t[l];
-}
-__host__ __device__ constexpr int cz27(int l) {
-    constexpr int t[27] = {0, 0, 0, 0, 0, -1, 1, 0, 0, 0, 0, -1, 1, 1, -1, 1, -1, 1, -1, 1, -1, -1, 1, -1, 1, -1, 1};
-    return t[l];
-}
+constexpr unsigned long long kCx = pack2({{0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, -1, 1, 0, 0, -1, 1, 0, 0, -1, 1, -1, 1, 1, -1, -1, 1}});
+constexpr unsigned long long kCy = pack2({{0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 1, -1, 0, 0, -1, 1, 1, -1, -1, 1, 1, -1, 1, -1}});
+constexpr unsigned long long kCz = pack2({{0, 0, 0, 0, 0, -1, 1, 0, 0, 0, 0, -1, 1, 1, -1, 1, -1, 1, -1, 1, -1, -1, 1, -1, 1, -1, 1}});
+}  // namespace detail
+
+__host__ __device__ constexpr int cx27(int l) { return (int)((detail::kCx >> (2 * l)) & 3ull) - 1; }
+__host__ __device__ constexpr int cy27(int l) { return (int)((detail::kCy >> (2 * l)) & 3ull) - 1; }
+__host__ __device__ constexpr int cz27(int l) { return (int)((detail::kCz >> (2 * l)) & 3ull) - 1; }
 
 template <int Q>
 struct Lattice {
@@ -48,8 +56,16 @@ struct Lattice {
             if (cx(m) == x && cy(m) == y && cz(m) == z) return m;
         return -1;
     }
-    __host__ __device__ static constexpr int opposite(int l) { return find(-cx(l), -cy(l), -cz(l)); }
+    // Opposite directions are the adjacent pairs (1,2), (3,4), ... (0-based), checked below.
+    __host__ __device__ static constexpr int opposite(int l) { return l == 0 ? 0 : ((l & 1) ? l + 1 : l - 1); }
+    static constexpr bool pairs_ok() {
+        for (int l = 0; l < Q; ++l)
+            if (find(-cx(l), -cy(l), -cz(l)) != opposite(l)) return false;
+        return true;
+    }
 };
+static_assert(Lattice<19>::pairs_ok() && Lattice<27>::pairs_ok(), "opposite directions are adjacent pairs");
+static_assert(cx27(0) == 0 && cx27(2) == -1 && cy27(3) == 1 && cz27(26) == 1 && cy27(26) == -1, "packed tables");
 
 // Reals of the compute type in the reference's rounding: w evaluated in that precision.
 template <int Q, typename R>
