@@ -199,16 +199,19 @@ int lbm_adjoint_run(lbm_ctx* c, int it0, int nsteps, int every, const int32_t* b
         }
         for (int seg = nck - 1; seg >= 0; --seg) {
             const int n0 = seg * every, n1 = std::min(n0 + every, nsteps);
-            if (int r = copy_dev(c, c->buf[0], ckp + (size_t)seg * 2 * bb, bb)) return r;
-            if (int r = copy_dev(c, c->buf[1], ckp + (size_t)seg * 2 * bb + bb, bb)) return r;
-            c->cur = ck_cur[seg];
-            c->pending = false;
-            c->use_ext = false;
-            // the inputs X(n0) .. X(n1-1) of the segment's steps
-            if (int r = copy_dev(c, snp, c->buf[c->cur], bb)) return r;
+            // the inputs X(n0) .. X(n1-1) of the segment's steps, slot m holding X(n0+m): slot 0 is
+            // the checkpoint's X(n0); step n0+m is written straight into slot m after that slot
+            // received the ghost shell of X(n0+m-2) (slot m-2, or the checkpoint's X(n0-1)), which
+            // is what the buffer receiving that step holds in the forward run's ping-pong
+            const char* ckx = ckp + (size_t)seg * 2 * bb;
+            const char* x_n0 = ckx + (size_t)ck_cur[seg] * bb;
+            const char* x_before = ckx + (size_t)(ck_cur[seg] ^ 1) * bb;
+            if (int r = copy_dev(c, snp, x_n0, bb)) return r;
             for (int n = n0 + 1; n < n1; ++n) {
-                if (int r = lbm_step(c, it0 + n - 1, 1)) return r;
-                if (int r = copy_dev(c, snp + (size_t)(n - n0) * bb, c->buf[c->cur], bb)) return r;
+                const int m = n - n0;
+                char* dst = snp + (size_t)m * bb;
+                HIPCHK(c, c->ops->ghost_copy(c->lay, m >= 2 ? snp + (size_t)(m - 2) * bb : x_before, dst, c->stream));
+                if (int r = step_buffers(c, snp + (size_t)(m - 1) * bb, dst, it0 + n)) return r;
             }
             for (int n = n1; n > n0; --n) {
                 const int bd = ck_cur[seg] ^ ((n - n0) & 1), bs = bd ^ 1;

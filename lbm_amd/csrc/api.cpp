@@ -265,6 +265,25 @@ StepHost step_host(const lbm_ctx* c, bool fold) {
     return h;
 }
 
+// One step from src into dst (both buffers of the context's layout), as lbm_step performs a step
+// after the first: dst must hold the ghosts of the state before src.  Used by the adjoint's
+// recompute, which writes each step into a snapshot slot of its own.  No turbulence forcing, no
+// averaging, no external moments (the adjoint's preconditions).
+int step_buffers(lbm_ctx* c, const void* src, void* dst, int it) {
+    StepHost h = step_host(c, c->fold_on);
+    if (c->inflow_fold_on) {
+        h.inflow_fold = 1;
+        h.shear = c->shear.p;
+        h.rho0 = c->d.rho0;
+        inflow_values(c, it, &h.utime, &h.cosd, &h.sind);
+    }
+    h.src = src;
+    h.dst = dst;
+    h.stream = 1;
+    HIPCHK(c, c->ops->step(c->lay, h, c->stream));
+    return boundary(c, dst, src, it, c->fold_on ? c->fold_dir : -1, c->inflow_fold_on);
+}
+
 // boundarycond(f1, f2) for the non-periodic directions (src/m_boundarycond.F90:166-365).
 // skip_dir: a closed axis whose face pass the step kernel has already performed.
 int boundary(lbm_ctx* c, void* f1, const void* f2, int it, int skip_dir, bool skip_inflow, bool timed) {

@@ -86,6 +86,9 @@ lbm::StepHost step_host(const lbm_ctx* c, bool fold);
 // skip_inflow: inflow/outflow planes done by the step kernel
 int boundary(lbm_ctx* c, void* f1, const void* f2, int it, int skip_dir = -1, bool skip_inflow = false,
              bool timed = false);
+// one step (iteration it) from buffer src into buffer dst, which holds the ghosts of the state
+// before src; the context's own buffers are not touched
+int step_buffers(lbm_ctx* c, const void* src, void* dst, int it);
 // single domains fold the closed face and the inflow planes into the step kernel unless LBM_FOLD=0
 // (measured on MI355X, D3Q27 384x192x128: one launch per step instead of three, +3 % for FP16
 // storage, +0.5 % for FP32)

@@ -125,6 +125,7 @@ struct Ops {
     // halo planes of j (side 0: j = 1 / ghost 0, side 1: j = ny / ghost ny+1)
     hipError_t (*halo_pack)(const Layout&, const void* f, int side, void* buf, hipStream_t);
     hipError_t (*halo_unpack)(const Layout&, void* f, int side, const void* buf, int fold_dir, hipStream_t);
+    hipError_t (*ghost_copy)(const Layout&, const void* src, void* dst, hipStream_t);
     int real_bytes;   // compute type size
     int raw_bytes;    // storage size
 };

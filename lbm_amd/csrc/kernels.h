@@ -777,6 +777,47 @@ __global__ void bc_closed_kernel(Layout L, void* f1, const void* f2, int opt_lo,
     }
 }
 
+// Ghost shell of a buffer (the ghost cells of the non-periodic axes, all populations; periodic
+// axes' ghosts are never read) copied from src to dst: a buffer about to receive a step's output
+// then holds the ghosts it would hold had it received the step before it (the adjoint's recompute
+// writes each step into a snapshot slot of its own instead of ping-ponging two buffers).
+// Work items in storage coordinates (a, b, c) along (ax0, ax1, ax2): blocks [0, nr1) copy the rows
+// b = 0, n1+1 (every c), blocks [nr1, nr1 + nr2) the rows c = 0, n2+1 (interior b), the remaining
+// blocks the two end cells a = 0, n0+1 of the interior rows; nr1, nr2, the end cells are zero for a
+// periodic axis.
+template <int Q, class C>
+__global__ void __launch_bounds__(256) ghost_copy_kernel(Layout L, const void* src, void* dst, int nr1, int nr2) {
+    using raw = typename C::raw;
+    const int n0 = L.n(L.ax0), n1 = L.n(L.ax1), n2 = L.n(L.ax2);
+    const raw* __restrict__ s = static_cast<const raw*>(src);
+    raw* __restrict__ d = static_cast<raw*>(dst);
+    auto copy = [&](int a, int b, int c) {
+        int i, j, k;
+        L.ijk(a, b, c, i, j, k);
+        const int o = L.cell(i, j, k);
+#pragma unroll
+        for (int l = 0; l < Q; ++l) d[(size_t)l * L.plane + o] = s[(size_t)l * L.plane + o];
+    };
+    const int r = blockIdx.x;
+    if (r < nr1 + nr2) {
+        int b, c;
+        if (r < nr1) {
+            b = (r & 1) ? n1 + 1 : 0;
+            c = r >> 1;
+        } else {
+            const int r2 = r - nr1;
+            c = (r2 & 1) ? n2 + 1 : 0;
+            b = (r2 >> 1) + 1;
+        }
+        for (int a = threadIdx.x; a <= n0 + 1; a += blockDim.x) copy(a, b, c);
+        return;
+    }
+    const int t = (r - nr1 - nr2) * blockDim.x + threadIdx.x;
+    if (t >= 2 * n1 * n2) return;
+    const int rest = t >> 1;
+    copy((t & 1) ? n0 + 1 : 0, rest % n1 + 1, rest / n1 + 1);
+}
+
 // Copy of the cells listed in sl (periodic axes wrapped) taken before the closed passes.
 template <int Q, class C>
 __global__ void bc_snapshot_kernel(Layout L, const void* f1, void* snapv, const SnapList sl) {

@@ -135,6 +135,18 @@ struct Launch {
     }
 
     // side 0 or 1, or 2: both (side 0's halo first in buf)
+    static hipError_t ghost_copy(const Layout& L, const void* src, void* dst, hipStream_t s) {
+        const int per[3] = {L.per_i, L.per_j, L.per_k};
+        const int n1 = L.n(L.ax1), n2 = L.n(L.ax2);
+        const int nr1 = per[L.ax1] ? 0 : 2 * (n2 + 2);
+        const int nr2 = per[L.ax2] ? 0 : 2 * n1;
+        const long long ne = per[L.ax0] ? 0 : 2LL * n1 * n2;
+        const long long blocks = nr1 + nr2 + (ne + 255) / 256;
+        if (blocks == 0) return hipSuccess;
+        hipLaunchKernelGGL((ghost_copy_kernel<Q, C>), dim3((unsigned)blocks), dim3(256), 0, s, L, src, dst, nr1, nr2);
+        return hipGetLastError();
+    }
+
     static dim3 halo_grid(const Layout& L, int nside) {
         const long long n = (long long)nside * halo_count<Q>() * (L.nz + 2) * (L.nx + 2);
         return dim3((unsigned)((n + 255) / 256));
@@ -365,7 +377,8 @@ struct Launch {
         static const Ops o = {&step, &closed, &snapshot, &edges, &inflow, &aos_in, &state_out, &feq_init, &totals, &average,
                               sizeof(MemberTab<R>), &ens_pack, &ens_step,
                               kAdj ? &adj_step : nullptr, kAdj ? &adj_inflow : nullptr, kAdj ? &adj_face : nullptr,
-                              &objective, &halo_pack, &halo_unpack, (int)sizeof(R), (int)sizeof(typename C::raw)};
+                              &objective, &halo_pack, &halo_unpack, &ghost_copy, (int)sizeof(R),
+                              (int)sizeof(typename C::raw)};
         return &o;
     }
 };
