@@ -177,7 +177,7 @@ def bench_ensemble(a):
                    "cells": cells, "members": a.members, "storage": a.storage, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "step_kernel_batch + inflow_kernel_batch (graph)", "bytes_per_cell": bpc_bytes,
+                     "kernel": "step_kernel_batch (graph; faces and inflow planes folded)", "bytes_per_cell": bpc_bytes,
                      "kernel_ms": round(step_ms, 5)},
         "tuned_launch": {"block": block, "blocks_per_cu": bpc, "ms_per_step": round(tune_ms, 5)},
         "per_member_launches": {"value": round(mlups_pm, 1), "unit": "MLUPS", "steps": n_pm,

@@ -35,10 +35,10 @@ struct lbm_ens {
     int block = 256;
     int bpc = 0;                   // persistent grid of bpc blocks per CU; 0: one block per tile
     // device tables and their host images
-    DevBuf tab, start, istart, imem, vals, clock, backup;
+    DevBuf tab, start, vals, clock, backup;
     std::vector<unsigned char> h_tab, h_vals;
-    std::vector<int> h_start, h_istart, h_imem;
-    int h_clock[2] = {0, 0};
+    std::vector<int> h_start;
+    int h_clock[3] = {0, 0, 0};
     EnsLaunch L{};
     // captured graphs by number of steps, valid for the launch in `key`
     std::map<int, hipGraphExec_t> graphs;
@@ -93,9 +93,7 @@ int build_tables(lbm_ens* e) {
     const size_t mb = e->ops->member_bytes;
     e->h_tab.assign(mb * n, 0);
     e->h_start.assign(n, 0);
-    e->h_istart.clear();
-    e->h_imem.clear();
-    long long total = 0, itotal = 0;
+    long long total = 0;
     for (int m = 0; m < n; ++m) {
         lbm_ctx* c = e->mem[m];
         if (c->avg_on) return efail(e, LBM_ESTATE, "member " + std::to_string(m) + ": time averaging is not batched; switch it off");
@@ -109,31 +107,20 @@ int build_tables(lbm_ens* e) {
         for (int q = 0; q < 3; ++q) mh.turb[q] = c->has_turb ? c->turb[q].p : nullptr;
         mh.shear = c->shear.p;
         mh.rho0 = c->d.rho0;
-        int tiles = 0, itiles = 0;
-        e->ops->ens_pack(mh, e->block, e->h_tab.data() + mb * m, &tiles, &itiles);
+        mh.inflow = c->d.ibnd == 1;
+        int tiles = 0;
+        e->ops->ens_pack(mh, e->block, e->h_tab.data() + mb * m, &tiles);
         e->h_start[m] = (int)total;
         total += tiles;
-        if (c->d.ibnd == 1) {
-            e->h_istart.push_back((int)itotal);
-            e->h_imem.push_back(m);
-            itotal += itiles;
-        }
     }
-    if (total >= (1LL << 31) || itotal >= (1LL << 31)) return efail(e, LBM_EINVAL, "ensemble too large for 32-bit tile indices");
+    if (total >= (1LL << 31)) return efail(e, LBM_EINVAL, "ensemble too large for 32-bit tile indices");
     bool moved = false;
-    const size_t ninf = e->h_imem.size();
     if (int rc = ealloc(e, e->tab, mb * n, &moved)) return rc;
     if (int rc = ealloc(e, e->start, sizeof(int) * n, &moved)) return rc;
-    if (int rc = ealloc(e, e->istart, sizeof(int) * std::max<size_t>(ninf, 1), &moved)) return rc;
-    if (int rc = ealloc(e, e->imem, sizeof(int) * std::max<size_t>(ninf, 1), &moved)) return rc;
-    if (int rc = ealloc(e, e->clock, sizeof(int) * 2, &moved)) return rc;
+    if (int rc = ealloc(e, e->clock, sizeof(e->h_clock), &moved)) return rc;
     if (int rc = ealloc(e, e->vals, (size_t)kChunk * n * 3 * e->ops->real_bytes, &moved)) return rc;
     EHIP(e, hipMemcpyAsync(e->tab.p, e->h_tab.data(), mb * n, hipMemcpyHostToDevice, e->stream));
     EHIP(e, hipMemcpyAsync(e->start.p, e->h_start.data(), sizeof(int) * n, hipMemcpyHostToDevice, e->stream));
-    if (ninf) {
-        EHIP(e, hipMemcpyAsync(e->istart.p, e->h_istart.data(), sizeof(int) * ninf, hipMemcpyHostToDevice, e->stream));
-        EHIP(e, hipMemcpyAsync(e->imem.p, e->h_imem.data(), sizeof(int) * ninf, hipMemcpyHostToDevice, e->stream));
-    }
     EnsLaunch& L = e->L;
     L.tab = e->tab.p;
     L.start = static_cast<const int*>(e->start.p);
@@ -141,20 +128,13 @@ int build_tables(lbm_ens* e) {
     L.total = (int)total;
     L.block = e->block;
     L.grid = e->bpc > 0 ? (int)std::min<long long>(total, (long long)e->bpc * e->cus) : (int)total;
-    L.istart = static_cast<const int*>(e->istart.p);
-    L.imem = static_cast<const int*>(e->imem.p);
-    L.ninf = (int)ninf;
-    L.itotal = (int)itotal;
-    L.iblock = 256;
-    L.igrid = (int)std::min<long long>(std::max<long long>(itotal, 1), 4LL * e->cus);
     L.vals = e->vals.p;
     L.clock = static_cast<int*>(e->clock.p);
     L.bgk3 = e->bgk3;
     L.hrr = e->hrr;
     L.vre = e->vre;
     std::vector<long long> key = {(long long)(size_t)L.tab, (long long)(size_t)L.start, L.nmem, L.total, L.block,
-                                  L.grid, (long long)(size_t)L.istart, (long long)(size_t)L.imem, L.ninf, L.itotal,
-                                  L.igrid, (long long)(size_t)L.vals, (long long)(size_t)L.clock};
+                                  L.grid, (long long)(size_t)L.vals, (long long)(size_t)L.clock};
     if (moved || key != e->key) {
         drop_graphs(e);
         e->key = key;
@@ -220,7 +200,8 @@ int batched(lbm_ens* e, int it0, int nsteps) {
             }
         EHIP(e, hipMemcpyAsync(e->vals.p, e->h_vals.data(), e->h_vals.size(), hipMemcpyHostToDevice, e->stream));
         e->h_clock[0] = it0 + done + 1;
-        e->h_clock[1] = done;
+        e->h_clock[1] = done;   // buffer parity: steps since the members' current buffers were taken
+        e->h_clock[2] = done;   // the uploaded table's first step
         EHIP(e, hipMemcpyAsync(e->clock.p, e->h_clock, sizeof(e->h_clock), hipMemcpyHostToDevice, e->stream));
         hipEvent_t t0 = nullptr;
         if (e->timing && (t0 = next_ev(e))) hipEventRecord(t0, e->stream);
@@ -307,7 +288,7 @@ int lbm_ensemble_destroy(lbm_ens* e) {
     if (!e) return LBM_OK;
     if (e->stream) hipStreamSynchronize(e->stream);
     drop_graphs(e);
-    for (DevBuf* b : {&e->tab, &e->start, &e->istart, &e->imem, &e->vals, &e->clock, &e->backup}) dev_free(*b);
+    for (DevBuf* b : {&e->tab, &e->start, &e->vals, &e->clock, &e->backup}) dev_free(*b);
     for (hipEvent_t x : e->ev) hipEventDestroy(x);
     for (hipEvent_t x : e->ev_mem)
         if (x) hipEventDestroy(x);

@@ -68,19 +68,17 @@ struct MemberHost {
     const void* turb[3];   // slice 0 of uu, vv, ww (nullptr: off)
     const void* shear;     // uvel_shear(nz)
     double rho0;
+    int inflow;            // inflow/outflow planes of i (folded into the batched step)
 };
 
-// One batched step: the step kernel over every member's tiles, the inflow kernel over the
-// inflow members' planes, the clock tick.
+// One batched step: the step kernel over every member's tiles (faces and inflow planes folded),
+// the clock tick.
 struct EnsLaunch {
     const void* tab;                 // device MemberTab<R>[nmem]
     const int* start;                // device tile starts [nmem]
     int nmem, total, block, grid;
-    const int* istart;               // device inflow tile starts [ninf]
-    const int* imem;                 // device inflow member indices [ninf]
-    int ninf, itotal, iblock, igrid;
-    const void* vals;                // device (utime, cos, sin)[steps][nmem]
-    int* clock;                      // device {it, steps since set}
+    const void* vals;                // device (utime, cos, sin)[steps of the table][nmem]
+    int* clock;                      // device {it, steps since set, table's first step}
     int bgk3, hrr, vre;
 };
 
@@ -113,7 +111,7 @@ struct Ops {
     hipError_t (*average)(const Layout&, const void* cur, int stream_interior, void* avg, hipStream_t);
     // ensembles
     size_t member_bytes;   // sizeof(MemberTab<R>)
-    void (*ens_pack)(const MemberHost&, int block, void* out, int* tiles, int* inflow_tiles);
+    void (*ens_pack)(const MemberHost&, int block, void* out, int* tiles);
     hipError_t (*ens_step)(const EnsLaunch&, hipStream_t);
     // adjoint (nullptr for the 16-bit storage codecs)
     hipError_t (*adj_step)(const Layout&, const StepHost&, const void* lam_y, void* lam_x, const AdjBox&, double kappa,

@@ -1012,11 +1012,13 @@ __global__ void __launch_bounds__(256) halo_unpack_kernel(Layout L, void* f, int
 
 // ------------------------------------------------------------------------------------------
 // Ensembles: many domains of one lattice, storage and collision model advanced by one launch of
-// a persistent kernel per step phase.  Every domain's cells are cut into tiles of blockDim.x
-// consecutive cells of one (ax0, ax1) plane; the tiles of all members form one list that the
-// blocks of the grid walk with stride gridDim.x.  Per-step values are read from a device clock
-// (clock[0] = it of the step being computed, clock[1] = steps done since the clock was set) so
-// that a captured graph of steps replays without host updates.
+// a persistent kernel per step.  Every domain's cells are cut into tiles of blockDim.x consecutive
+// cells of one (ax0, ax1) plane; the tiles of all members form one list that the blocks of the
+// grid walk with stride gridDim.x.  Per-step values are read from a device clock (clock[0] = it of
+// the step being computed, clock[1] = steps done since the clock was set, clock[2] = the value of
+// clock[1] at the first step of the uploaded table of inflow values) so that a captured graph of
+// steps replays without host updates.  Each member's closed face and inflow/outflow planes are
+// folded into the step (at most one closed axis per member).
 // ------------------------------------------------------------------------------------------
 template <typename R>
 struct MemberTab {
@@ -1026,9 +1028,6 @@ struct MemberTab {
     int nrturb, slice;      // turbulence slices of ny*nz reals; nrturb = 0: off
     const R* turb[3];       // slice 0 of uu, vv, ww
     int n0, tiles_plane;    // cells along ax0; tiles per (ax0, ax1) plane
-    const R* shear;         // inflow members: uvel_shear(nz)
-    R rho0;
-    int jfast;
 };
 
 // Member owning tile t: the last m with start[m] <= t (start[0] = 0, increasing).
@@ -1042,13 +1041,16 @@ __device__ __forceinline__ int owner(const int* __restrict__ start, int n, int t
     return lo;
 }
 
+// vals: (uvel_time(it), cos(udir), sin(udir)) per step of the table and member.
 template <int Q, class C, bool BGK3, bool HRR, bool VRE>
 __global__ void __launch_bounds__(1024) step_kernel_batch(const MemberTab<typename C::real>* __restrict__ tab,
                                                           const int* __restrict__ start, int nmem, int total,
+                                                          const typename C::real* __restrict__ vals,
                                                           const int* __restrict__ clock) {
     using R = typename C::real;
     const int it = clock[0];
     const int s = clock[1];
+    const int sv = s - clock[2];
     for (int t = blockIdx.x; t < total; t += gridDim.x) {
         const int m = owner(start, nmem, t);
         const MemberTab<R>& M = tab[m];
@@ -1056,6 +1058,12 @@ __global__ void __launch_bounds__(1024) step_kernel_batch(const MemberTab<typena
         const int p = (M.cur0 + s) & 1;
         a.src = M.buf[p];
         a.dst = M.buf[p ^ 1];
+        if (a.inflow_fold) {
+            const R* v = vals + ((size_t)sv * nmem + m) * 3;
+            a.utime = v[0];
+            a.cosd = v[1];
+            a.sind = v[2];
+        }
         if (M.nrturb > 0) {
             int lit = it % M.nrturb;             // src/m_inflow_turbulence_forcing.F90:45-46
             if (lit == 0) lit = M.nrturb;
@@ -1069,29 +1077,6 @@ __global__ void __launch_bounds__(1024) step_kernel_batch(const MemberTab<typena
         const int e = (local - cc * M.tiles_plane) * (int)blockDim.x + (int)threadIdx.x;
         const int cb = e / M.n0;
         step_cell<Q, C, BGK3, HRR, VRE, 1>(a, e - cb * M.n0 + 1, cb + 1, cc + 1);
-    }
-}
-
-// The inflow/outflow planes of the members listed in mem[0..ninf) (tiles from istart), on the
-// buffer the step just wrote.  vals holds (uvel_time(it), cos, sin) per step and member.
-template <int Q, class C>
-__global__ void inflow_kernel_batch(const MemberTab<typename C::real>* __restrict__ tab, const int* __restrict__ istart,
-                                    const int* __restrict__ mem, int ninf, int total, int nmem,
-                                    const typename C::real* __restrict__ vals, const int* __restrict__ clock) {
-    using R = typename C::real;
-    const int s = clock[1];
-    for (int t = blockIdx.x; t < total; t += gridDim.x) {
-        const int q = owner(istart, ninf, t);
-        const int m = mem[q];
-        const MemberTab<R>& M = tab[m];
-        const Layout& L = M.a.lay;
-        const int nf = M.jfast ? L.ny : L.nz;
-        const int e = (t - istart[q]) * (int)blockDim.x + (int)threadIdx.x;
-        const int x = e % nf + 1, y = e / nf + 1;
-        const int j = M.jfast ? x : y, k = M.jfast ? y : x;
-        if (j > L.ny || k > L.nz) continue;
-        const R* v = vals + ((size_t)s * nmem + m) * 3;
-        inflow_cell<Q, C>(L, M.buf[((M.cur0 + s) & 1) ^ 1], M.shear, v[0], v[1], v[2], M.rho0, j, k);
     }
 }
 

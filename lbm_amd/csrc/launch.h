@@ -260,7 +260,7 @@ struct Launch {
     }
 
     // ---- ensembles
-    static void ens_pack(const MemberHost& m, int block, void* out, int* tiles, int* inflow_tiles) {
+    static void ens_pack(const MemberHost& m, int block, void* out, int* tiles) {
         const Layout& L = m.lay;
         MemberTab<R> t{};
         t.a = args_of(L, m.h);
@@ -275,19 +275,19 @@ struct Launch {
         t.n0 = L.n(L.ax0);
         const long long plane = (long long)L.n(L.ax0) * L.n(L.ax1);
         t.tiles_plane = (int)((plane + block - 1) / block);
-        t.shear = static_cast<const R*>(m.shear);
-        t.rho0 = (R)m.rho0;
-        t.jfast = (L.ax0 == 1) || (L.ax0 == 0 && L.ax1 == 1);
+        t.a.inflow_fold = m.inflow;
+        t.a.shear = static_cast<const R*>(m.shear);
+        t.a.rho0 = (R)m.rho0;
         std::memcpy(out, &t, sizeof(t));
         *tiles = t.tiles_plane * L.n(L.ax2);
-        *inflow_tiles = (L.ny * L.nz + 255) / 256;
     }
 
     template <bool B3, bool H, bool V>
     static void ens_step_t(const EnsLaunch& e, hipStream_t s) {
         if constexpr (kNT)
             hipLaunchKernelGGL((step_kernel_batch<Q, C, B3, H, V>), dim3(e.grid), dim3(e.block), 0, s,
-                               static_cast<const MemberTab<R>*>(e.tab), e.start, e.nmem, e.total, e.clock);
+                               static_cast<const MemberTab<R>*>(e.tab), e.start, e.nmem, e.total,
+                               static_cast<const R*>(e.vals), e.clock);
     }
 
     static hipError_t ens_step(const EnsLaunch& e, hipStream_t s) {
@@ -302,10 +302,6 @@ struct Launch {
             case 6: ens_step_t<true, true, false>(e, s); break;
             default: ens_step_t<true, true, true>(e, s); break;
         }
-        if (e.ninf > 0)
-            hipLaunchKernelGGL((inflow_kernel_batch<Q, C>), dim3(e.igrid), dim3(e.iblock), 0, s,
-                               static_cast<const MemberTab<R>*>(e.tab), e.istart, e.imem, e.ninf, e.itotal, e.nmem,
-                               static_cast<const R*>(e.vals), e.clock);
         hipLaunchKernelGGL((clock_tick_kernel<Q, C>), dim3(1), dim3(64), 0, s, e.clock);
         return hipGetLastError();
     }

@@ -149,3 +149,37 @@ def test_invalid_members_are_rejected():
             Ensemble(bad)
     with pytest.raises(LBMError, match="no state"):
         Ensemble([d1]).step(0, 1)
+
+
+def test_more_steps_than_one_table_of_inflow_values():
+    """One call of more than 1024 steps: the per-step inflow values are uploaded in tables of 1024
+    steps, the buffer parity runs on across them.  Small inflow members (inflow planes folded into
+    the batched step), with a time series of the inflow speed that changes every step."""
+    members = [((12, 8, 6), dict(ibnd=1, jbnd=0, kbnd=12), dict(udir=10.0)),
+               ((10, 6, 8), dict(ibnd=1, jbnd=0, kbnd=0), dict()),
+               ((8, 8, 6), dict(ibnd=0, jbnd=0, kbnd=11), dict())]
+    nsteps = 1100
+    series = np.linspace(0.03, 0.05, nsteps + 2)
+    print(f"ensemble of {len(members)} members, {nsteps} steps in one call", flush=True)
+
+    def build():
+        doms = make(members=members)
+        for d, (_, bnd, ex) in zip(doms, members):
+            if bnd["ibnd"] == 1:
+                d.set_inflow_series(1, series, np.full(nsteps + 2, ex.get("udir", 0.0)))
+        return doms
+
+    a = build()
+    for d in a:
+        d.step(1, nsteps)
+    ref = states(a)
+    for d in a:
+        d.close()
+    b = build()
+    ens = Ensemble(b)
+    ens.step(1, nsteps)
+    ens.synchronize()
+    ens.close()
+    assert_same(states(b), ref, "1100 steps in one call")
+    for d in b:
+        d.close()
