@@ -273,20 +273,21 @@ def test_storage_axis_choices(codes):
     assert np.abs(rho - r0).max() <= 1e-5 and max(np.abs(u - u0).max(), np.abs(v - v0).max()) <= 1e-6
 
 
-@pytest.mark.parametrize("codes,storage", [((1, 0, 12), "f32"), ((0, 21, 0), "f32"), ((11, 0, 0), "f32"),
-                                           ((1, 0, 22), "f32"), ((1, 12, 0), "f32"), ((1, 0, 0), "f32"),
-                                           ((1, 21, 11), "f32"), ((1, 0, 12), "f16s"), ((1, 12, 0), "f64")])
-def test_folded_boundary_passes_are_bit_identical(codes, storage, monkeypatch):
+@pytest.mark.parametrize("codes,storage,q", [((1, 0, 12), "f32", 27), ((0, 21, 0), "f32", 27), ((11, 0, 0), "f32", 27),
+                                             ((1, 0, 22), "f32", 27), ((1, 12, 0), "f32", 27), ((1, 0, 0), "f32", 27),
+                                             ((1, 21, 11), "f32", 27), ((1, 0, 12), "f16s", 27), ((1, 12, 0), "f64", 27),
+                                             ((1, 0, 12), "f32", 19), ((0, 0, 21), "bf16s", 19), ((1, 11, 0), "f32", 19)])
+def test_folded_boundary_passes_are_bit_identical(codes, storage, q, monkeypatch):
     """With at most one closed axis the step kernel performs that axis's face pass and the
     inflow/outflow planes of i itself (ensembles always fold the face); the separate boundary
     kernels (LBM_FOLD=0) must give the same bits, with solids on the inflow and wall planes and
     inflow turbulence.  Two closed axes (edges) keep the separate kernels either way."""
-    q, dims, steps = 27, (20, 14, 12), 25
+    dims, steps = (20, 14, 12), 25
     o = dict(ibgk=3, ihrr=1, ivreman=1, smagorinsky=0.16733, kinevisc=2e-3, ibnd=codes[0], jbnd=codes[1], kbnd=codes[2])
     f0 = inputs.perturbed_uniform_flow(q, dims, seed=13, u0=(0.04, 0.01, -0.01), order=3)
     mask = inputs.box_mask(dims, (1, 5, 1), (3, 8, 3))
     tu = inputs.turbulence_fields(13, dims[1], dims[2], 10)
-    print(f"fold codes={codes} storage={storage}", flush=True)
+    print(f"fold codes={codes} storage={storage} q={q}", flush=True)
     out = []
     for fold in ("1", "0"):
         monkeypatch.setenv("LBM_FOLD", fold)
