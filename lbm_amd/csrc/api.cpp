@@ -356,7 +356,14 @@ int lbm_create(const lbm_desc* desc, lbm_ctx** out) {
         case LBM_STORE_F64T: ops = ops_f64t(d.q); break;
         default: return fail(nullptr, LBM_EINVAL, "unknown storage kind");
     }
-    const Layout lay = make_layout(d.nx, d.ny, d.nz, ops->raw_bytes, d.ibnd == 0, d.jbnd == 0, d.kbnd == 0);
+    // contiguous storage axis: the first periodic axis (i, else j, else k), else i; LBM_FAST_AXIS
+    // (0, 1, 2 for i, j, k) overrides it
+    int fast = -1;
+    if (const char* fa = getenv("LBM_FAST_AXIS")) {
+        fast = atoi(fa);
+        if (fast < 0 || fast > 2) return fail(nullptr, LBM_EINVAL, "LBM_FAST_AXIS must be 0, 1 or 2");
+    }
+    const Layout lay = make_layout(d.nx, d.ny, d.nz, ops->raw_bytes, d.ibnd == 0, d.jbnd == 0, d.kbnd == 0, fast);
     // the step kernel of 8-byte (and tangent) storage is compiled for at most 256 threads per block
     // (more registers per lane, no spills)
     const int max_threads = ops->raw_bytes >= 8 ? 256 : 1024;
