@@ -28,11 +28,11 @@ def tile_of(arr, j0, j1, ny, axis):
     return np.take(arr, idx, axis=axis)
 
 
-def make(dims, storage, codes, o, halo, f0, mask, tu, j0=1, j1=None):
+def make(dims, storage, codes, o, halo, f0, mask, tu, j0=1, j1=None, q=Q):
     nx, ny, nz = dims
     j1 = ny if j1 is None else j1
     sub = (nx, j1 - j0 + 1, nz)
-    dom = Domain(Q, sub, storage, ibnd=codes[0], jbnd=BND_HALO if halo else codes[1], kbnd=codes[2], **o)
+    dom = Domain(q, sub, storage, ibnd=codes[0], jbnd=BND_HALO if halo else codes[1], kbnd=codes[2], **o)
     dom.set_state(tile_of(f0, j0, j1, ny, 1) if halo else f0)
     dom.set_inflow(0.04, 10.0)
     if mask is not None:
@@ -82,25 +82,28 @@ CASES = {
     "periodic_i_closedk_bgk2": ((0, 0, 22), "f32", dict(ibgk=2, ihrr=0, ivreman=0)),
     "inflow_periodic_k": ((1, 0, 0), "f32", dict(ibgk=3, ihrr=1, ivreman=0)),
     "closed_i_periodic_k": ((11, 0, 0), "f64", dict(ibgk=3, ihrr=1, ivreman=1, smagorinsky=0.16733)),
+    "all_periodic": ((0, 0, 0), "f32", dict(ibgk=3, ihrr=1, ivreman=1, smagorinsky=0.16733)),
+    "q19_inflow_closedk": ((1, 0, 12), "f32", dict(ibgk=3, ihrr=1, ivreman=1, smagorinsky=0.16733, q=19)),
 }
 
 
 def run_case(name, splits, both=False):
     codes, storage, o = CASES[name]
     o = dict(o, kinevisc=2e-3)
+    q = o.pop("q", Q)
     dims, steps = (18, 13, 10), 20
     nx, ny, nz = dims
     # a start whose j ghost planes are the periodic copies, as the reference's records hold them
-    f0 = tile_of(inputs.perturbed_uniform_flow(Q, dims, seed=21, u0=(0.04, 0.012, -0.01), order=3), 1, ny, ny, 1)
+    f0 = tile_of(inputs.perturbed_uniform_flow(q, dims, seed=21, u0=(0.04, 0.012, -0.01), order=3), 1, ny, ny, 1)
     # solids next to the inflow plane and the lower wall, across the tile boundary j = 5|6
     mask = inputs.box_mask(dims, (1, 4, 1), (3, 7, 2))
     tu = inputs.turbulence_fields(21, ny, nz, 10)
     print(f"halo case {name} storage={storage} tiles={splits}", flush=True)
-    ref = make(dims, storage, codes, o, False, f0, mask, tu)
+    ref = make(dims, storage, codes, o, False, f0, mask, tu, q=q)
     ref.step(1, steps)
     fr = ref.get_state()
     ref.close()
-    tiles = [make(dims, storage, codes, o, True, f0, mask, tu, j0, j1) for j0, j1 in splits]
+    tiles = [make(dims, storage, codes, o, True, f0, mask, tu, j0, j1, q=q) for j0, j1 in splits]
     nb = tiles[0].halo_bytes() * (2 if both else 1)
     bufs = [[torch.empty(nb, dtype=torch.uint8, device="cuda") for _ in range(2)] for _ in tiles]
     for it in range(1, steps + 1):
@@ -115,7 +118,7 @@ def run_case(name, splits, both=False):
     if len(splits) == 1:
         # the record's ghost planes (the previous step's boundary pass): for a tile that is its own
         # neighbour, the crossing populations of its halos are the periodic run's ghost values
-        cys = cy_of(Q)
+        cys = cy_of(q)
         f = states[0]
         assert np.array_equal(f[:, 0][..., cys == 1], fr[:, 0][..., cys == 1])
         assert np.array_equal(f[:, -1][..., cys == -1], fr[:, -1][..., cys == -1])
