@@ -175,7 +175,8 @@ int lbm_get_averages(lbm_ctx* ctx, void* uave, void* vave, void* wave, void* ti,
 /* Global mass and momentum of the interior state in FP64 (accumulated on the device). */
 int lbm_get_totals(lbm_ctx* ctx, double* mass, double* momentum3);
 
-/* Bytes of HBM the context holds, and the device layout pitch (elements per row). */
+/* Bytes of HBM the two population buffers hold, the device layout pitch (elements per row) and
+ * the elements of one per-cell array (the padded grid in rows of that pitch). */
 int lbm_get_info(lbm_ctx* ctx, int64_t* device_bytes, int32_t* row_pitch, int64_t* plane_elems);
 
 /*

@@ -101,7 +101,7 @@ int lbm_set_tangent(lbm_ctx* c, const double* dforce, const double* dshear) {
     // the value parts stay; only the derivative parts are replaced
     if (c->has_force)
         for (int q = 0; q < 3; ++q) {
-            std::vector<Dual> img((size_t)L.plane);
+            std::vector<Dual> img((size_t)L.cplane);
             HIPCHK(c, hipMemcpyAsync(img.data(), c->force[q].p, img.size() * sizeof(Dual), hipMemcpyDeviceToHost, c->stream));
             HIPCHK(c, hipStreamSynchronize(c->stream));
             for (auto& x : img) x.d = 0.0;
@@ -109,7 +109,7 @@ int lbm_set_tangent(lbm_ctx* c, const double* dforce, const double* dshear) {
                 for (int k = 0; k <= L.nz + 1; ++k)
                     for (int j = 0; j <= L.ny + 1; ++j)
                         for (int i = 0; i <= L.nx + 1; ++i)
-                            img[(size_t)L.cell(i, j, k)].d = dforce[3 * (((size_t)k * (L.ny + 2) + j) * (L.nx + 2) + i) + q];
+                            img[(size_t)L.cidx(i, j, k)].d = dforce[3 * (((size_t)k * (L.ny + 2) + j) * (L.nx + 2) + i) + q];
             HIPCHK(c, hipMemcpyAsync(c->force[q].p, img.data(), img.size() * sizeof(Dual), hipMemcpyHostToDevice, c->stream));
             HIPCHK(c, hipStreamSynchronize(c->stream));
         }
@@ -142,7 +142,7 @@ int lbm_adjoint_run(lbm_ctx* c, int it0, int nsteps, int every, const int32_t* b
 
     const size_t bb = c->buf_bytes;
     const int nck = (nsteps + every - 1) / every;
-    const size_t plane = (size_t)c->lay.plane;
+    const size_t plane = (size_t)c->lay.cplane;   // per-cell arrays (force gradient)
     DevBuf ck, snaps, lam, fin, gF, gS, acc;
     auto cleanup = [&]() {
         for (DevBuf* x : {&ck, &snaps, &lam, &fin, &gF, &gS, &acc}) dev_free(*x);
@@ -259,7 +259,7 @@ int lbm_adjoint_run(lbm_ctx* c, int it0, int nsteps, int every, const int32_t* b
                 for (int k = 1; k <= d.nz; ++k)
                     for (int j = 1; j <= d.ny; ++j)
                         for (int i = 1; i <= d.nx; ++i) {
-                            const size_t o = c->lay.cell(i, j, k);
+                            const size_t o = c->lay.cidx(i, j, k);
                             double* g = grad_force + 3 * ((k * nyp + j) * nxp + i);
                             g[0] = hf[o];
                             g[1] = hf[plane + o];

@@ -310,11 +310,12 @@ __global__ void __launch_bounds__(256) adjoint_step_kernel(const StepArgs<typena
         // this cell's own slot of direction l is pulled by no interior cell: clear it
         if (unhit) lx[(size_t)l * L.plane + oc] = C::template encode<Q>(R(0), l);
     };
-    collide_adjoint_f<Q, BGK3, HRR, VRE, R>(a, Dg, k, oc, solid, Lg, inbox ? kappa : R(0), coll != 0, sink, lF);
+    const int ox = L.cidx(i, j, k);   // per-cell arrays (force, its gradient)
+    collide_adjoint_f<Q, BGK3, HRR, VRE, R>(a, Dg, k, ox, solid, Lg, inbox ? kappa : R(0), coll != 0, sink, lF);
     if (gF != nullptr && a.Fx != nullptr) {
-        gF[oc] += (double)lF[0];
-        gF[(size_t)L.plane + oc] += (double)lF[1];
-        gF[2 * (size_t)L.plane + oc] += (double)lF[2];
+        gF[ox] += (double)lF[0];
+        gF[(size_t)L.cplane + ox] += (double)lF[1];
+        gF[2 * (size_t)L.cplane + ox] += (double)lF[2];
     }
 }
 

@@ -34,8 +34,8 @@ struct Cell {
 
     Cell(bool force, bool solid_cell, double nu, double vconst) {
         lay = make_layout(3, 3, 3, sizeof(R), 0, 0, 0);
-        oc = lay.cell(1, 1, 1);
-        for (int c = 0; c < 3; ++c) F[c].assign((size_t)lay.plane, R(0));
+        oc = lay.cidx(1, 1, 1);
+        for (int c = 0; c < 3; ++c) F[c].assign((size_t)lay.cplane, R(0));
         a.lay = lay;
         a.stream = 1;
         a.kinevisc = (R)nu;

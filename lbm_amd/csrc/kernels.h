@@ -167,7 +167,7 @@ __host__ __device__ __forceinline__ void collide_cell(const StepArgs<typename C:
     if (a.avg != nullptr) {
         // time averaging of the moments after the previous step (src/m_averaging_full_kernel.F90:33-39)
         R* p = a.avg + oc;
-        const size_t P = (size_t)L.plane;
+        const size_t P = (size_t)L.cplane;
         p[0] += ux;
         p[P] += uy;
         p[2 * P] += uz;
@@ -537,7 +537,8 @@ __device__ __forceinline__ void step_cell(const StepArgs<typename C::real>& a, i
     constexpr bool BUF = false;
 #endif
     // descriptor byte count: an unsigned 32-bit field (the variant runs only below 4 GiB)
-    const int nbytes = (int)(uint32_t)((size_t)Q * pbytes < 0xffffffffull ? (size_t)Q * pbytes : 0xffffffffull);
+    const size_t tbytes = (size_t)L.total * sizeof(raw);
+    const int nbytes = (int)(uint32_t)(tbytes < 0xffffffffull ? tbytes : 0xffffffffull);
 #pragma unroll
     for (int l = 0; l < Q; ++l) {
         const uint32_t off = (uint32_t)(oz[Lt::cz(l) + 1] + oy[Lt::cy(l) + 1] + ox[Lt::cx(l) + 1]) * (uint32_t)sizeof(raw);
@@ -557,7 +558,8 @@ __device__ __forceinline__ void step_cell(const StepArgs<typename C::real>& a, i
     R out[Q];
     R tau = 0;
     const int oc = oc0;
-    collide_cell<Q, C, BGK3, HRR, VRE>(a, rv, i, j, k, ca, cb, cc, oc, out, tau);
+    const int occ = L.cidx(i, j, k);   // per-cell arrays
+    collide_cell<Q, C, BGK3, HRR, VRE>(a, rv, i, j, k, ca, cb, cc, occ, out, tau);
 
     char* __restrict__ dstb = static_cast<char*>(a.dst);
     const uint32_t ob = (uint32_t)oc * (uint32_t)sizeof(raw);
@@ -575,7 +577,7 @@ __device__ __forceinline__ void step_cell(const StepArgs<typename C::real>& a, i
         if constexpr ((VAR & 1) != 0) __builtin_nontemporal_store(C::template enc_dep<Q>(out[l], l), p);
         else *p = C::template enc_dep<Q>(out[l], l);
     }
-    if (VRE && a.tau_out != nullptr) a.tau_out[oc] = tau;
+    if (VRE && a.tau_out != nullptr) a.tau_out[occ] = tau;
     if (a.fold_dir >= 0) fold_closed_face<Q, C>(a, i, j, k, out);
     if (a.inflow_fold && (i == 1 || i == L.nx)) fold_inflow<Q, C>(a, i, j, k, out);
 }
@@ -680,10 +682,11 @@ __global__ void __launch_bounds__(256) step_kernel_pair(const StepArgs<typename 
         }
     }
     const int oc = L.cell(i, j, k);
+    const int ox = L.cidx(i, j, k);   // per-cell arrays (the pair is adjacent along ax0 there too)
     R outA[Q], outB[Q];
     R tauA = 0, tauB = 0;
-    collide_cell<Q, C, BGK3, HRR, VRE>(a, rA, i, j, k, ca, cb, cc, oc, outA, tauA);
-    collide_cell<Q, C, BGK3, HRR, VRE>(a, rB, i2, j2, k2, ca + 1, cb, cc, oc + 1, outB, tauB);
+    collide_cell<Q, C, BGK3, HRR, VRE>(a, rA, i, j, k, ca, cb, cc, ox, outA, tauA);
+    collide_cell<Q, C, BGK3, HRR, VRE>(a, rB, i2, j2, k2, ca + 1, cb, cc, ox + 1, outB, tauB);
     raw* __restrict__ dst = static_cast<raw*>(a.dst);
 #pragma unroll
     for (int l = 0; l < Q; ++l) {
@@ -693,8 +696,8 @@ __global__ void __launch_bounds__(256) step_kernel_pair(const StepArgs<typename 
         else *p = w;
     }
     if (VRE && a.tau_out != nullptr) {
-        a.tau_out[oc] = tauA;
-        a.tau_out[oc + 1] = tauB;
+        a.tau_out[ox] = tauA;
+        a.tau_out[ox + 1] = tauB;
     }
 }
 
@@ -1171,7 +1174,8 @@ __global__ void feq_init_kernel(Layout L, const H* rho, const H* u, const H* v, 
     for (int l = 0; l < Q; ++l)
         st<Q, C>(L, dst, l, oc, feq_dir<Q, BGK3, R>(l, r, ux, uy, uz, usq));
     if (i == i1 && j == j1 && k == k1) {
-        mr[oc] = r; mu[oc] = ux; mv[oc] = uy; mw[oc] = uz;
+        const int ox = L.cidx(i, j, k);
+        mr[ox] = r; mu[ox] = ux; mv[ox] = uy; mw[ox] = uz;
     }
 }
 
@@ -1202,8 +1206,8 @@ __global__ void average_kernel(Layout L, const void* cur, int stream_interior, t
     }
     const R inv = R(1) / (R(1) + dr);
     const R ux = jx * inv, uy = jy * inv, uz = jz * inv;
-    R* p = avg + L.cell(i, j, k);
-    const size_t P = (size_t)L.plane;
+    R* p = avg + L.cidx(i, j, k);
+    const size_t P = (size_t)L.cplane;
     p[0] += ux;
     p[P] += uy;
     p[2 * P] += uz;

@@ -58,7 +58,7 @@ struct Launch {
             }
         }
         // one 32-bit offset range covers every plane of the buffer
-        const bool small = (double)Q * (double)L.plane * sizeof(typename C::raw) < 4294967296.0;
+        const bool small = (double)L.total * sizeof(typename C::raw) < 4294967296.0;
         if constexpr (kNT) {
             if (nt && small) hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 9>), grid, block, 0, s, a);
             else if (nt) hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 1>), grid, block, 0, s, a);

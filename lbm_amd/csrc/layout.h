@@ -1,11 +1,17 @@
-// Device layout of one population buffer (structure of arrays).
+// Device layout of one population buffer.
 //
-// A buffer holds Q planes; plane l is the padded grid (0:nx+1, 0:ny+1, 0:nz+1) of population l.
-// One axis is stored contiguously (the "fast" axis); rows along it are padded so that interior
-// index 1 starts on a 128-byte boundary (storage column of index x is xoff + x with
-// xoff = 128/sizeof(raw) - 1) and the row pitch is a multiple of 128 bytes.  The two other axes
-// follow in natural order.  A 64-lane wave along the fast axis reads and writes whole cache
-// lines for populations with no velocity along that axis.
+// Cells of the padded grid (0:nx+1, 0:ny+1, 0:nz+1) lie in rows along one axis (the "fast" axis);
+// rows are padded so that interior index 1 starts on a 128-byte boundary (storage column of index
+// x is xoff + x with xoff = 128/sizeof(raw) - 1) and the row pitch is a multiple of 128 bytes.
+// The two other axes follow in natural order.  A 64-lane wave along the fast axis reads and
+// writes whole cache lines for populations with no velocity along that axis.
+//
+// Two arrangements of the Q populations, both addressed as  l * plane + cell(i, j, k):
+//   row-blocked (default when the buffer is below 4 GiB): [row][population][pitch] - plane is the
+//     row pitch and the row strides are Q times larger; the Q segments of a row are adjacent;
+//   planes: [population][row][pitch] - plane is the whole padded grid.
+// Per-cell arrays (tau, moments, forcing, averages) always use the plane arrangement of one
+// population: cidx(i, j, k), cplane elements.
 //
 // The fast axis is chosen per domain (make_layout): a periodic axis if there is one (i first),
 // so that the ghost planes the boundary kernels touch (closed faces, the inflow/outflow planes
@@ -27,13 +33,19 @@ struct Layout {
     int nx, ny, nz;
     int per_i, per_j, per_k;
     int ax0, ax1, ax2;      // storage order: ax0 contiguous, ax2 slowest (0 = i, 1 = j, 2 = k)
-    int si, sj, sk;         // element strides of i, j, k
+    int si, sj, sk;         // element strides of i, j, k in a population buffer
     int base;               // storage offset of cell (0,0,0)
     int pitch;              // elements per row along ax0
-    long long plane;        // elements per population plane (rounded to 256 B)
+    long long plane;        // element stride between populations (row pitch when row-blocked)
+    long long total;        // elements per population buffer (all Q populations)
+    int ci, cj, ck;         // element strides of i, j, k in per-cell arrays
+    long long cplane;       // elements per per-cell array (the padded grid, rounded to 256 B)
+    int rowblock;           // 1: row-blocked buffers
 
     __host__ __device__ inline int n(int a) const { return a == 0 ? nx : a == 1 ? ny : nz; }
     __host__ __device__ inline int cell(int i, int j, int k) const { return base + i * si + j * sj + k * sk; }
+    // index of cell (i, j, k) in a per-cell array
+    __host__ __device__ inline int cidx(int i, int j, int k) const { return base + i * ci + j * cj + k * ck; }
     __host__ __device__ static inline int wrap(int x, int n, int per) {
         return per ? (x == 0 ? n : (x == n + 1 ? 1 : x)) : x;
     }
@@ -49,8 +61,10 @@ struct Layout {
     }
 };
 
-// fast < 0: choose (first periodic axis, else i).
-inline Layout make_layout(int nx, int ny, int nz, int raw_bytes, int per_i, int per_j, int per_k, int fast = -1) {
+// fast < 0: choose (first periodic axis, else i).  q > 0 with rowblock: row-blocked buffers when a
+// buffer stays below 4 GiB (32-bit byte offsets across the whole buffer), planes otherwise.
+inline Layout make_layout(int nx, int ny, int nz, int raw_bytes, int per_i, int per_j, int per_k, int fast = -1,
+                          int q = 0, bool rowblock = false) {
     Layout L;
     L.nx = nx; L.ny = ny; L.nz = nz;
     L.per_i = per_i; L.per_j = per_j; L.per_k = per_k;
@@ -65,11 +79,23 @@ inline Layout make_layout(int nx, int ny, int nz, int raw_bytes, int per_i, int 
     st[L.ax0] = 1;
     st[L.ax1] = L.pitch;
     st[L.ax2] = L.pitch * (n1 + 2);
-    L.si = st[0]; L.sj = st[1]; L.sk = st[2];
+    L.ci = st[0]; L.cj = st[1]; L.ck = st[2];
     L.base = a - 1;
-    long long pl = (long long)L.pitch * (n1 + 2) * (n2 + 2);
-    long long ea = 256 / raw_bytes;
-    L.plane = (pl + ea - 1) / ea * ea;
+    const long long pl = (long long)L.pitch * (n1 + 2) * (n2 + 2);
+    const long long ea = 256 / raw_bytes;
+    L.cplane = (pl + ea - 1) / ea * ea;
+    const int nq = q > 0 ? q : 1;
+    L.rowblock = rowblock && q > 0 && (double)nq * (double)L.cplane * raw_bytes < 4294967296.0;
+    if (L.rowblock) {
+        st[L.ax1] = nq * L.pitch;
+        st[L.ax2] = nq * L.pitch * (n1 + 2);
+        L.plane = L.pitch;
+        L.total = (nq * pl + ea - 1) / ea * ea;
+    } else {
+        L.plane = L.cplane;
+        L.total = nq * L.cplane;
+    }
+    L.si = st[0]; L.sj = st[1]; L.sk = st[2];
     return L;
 }
 
