@@ -61,6 +61,7 @@ struct StepArgs {
     R utime, cosd, sind, rho0;
     int heavy_first;       // bit 0 / bit 1: cells 1 and n of ax1 / ax2 are dispatched first
     int halo_dir;          // axis whose ghost planes are halos filled by lbm_halo_unpack (-1: none)
+    int xcd_groups;        // 1: the blocks of one row group run on one XCD (step_kernel)
 };
 
 // Signed sum  sx*a + sy*b + sz*c  for compile-time signs in {-1,0,1}: adds only.
@@ -589,13 +590,34 @@ __device__ __forceinline__ void step_cell(const StepArgs<typename C::real>& a, i
 // launch's tail.
 __device__ __forceinline__ int heavy_order(int x, int n) { return x == 0 ? 1 : x == 1 ? n : x; }
 
+// With xcd_groups (several blocks along ax0 per row group), the blocks of one row group run on one
+// XCD: workgroups are dispatched to the 8 XCDs in turn, so hardware block b (x fastest) works on
+// row group ((b / 8) / gx) * 8 + b % 8, segment (b / 8) % gx.  The segments of a row group share the
+// cache lines at their edges (the +-1 shift along ax0), which another XCD's L2 would fetch again;
+// consecutive row groups still go to different XCDs.  Blocks past the last full set of 8 row
+// groups keep their own position.
 template <int Q, class C, bool BGK3, bool HRR, bool VRE, int VAR>
 __global__ void __launch_bounds__(sizeof(typename C::raw) >= 8 ? 256 : 1024, sizeof(typename C::raw) >= 8 ? 3 : 1) step_kernel(const StepArgs<typename C::real> a) {
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+    // (compiled out for the 16-byte tangent storage, which keeps the plain order: the order code
+    // there makes the compiler's register allocator fail)
+    if (sizeof(typename C::raw) <= 8 && a.xcd_groups) {
+        const int gx = gridDim.x, gy = gridDim.y;
+        const int b = bx + gx * (by + gy * bz);
+        const int rows = gy * gridDim.z;
+        if (b < 8 * gx * (rows / 8)) {
+            const int slot = b >> 3;
+            const int r = (slot / gx) * 8 + (b & 7);
+            bx = slot % gx;
+            by = r % gy;
+            bz = r / gy;
+        }
+    }
+    const int y = by * blockDim.y + threadIdx.y;
     const int n1 = a.lay.n(a.lay.ax1);
     const int cb = (a.heavy_first & 1) && y < n1 ? heavy_order(y, n1) : y + 1;
-    const int cc = (a.heavy_first & 2) ? heavy_order(blockIdx.z, a.lay.n(a.lay.ax2)) : blockIdx.z + 1;
-    step_cell<Q, C, BGK3, HRR, VRE, VAR>(a, blockIdx.x * blockDim.x + threadIdx.x + 1, cb, cc);
+    const int cc = (a.heavy_first & 2) ? heavy_order(bz, a.lay.n(a.lay.ax2)) : bz + 1;
+    step_cell<Q, C, BGK3, HRR, VRE, VAR>(a, bx * blockDim.x + threadIdx.x + 1, cb, cc);
 }
 
 // Two raw values packed in one aligned word: 32 bits for 16-bit storage, 64 bits for FP32.

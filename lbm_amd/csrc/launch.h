@@ -112,9 +112,11 @@ struct Launch {
     }
 
     static hipError_t step(const Layout& L, const StepHost& h, hipStream_t s) {
-        const StepArgs<R> a = args_of(L, h);
+        StepArgs<R> a = args_of(L, h);
         const dim3 block(h.block_x, h.block_y, 1);
         const dim3 grid((L.n(L.ax0) + h.block_x - 1) / h.block_x, (L.n(L.ax1) + h.block_y - 1) / h.block_y, L.n(L.ax2));
+        static const bool xcd = !(getenv("LBM_XCD_GROUPS") && atoi(getenv("LBM_XCD_GROUPS")) == 0);
+        a.xcd_groups = xcd && grid.x > 1;
         const int key = (h.bgk3 ? 4 : 0) | (h.hrr ? 2 : 0) | (h.vre ? 1 : 0);
         switch (key) {
             case 0: return step_t<false, false, false>(L, a, grid, block, h.variant, s);
