@@ -58,6 +58,10 @@ def build(verbose: bool = False) -> str:
     subprocess.run([HIPCC] + FLAGS + ["-x", "hip", "-o", os.path.join(BUILD, "adjoint_check"),
                                       os.path.join(CSRC, "adjoint_check.cpp")],
                    check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    # host-only check of the buffer layouts (layout_check.cpp)
+    subprocess.run([HIPCC] + FLAGS + ["-x", "hip", "-o", os.path.join(BUILD, "layout_check"),
+                                      os.path.join(CSRC, "layout_check.cpp")],
+                   check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs
     subprocess.run(cmd, check=True)
