@@ -272,9 +272,11 @@ def bench_adjoint(a):
 def bench_tile(a):
     """One GPU's share of the headline workload tiled along j (tile 4 of 8: j = 73..96, through the
     cylinder), with its halo exchange every step: pack both boundary planes, hand each buffer over
-    (a device-to-device copy on the context's stream where RCCL would send/recv), unpack."""
+    (a device-to-device copy on the context's stream where RCCL would send/recv), unpack.  With
+    --split each step is two launches (lbm_step_part), the pack after the first, and the transfer
+    runs on a stream of its own during the second."""
     import torch
-    from lbm_amd.domain import BND_HALO, Domain
+    from lbm_amd.domain import BND_HALO, PART_EDGE, PART_INNER, Domain
     w = workload(a.storage)
     nx, ny, nz = w["dims"]
     ntile = a.tiles
@@ -302,6 +304,8 @@ def bench_tile(a):
     snd = torch.empty(2 * nb, dtype=torch.uint8, device="cuda")   # side 0's halo, then side 1's
     rcv = torch.empty(2 * nb, dtype=torch.uint8, device="cuda")
     ext = torch.cuda.ExternalStream(dom.stream_ptr())
+    side = torch.cuda.Stream()
+    packed, moved = torch.cuda.Event(), torch.cuda.Event()
 
     def step(it):
         dom.step(it, 1)
@@ -310,6 +314,24 @@ def bench_tile(a):
             rcv[:nb].copy_(snd[nb:])
             rcv[nb:].copy_(snd[:nb])
         dom.halo_unpack(2, rcv.data_ptr())
+
+    def step_split(it):
+        # step it + 1 (as dom.step(it, 1)): edge rows, pack; the transfer on its own stream during
+        # the inner rows; unpack after both
+        dom.step_part(it + 1, PART_EDGE)
+        dom.halo_pack(2, snd.data_ptr())
+        packed.record(ext)
+        dom.step_part(it + 1, PART_INNER)
+        side.wait_event(packed)
+        with torch.cuda.stream(side):
+            rcv[:nb].copy_(snd[nb:])
+            rcv[nb:].copy_(snd[:nb])
+            moved.record(side)
+        ext.wait_event(moved)
+        dom.halo_unpack(2, rcv.data_ptr())
+
+    if a.split:
+        step = step_split
 
     for it in range(1, a.warmup + 1):
         step(it)
@@ -325,7 +347,8 @@ def bench_tile(a):
     finite = bool(np.isfinite(rho).all() and np.isfinite(u).all())
     dom.close()
     cells = nx * nt * nz
-    kern_ms = tm["step_ms"] / max(tm["n_step"], 1)
+    # split steps are not bracketed by the step timing
+    kern_ms = tm["step_ms"] / tm["n_step"] if tm["n_step"] else None
     bpc = bytes_per_cell(w["q"], a.storage)
     ms = secs / a.steps * 1e3
     print(json.dumps({
@@ -335,8 +358,10 @@ def bench_tile(a):
         "vs_baseline": None, "dtype": a.storage, "data": "synthetic (the headline workload's inputs, tile j=%d..%d)" % (j0, j1),
         "config": {"workload": "tile %dx%dx%d + 2 halo planes of %d bytes" % (nx, nt, nz, nb), "storage": a.storage,
                    "parallelism": "single GPU (one tile of a j-decomposition)"},
-        "step_kernel_ms": round(kern_ms, 5), "step_kernel_GBps": round(bpc * cells / (kern_ms * 1e-3) / 1e9, 1),
-        "halo_ms_per_step": round(ms - kern_ms, 5), "finite": finite}), flush=True)
+        "split_steps": bool(a.split),
+        "step_kernel_ms": round(kern_ms, 5) if kern_ms else None,
+        "step_kernel_GBps": round(bpc * cells / (kern_ms * 1e-3) / 1e9, 1) if kern_ms else None,
+        "halo_ms_per_step": round(ms - kern_ms, 5) if kern_ms else None, "finite": finite}), flush=True)
 
 
 def main():
@@ -353,6 +378,8 @@ def main():
     ap.add_argument("--block", type=int, nargs=2, default=(0, 0))
     ap.add_argument("--workload", default="headline", choices=["headline", "ensemble", "adjoint", "tile"])
     ap.add_argument("--tiles", type=int, default=8)
+    ap.add_argument("--split", action="store_true",
+                    help="tile workload: split steps (edge rows, pack, inner rows during the transfer)")
     ap.add_argument("--members", type=int, default=64)
     a = ap.parse_args()
     if a.steps is None:

@@ -153,6 +153,22 @@ int lbm_halo_bytes(lbm_ctx* ctx, size_t* bytes);
 int lbm_halo_pack(lbm_ctx* ctx, int side, void* dev_buf);
 int lbm_halo_unpack(lbm_ctx* ctx, int side, const void* dev_buf);
 
+/* Step it of a halo domain in two launches, so that the halo transfer can run during the
+ * second: LBM_PART_EDGE steps the cells with j = 1, 2, ny-1, ny (all of them when ny < 5), after
+ * which lbm_halo_pack packs this step's boundary planes; LBM_PART_INNER steps j = 3 .. ny-2 and
+ * completes the step.  The inner cells read no halo and write nothing the pack reads, so the
+ * caller's transfer (on its own stream, after an event recorded on lbm_stream following the pack)
+ * overlaps the inner launch; lbm_halo_unpack then follows the inner part on lbm_stream.  The pair
+ * gives lbm_step(ctx, it - 1, 1)'s result bit for bit.  Between the parts only lbm_halo_pack,
+ * lbm_halo_bytes, lbm_stream and lbm_synchronize may be called; lbm_step, lbm_step_part(EDGE),
+ * lbm_set_state, lbm_get_state and lbm_get_macros fail with LBM_ESTATE.  Needs the inflow/outflow
+ * planes folded into the step kernel (the default) and j off the contiguous axis.  The
+ * reference's MPI path exchanges between whole steps (src/main.F90:318,
+ * src/m_mpi_halo_exchange_j.F90:55-120); the split has no counterpart there. */
+#define LBM_PART_EDGE 1
+#define LBM_PART_INNER 2
+int lbm_step_part(lbm_ctx* ctx, int it, int part);
+
 /* Kernel timing with HIP events on the context's stream.  on = 0: off; on = n >= 1: the step
  * kernel of every n-th step (the first after the call, then every n) and that step's group of
  * boundary kernels, if it launches any, are bracketed by events (each event record costs the

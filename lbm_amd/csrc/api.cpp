@@ -323,6 +323,12 @@ int boundary(lbm_ctx* c, void* f1, const void* f2, int it, int skip_dir, bool sk
     return LBM_OK;
 }
 
+int check_no_part(lbm_ctx* c) {
+    if (c->part_open)
+        return fail(c, LBM_ESTATE, "a split step is open: finish it with lbm_step_part(ctx, it, LBM_PART_INNER)");
+    return LBM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -477,6 +483,7 @@ int lbm_destroy(lbm_ctx* c) {
 
 int lbm_set_state(lbm_ctx* c, const void* f, int realkind) {
     if (!c || !f) return fail(c, LBM_EINVAL, "null argument");
+    if (int rc = check_no_part(c)) return rc;
     if (int rc = check_kind(c, realkind)) return rc;
     const size_t n = ncell_padded(c) * c->d.q;
     if (int rc = dev_alloc(c, c->stage, n * realkind)) return rc;
@@ -524,42 +531,74 @@ int lbm_init_equilibrium(lbm_ctx* c, const void* rho, const void* u, const void*
     return LBM_OK;
 }
 
-int lbm_step(lbm_ctx* c, int it0, int nsteps) {
-    if (!c) return fail(c, LBM_EINVAL, "null context");
-    if (!c->have_state) return fail(c, LBM_ESTATE, "no state: call lbm_set_state or lbm_init_equilibrium first");
-    if (nsteps < 0) return fail(c, LBM_EINVAL, "nsteps < 0");
-    const lbm_desc& d = c->d;
+namespace {
+// The step arguments that stay the same from step to step.
+StepHost step_base(lbm_ctx* c) {
     StepHost h = step_host(c, c->fold_on);
     if (c->inflow_fold_on) {
         h.inflow_fold = 1;
         h.shear = c->shear.p;
-        h.rho0 = d.rho0;
+        h.rho0 = c->d.rho0;
         h.heavy_first |= c->lay.ax1 == 0 ? 1 : c->lay.ax2 == 0 ? 2 : 0;
     }
-    const size_t slice = (size_t)d.ny * d.nz * c->ops->real_bytes;
+    return h;
+}
+
+// Buffers and per-step inputs of step it.
+void step_inputs(lbm_ctx* c, StepHost& h, int it) {
+    const lbm_desc& d = c->d;
+    if (c->pending) {
+        // collide the uploaded state in place: read the copy in the other buffer
+        h.src = c->buf[c->cur ^ 1];
+        h.dst = c->buf[c->cur];
+        h.stream = 0;
+    } else {
+        h.src = c->buf[c->cur];
+        h.dst = c->buf[c->cur ^ 1];
+        h.stream = 1;
+    }
+    for (int q = 0; q < 4; ++q) h.ext[q] = c->use_ext ? c->ext[q].p : nullptr;
+    // the step kernel adds the moments of its input (the state after the previous step)
+    h.avg = (c->avg_on && c->avg_pending) ? c->avg.p : nullptr;
+    if (h.avg) ++c->avg_count;
+    if (c->has_turb) {
+        const size_t slice = (size_t)d.ny * d.nz * c->ops->real_bytes;
+        int lit = it % c->nrturb;           // src/m_inflow_turbulence_forcing.F90:45-46
+        if (lit == 0) lit = c->nrturb;
+        for (int q = 0; q < 3; ++q) h.turb[q] = static_cast<const char*>(c->turb[q].p) + (size_t)lit * slice;
+    } else {
+        h.turb[0] = h.turb[1] = h.turb[2] = nullptr;
+    }
+    if (h.inflow_fold) inflow_values(c, it, &h.utime, &h.cosd, &h.sind);
+}
+
+// After the step kernel: the new state is current.
+void step_advance(lbm_ctx* c) {
+    if (!c->pending) c->cur ^= 1;
+    c->pending = false;
+    c->use_ext = false;
+}
+
+// The boundary passes that the step kernel did not perform, and the bookkeeping of a finished step.
+int step_complete(lbm_ctx* c, int it, bool timed) {
+    if (int rc = boundary(c, c->buf[c->cur], c->buf[c->cur ^ 1], it, c->fold_on ? c->fold_dir : -1,
+                          c->inflow_fold_on, timed))
+        return rc;
+    c->ghost_raw[c->cur] = false;
+    c->avg_pending = c->avg_on;
+    return LBM_OK;
+}
+
+}  // namespace
+
+int lbm_step(lbm_ctx* c, int it0, int nsteps) {
+    if (!c) return fail(c, LBM_EINVAL, "null context");
+    if (!c->have_state) return fail(c, LBM_ESTATE, "no state: call lbm_set_state or lbm_init_equilibrium first");
+    if (nsteps < 0) return fail(c, LBM_EINVAL, "nsteps < 0");
+    if (int rc = check_no_part(c)) return rc;
+    StepHost h = step_base(c);
     for (int it = it0 + 1; it <= it0 + nsteps; ++it) {
-        if (c->pending) {
-            // collide the uploaded state in place: read the copy in the other buffer
-            h.src = c->buf[c->cur ^ 1];
-            h.dst = c->buf[c->cur];
-            h.stream = 0;
-        } else {
-            h.src = c->buf[c->cur];
-            h.dst = c->buf[c->cur ^ 1];
-            h.stream = 1;
-        }
-        for (int q = 0; q < 4; ++q) h.ext[q] = c->use_ext ? c->ext[q].p : nullptr;
-        // the step kernel adds the moments of its input (the state after the previous step)
-        h.avg = (c->avg_on && c->avg_pending) ? c->avg.p : nullptr;
-        if (h.avg) ++c->avg_count;
-        if (c->has_turb) {
-            int lit = it % c->nrturb;           // src/m_inflow_turbulence_forcing.F90:45-46
-            if (lit == 0) lit = c->nrturb;
-            for (int q = 0; q < 3; ++q) h.turb[q] = static_cast<const char*>(c->turb[q].p) + (size_t)lit * slice;
-        } else {
-            h.turb[0] = h.turb[1] = h.turb[2] = nullptr;
-        }
-        if (h.inflow_fold) inflow_values(c, it, &h.utime, &h.cosd, &h.sind);
+        step_inputs(c, h, it);
         // events on every timing_every-th step only: each record costs the stream a few us
         const bool timed = c->timing && c->timing_seq++ % c->timing_every == 0;
         hipEvent_t e0 = nullptr;
@@ -569,15 +608,44 @@ int lbm_step(lbm_ctx* c, int it0, int nsteps) {
             hipEvent_t e1 = next_event(c, c->ev_step, c->n_step_ev);
             if (e1) hipEventRecord(e1, c->stream);
         }
-        if (!c->pending) c->cur ^= 1;
-        c->pending = false;
-        c->use_ext = false;
-        if (int rc = boundary(c, c->buf[c->cur], c->buf[c->cur ^ 1], it, c->fold_on ? c->fold_dir : -1,
-                              c->inflow_fold_on, timed))
-            return rc;
-        c->ghost_raw[c->cur] = false;
-        c->avg_pending = c->avg_on;
+        step_advance(c);
+        if (int rc = step_complete(c, it, timed)) return rc;
     }
+    return LBM_OK;
+}
+
+int lbm_step_part(lbm_ctx* c, int it, int part) {
+    if (!c) return fail(c, LBM_EINVAL, "null context");
+    if (part != LBM_PART_EDGE && part != LBM_PART_INNER)
+        return fail(c, LBM_EINVAL, "part must be LBM_PART_EDGE or LBM_PART_INNER");
+    if (part == LBM_PART_INNER) {
+        if (!c->part_open) return fail(c, LBM_ESTATE, "no split step is open: run its LBM_PART_EDGE first");
+        if (it != c->part_it) return fail(c, LBM_EINVAL, "it differs from the open split step's");
+        c->part_open = false;
+        if (c->part_h.part) {
+            StepHost h = c->part_h;
+            h.part = LBM_PART_INNER;
+            HIPCHK(c, c->ops->step(c->lay, h, c->stream));
+        }
+        return step_complete(c, it, false);
+    }
+    if (!c->have_state) return fail(c, LBM_ESTATE, "no state: call lbm_set_state first");
+    if (int rc = check_no_part(c)) return rc;
+    if (c->d.jbnd != LBM_BND_HALO) return fail(c, LBM_EINVAL, "split steps need a halo domain (jbnd = LBM_BND_HALO)");
+    if (c->d.ibnd == 1 && !c->inflow_fold_on)
+        return fail(c, LBM_EINVAL, "split steps need the inflow/outflow planes in the step kernel (LBM_INFLOW_FOLD)");
+    const int axis = c->lay.ax1 == 1 ? 1 : c->lay.ax2 == 1 ? 2 : 0;
+    if (!axis) return fail(c, LBM_EINVAL, "split steps need j off the contiguous axis (LBM_FAST_AXIS)");
+    StepHost h = step_base(c);
+    step_inputs(c, h, it);
+    // tiles thinner than 5 cells along j: the first part is the whole step
+    h.part = c->d.ny >= 5 ? LBM_PART_EDGE : 0;
+    h.part_axis = axis;
+    HIPCHK(c, c->ops->step(c->lay, h, c->stream));
+    step_advance(c);
+    c->part_h = h;
+    c->part_it = it;
+    c->part_open = true;
     return LBM_OK;
 }
 
@@ -662,6 +730,7 @@ int lbm_get_averages(lbm_ctx* c, void* uave, void* vave, void* wave, void* ti, i
 
 int lbm_get_state(lbm_ctx* c, void* f, int realkind) {
     if (!c || !f) return fail(c, LBM_EINVAL, "null argument");
+    if (int rc = check_no_part(c)) return rc;
     if (!c->have_state) return fail(c, LBM_ESTATE, "no state");
     if (int rc = check_kind(c, realkind)) return rc;
     const size_t n = ncell_padded(c) * c->d.q;
@@ -677,6 +746,7 @@ int lbm_get_state(lbm_ctx* c, void* f, int realkind) {
 
 int lbm_get_macros(lbm_ctx* c, void* rho, void* u, void* v, void* w, int realkind) {
     if (!c || !rho || !u || !v || !w) return fail(c, LBM_EINVAL, "null argument");
+    if (int rc = check_no_part(c)) return rc;
     if (!c->have_state) return fail(c, LBM_ESTATE, "no state");
     if (int rc = check_kind(c, realkind)) return rc;
     const size_t nc = ncell_padded(c);

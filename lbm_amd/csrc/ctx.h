@@ -58,6 +58,10 @@ struct lbm_ctx {
                                    // step kernel (with fold_on, or with no closed axis)
     int variant = 1;               // step-kernel variant (LBM_STEP_VARIANT), see impl.h; non-temporal
                                    // stores measured fastest on MI355X (5.32 vs 5.11 TB/s)
+    // a split step (lbm_step_part) whose first part has run: its step number and arguments
+    bool part_open = false;
+    int part_it = 0;
+    lbm::StepHost part_h{};
     // timing
     bool timing = false;
     int timing_every = 1;          // events around every timing_every-th step

@@ -62,6 +62,10 @@ struct StepArgs {
     int heavy_first;       // bit 0 / bit 1: cells 1 and n of ax1 / ax2 are dispatched first
     int halo_dir;          // axis whose ghost planes are halos filled by lbm_halo_unpack (-1: none)
     int xcd_groups;        // 1: the blocks of one row group run on one XCD (step_kernel)
+    // split step along the halo axis (step_kernel): 0 every cell; 1 the cells within two of a halo
+    // plane (coordinates 1, 2, n-1, n; the launch covers 4 along part_axis); 2 the others (3..n-2,
+    // the launch covers n-4).  part_axis: 1 the halo axis is ax1, 2 it is ax2.
+    int part, part_axis;
 };
 
 // Signed sum  sx*a + sy*b + sz*c  for compile-time signs in {-1,0,1}: adds only.
@@ -613,7 +617,16 @@ __global__ void __launch_bounds__(sizeof(typename C::raw) >= 8 ? 256 : 1024, siz
             bz = r / gy;
         }
     }
-    const int y = by * blockDim.y + threadIdx.y;
+    int y = by * blockDim.y + threadIdx.y;
+    if (a.part) {
+        // launch coordinate c (0-based) along the halo axis -> the part's cell, or n (the cell
+        // n + 1, which step_cell skips) past the part's rows
+        const int n = a.lay.n(a.part_axis == 1 ? a.lay.ax1 : a.lay.ax2);
+        const int c = a.part_axis == 1 ? y : bz;
+        const int m = a.part == 1 ? (c < 2 ? c : c < 4 ? n - 4 + c : n) : (c < n - 4 ? c + 2 : n);
+        if (a.part_axis == 1) y = m;
+        else bz = m;
+    }
     const int n1 = a.lay.n(a.lay.ax1);
     const int cb = (a.heavy_first & 1) && y < n1 ? heavy_order(y, n1) : y + 1;
     const int cc = (a.heavy_first & 2) ? heavy_order(bz, a.lay.n(a.lay.ax2)) : bz + 1;

@@ -42,7 +42,7 @@ struct Launch {
     static hipError_t step_t(const Layout& L, const StepArgs<R>& a, dim3 grid, dim3 block, int var, hipStream_t s) {
         const bool nt = var & 1;
         if constexpr (sizeof(typename C::raw) <= 4) {
-            if ((var & 4) && a.fold_dir < 0 && !a.inflow_fold) {
+            if ((var & 4) && a.fold_dir < 0 && !a.inflow_fold && !a.part) {
                 // two cells per lane along the contiguous axis
                 const long long nt_slab = (long long)(L.n(L.ax0) / 2) * L.n(L.ax1);
                 const dim3 g((unsigned)((nt_slab + 255) / 256), L.n(L.ax2), 1);
@@ -108,13 +108,23 @@ struct Launch {
         a.rho0 = (R)h.rho0;
         a.heavy_first = h.heavy_first;
         a.halo_dir = h.halo_dir;
+        a.part = h.part;
+        a.part_axis = h.part_axis;
         return a;
     }
 
     static hipError_t step(const Layout& L, const StepHost& h, hipStream_t s) {
         StepArgs<R> a = args_of(L, h);
         const dim3 block(h.block_x, h.block_y, 1);
-        const dim3 grid((L.n(L.ax0) + h.block_x - 1) / h.block_x, (L.n(L.ax1) + h.block_y - 1) / h.block_y, L.n(L.ax2));
+        // a split step covers 4 (part 1) or n - 4 (part 2) along the halo axis (n >= 5)
+        int n1 = L.n(L.ax1), n2 = L.n(L.ax2);
+        if (h.part) {
+            int& n = h.part_axis == 1 ? n1 : n2;
+            n = h.part == 1 ? 4 : n - 4;
+            // the kernel's part mapping replaces the dispatch order along the halo axis
+            a.heavy_first &= h.part_axis == 1 ? ~1 : ~2;
+        }
+        const dim3 grid((L.n(L.ax0) + h.block_x - 1) / h.block_x, (n1 + h.block_y - 1) / h.block_y, n2);
         static const bool xcd = !(getenv("LBM_XCD_GROUPS") && atoi(getenv("LBM_XCD_GROUPS")) == 0);
         a.xcd_groups = xcd && grid.x > 1;
         const int key = (h.bgk3 ? 4 : 0) | (h.hrr ? 2 : 0) | (h.vre ? 1 : 0);

@@ -23,6 +23,7 @@ LIB_PATH = os.path.join(HERE, "liblbm_hip.so")
 
 STORAGE = {"f32": 0, "f64": 1, "f16s": 2, "bf16s": 3, "f64t": 4}
 BND_HALO = 3   # jbnd: ghost planes of j filled by halo_unpack (a domain tiled along j)
+PART_EDGE, PART_INNER = 1, 2   # step_part: the cells next to the halos, then the others
 
 EXPORTS = ["lbm_create", "lbm_destroy", "lbm_last_error", "lbm_set_state", "lbm_init_equilibrium",
            "lbm_get_state", "lbm_get_macros", "lbm_get_tau", "lbm_set_blanking", "lbm_set_ablvisc",
@@ -34,7 +35,7 @@ EXPORTS = ["lbm_create", "lbm_destroy", "lbm_last_error", "lbm_set_state", "lbm_
            "lbm_ensemble_tune", "lbm_ensemble_set_timing", "lbm_ensemble_timing_read",
            "lbm_objective_run", "lbm_adjoint_run", "lbm_set_tangent", "lbm_objective_tangent",
            "lbm_generate_turbulence", "lbm_set_averaging", "lbm_get_averages",
-           "lbm_halo_bytes", "lbm_halo_pack", "lbm_halo_unpack"]
+           "lbm_halo_bytes", "lbm_halo_pack", "lbm_halo_unpack", "lbm_step_part"]
 
 
 class Desc(C.Structure):
@@ -82,6 +83,7 @@ def load_library(path: str = LIB_PATH):
         "lbm_halo_bytes": ([vp, C.POINTER(C.c_size_t)], i32),
         "lbm_halo_pack": ([vp, i32, vp], i32),
         "lbm_halo_unpack": ([vp, i32, vp], i32),
+        "lbm_step_part": ([vp, i32, i32], i32),
         "lbm_set_timing": ([vp, i32], i32),
         "lbm_timing_read": ([vp, C.POINTER(dbl), C.POINTER(i32), C.POINTER(dbl), C.POINTER(i32)], i32),
         "lbm_get_totals": ([vp, C.POINTER(dbl), C.POINTER(dbl)], i32),
@@ -310,6 +312,11 @@ class Domain:
     # -- time stepping -------------------------------------------------------------------
     def step(self, it0, nsteps=1):
         self._chk(self.lib.lbm_step(self.h, int(it0), int(nsteps)), "lbm_step")
+
+    def step_part(self, it, part):
+        """Step ``it`` of a halo domain in two launches: PART_EDGE (j = 1, 2, ny-1, ny), after which
+        halo_pack may run, then PART_INNER (the other cells), which completes the step."""
+        self._chk(self.lib.lbm_step_part(self.h, int(it), int(part)), "lbm_step_part")
 
     def synchronize(self):
         self._chk(self.lib.lbm_synchronize(self.h), "lbm_synchronize")

@@ -14,6 +14,8 @@ module lbm_hip
    integer(c_int), parameter, public :: LBM_STORE_F32 = 0, LBM_STORE_F64 = 1, &
                                         LBM_STORE_F16S = 2, LBM_STORE_BF16S = 3
    integer(c_int), parameter, public :: LBM_BND_HALO = 3   ! jbnd of a domain tiled along j
+   ! lbm_step_part: the cells next to the halo planes (then lbm_halo_pack), then the others
+   integer(c_int), parameter, public :: LBM_PART_EDGE = 1, LBM_PART_INNER = 2
 
    type, bind(C), public :: lbm_desc
       integer(c_int32_t) :: nx, ny, nz, q, storage
@@ -27,7 +29,8 @@ module lbm_hip
              lbm_init_equilibrium, lbm_set_blanking, lbm_set_ablvisc, lbm_set_inflow, &
              lbm_set_inflow_series, lbm_set_turbulence, lbm_set_external_forcing, lbm_step, &
              lbm_synchronize, lbm_get_totals, lbm_check, lbm_error_message, lbm_generate_turbulence, &
-             lbm_set_averaging, lbm_get_averages, lbm_stream, lbm_halo_bytes, lbm_halo_pack, lbm_halo_unpack
+             lbm_set_averaging, lbm_get_averages, lbm_stream, lbm_halo_bytes, lbm_halo_pack, lbm_halo_unpack, &
+             lbm_step_part
 
    interface
       integer(c_int) function lbm_create(desc, ctx) bind(C, name='lbm_create')
@@ -136,6 +139,11 @@ module lbm_hip
          import :: c_int, c_ptr
          type(c_ptr), value :: ctx, dev_buf
          integer(c_int), value :: side
+      end function
+      integer(c_int) function lbm_step_part(ctx, it, part) bind(C, name='lbm_step_part')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: it, part
       end function
       integer(c_int) function lbm_set_averaging(ctx, on) bind(C, name='lbm_set_averaging')
          import :: c_int, c_ptr

@@ -3,14 +3,15 @@ lbm_halo_pack / lbm_halo_unpack, the way one process per GPU would run them with
 between the two calls (src/m_mpi_halo_exchange_j.F90:55-120).  On one device the exchange is a
 device buffer handed from one context to the other.  A periodic-j domain split into tiles must
 give the untiled run's state bit for bit: the interior, the ghost planes of i and k, and the
-crossing populations of the j halos.
+crossing populations of the j halos; so must split steps (lbm_step_part: the cells next to the
+halos, the pack, then the other cells while the transfer runs on another stream).
 """
 import numpy as np
 import pytest
 import torch
 
 from lbm_amd import inputs
-from lbm_amd.domain import BND_HALO, Domain
+from lbm_amd.domain import BND_HALO, PART_EDGE, PART_INNER, Domain
 from lbm_amd.lattice import lattice
 
 pytestmark = pytest.mark.gpu
@@ -76,6 +77,35 @@ def exchange_both(tiles, bufs):
         d.synchronize()
 
 
+def split_step(tiles, bufs, it, streams):
+    """Step it (what step(it - 1, 1) runs) as lbm_step_part's two launches with the transfer during
+    the inner one: per tile the
+    edge rows and the pack (both sides) on the tile's stream, an event; the inner rows; the
+    transfer on a stream of its own after every tile's event; each tile's unpack on its stream
+    after the transfer."""
+    n = len(tiles)
+    nb = bufs[0][0].numel() // 2
+    packed = []
+    for t, d in enumerate(tiles):
+        d.step_part(it, PART_EDGE)
+        d.halo_pack(2, bufs[t][0].data_ptr())
+        ev = torch.cuda.Event()
+        ev.record(streams[t])
+        packed.append(ev)
+        d.step_part(it, PART_INNER)
+    side = torch.cuda.current_stream()
+    for ev in packed:
+        side.wait_event(ev)
+    for t in range(n):
+        bufs[t][1][:nb].copy_(bufs[(t - 1) % n][0][nb:])   # ghost j = 0 <- plane ny of the tile below
+        bufs[t][1][nb:].copy_(bufs[(t + 1) % n][0][:nb])   # ghost j = ny+1 <- plane 1 of the tile above
+    moved = torch.cuda.Event()
+    moved.record(side)
+    for t, d in enumerate(tiles):
+        streams[t].wait_event(moved)
+        d.halo_unpack(2, bufs[t][1].data_ptr())
+
+
 CASES = {
     "inflow_closedk_solids_turb": ((1, 0, 12), "f32", dict(ibgk=3, ihrr=1, ivreman=1, smagorinsky=0.16733)),
     "inflow_closedk_f16s": ((1, 0, 21), "f16s", dict(ibgk=3, ihrr=1, ivreman=1, smagorinsky=0.16733)),
@@ -87,7 +117,7 @@ CASES = {
 }
 
 
-def run_case(name, splits, both=False):
+def run_case(name, splits, both=False, split=False):
     codes, storage, o = CASES[name]
     o = dict(o, kinevisc=2e-3)
     q = o.pop("q", Q)
@@ -104,12 +134,18 @@ def run_case(name, splits, both=False):
     fr = ref.get_state()
     ref.close()
     tiles = [make(dims, storage, codes, o, True, f0, mask, tu, j0, j1, q=q) for j0, j1 in splits]
-    nb = tiles[0].halo_bytes() * (2 if both else 1)
+    nb = tiles[0].halo_bytes() * (2 if both or split else 1)
     bufs = [[torch.empty(nb, dtype=torch.uint8, device="cuda") for _ in range(2)] for _ in tiles]
+    streams = [torch.cuda.ExternalStream(d.stream_ptr()) for d in tiles]
     for it in range(1, steps + 1):
+        if split:
+            split_step(tiles, bufs, it + 1, streams)   # the step that d.step(it, 1) runs
+            continue
         for d in tiles:
             d.step(it, 1)
         (exchange_both if both else exchange)(tiles, bufs)
+    for d in tiles:
+        d.synchronize()
     states = [d.get_state() for d in tiles]
     for d in tiles:
         d.close()
@@ -149,6 +185,15 @@ def test_two_tiles_separate_inflow_pass(name, monkeypatch):
     run_case(name, [(1, 5), (6, 13)])
 
 
+@pytest.mark.parametrize("splits", [[(1, 4), (5, 9), (10, 13)], [(1, 8), (9, 13)]], ids=["4-5-4", "8-5"])
+@pytest.mark.parametrize("name", list(CASES))
+def test_split_steps_match_untiled(name, splits):
+    """Edge rows, pack, inner rows with the transfer on another stream, unpack: the untiled run bit
+    for bit.  Tiles of 4 rows (one launch: the edge part is the whole tile), 5 (an inner part of
+    one row) and 8 (four inner rows)."""
+    run_case(name, splits, split=True)
+
+
 def test_three_tiles_fp16():
     run_case("inflow_closedk_f16s", [(1, 4), (5, 9), (10, 13)])
 
@@ -168,4 +213,30 @@ def test_halo_errors():
     with pytest.raises(RuntimeError, match="side must be"):
         d.set_state(inputs.perturbed_uniform_flow(Q, (12, 6, 6), seed=1, u0=(0.02, 0, 0), order=3))
         d.halo_unpack(3, buf.data_ptr())
+    d.close()
+
+
+def test_split_step_errors():
+    f = inputs.perturbed_uniform_flow(Q, (12, 6, 6), seed=3, u0=(0.02, 0, 0), order=3)
+    d = Domain(Q, (12, 6, 6), "f32", ibnd=1, jbnd=0, kbnd=12)
+    d.set_state(f)
+    with pytest.raises(RuntimeError, match="halo domain"):
+        d.step_part(1, PART_EDGE)
+    d.close()
+    d = Domain(Q, (12, 6, 6), "f32", ibnd=1, jbnd=BND_HALO, kbnd=12)
+    d.set_state(f)
+    with pytest.raises(RuntimeError, match="no split step is open"):
+        d.step_part(1, PART_INNER)
+    with pytest.raises(RuntimeError, match="part must be"):
+        d.step_part(1, 3)
+    d.step_part(1, PART_EDGE)
+    for call in (lambda: d.step(1, 1), lambda: d.step_part(1, PART_EDGE), d.get_state,
+                 lambda: d.set_state(f)):
+        with pytest.raises(RuntimeError, match="split step is open"):
+            call()
+    with pytest.raises(RuntimeError, match="differs"):
+        d.step_part(2, PART_INNER)
+    d.step_part(1, PART_INNER)
+    d.step(1, 1)
+    assert np.isfinite(d.get_state()).all()
     d.close()
