@@ -364,11 +364,58 @@ def bench_tile(a):
         "halo_ms_per_step": round(ms - kern_ms, 5) if kern_ms else None, "finite": finite}), flush=True)
 
 
+def bench_q19(a):
+    """The config-1 physics at the storage-study size: D3Q19 BGK (second-order equilibrium), tau = 0.6
+    (nu = 1/30), periodic i and j, no-slip k, 768x256x192, u = (0.05, 0, 0) + U(-1e-3, 1e-3) per
+    component (seed 1), no solids, no forcing."""
+    from lbm_amd import inputs
+    from lbm_amd.domain import Domain
+    dims = (768, 256, 192)
+    nx, ny, nz = dims
+    print(f"[bench] building inputs {dims} q=19 storage={a.storage}", file=sys.stderr, flush=True)
+    dt = np.float64 if a.storage == "f64" else np.float32
+    f0 = inputs.perturbed_uniform_flow(19, dims, seed=1, u0=(0.05, 0.0, 0.0), order=2, dtype=dt)
+    dom = Domain(19, dims, a.storage, ibgk=2, ihrr=0, ivreman=0, kinevisc=0.0333333333, ibnd=0, jbnd=0,
+                 kbnd=11, block=tuple(a.block))
+    dom.set_state(f0)
+    del f0
+    print(f"[bench] warmup {a.warmup}", file=sys.stderr, flush=True)
+    dom.step(1, a.warmup)
+    dom.synchronize()
+    dom.set_timing(True, every=10)
+    print(f"[bench] timed {a.steps} steps", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    dom.step(1 + a.warmup, a.steps)
+    dom.synchronize()
+    secs = time.perf_counter() - t0
+    tm = dom.timing_read()
+    rho, u, v, ww = dom.macrovars()
+    finite = bool(np.isfinite(rho).all() and np.isfinite(u).all())
+    dom.close()
+    cells = nx * ny * nz
+    kern_ms = tm["step_ms"] / max(tm["n_step"], 1)
+    bpc = bytes_per_cell(19, a.storage)
+    achieved = bpc * cells / (kern_ms * 1e-3) / 1e9
+    print(json.dumps({
+        "metric": "MLUPS + %HBM roofline (D3Q19 BGK)", "value": round(cells * a.steps / secs / 1e6, 1),
+        "unit": "MLUPS", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(secs / a.steps * 1e3, 5), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": a.storage, "data": "synthetic (seeded perturbed uniform flow)",
+        "config": {"workload": "D3Q19 BGK tau=0.6, 768x256x192, periodic i and j, no-slip k", "cells": cells,
+                   "storage": a.storage, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_from_profiles("step_kernel_q19_bgk_" + a.storage), "kernel": "step_kernel",
+                     "bytes_per_cell": bpc, "kernel_ms": round(kern_ms, 5),
+                     "bc_ms_per_step": round(tm["bc_ms"] / max(tm["n_bc"], 1), 5)},
+        "cpu_baseline": None, "finite": finite}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None,
-                    help="timed steps (default: the workload's configuration: headline and tile 1000, "
+                    help="timed steps (default: the workload's configuration: headline, tile and q19 1000, "
                          "ensemble 500, adjoint 400)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--storage", default="f32", choices=["f32", "f64", "f16s", "bf16s"])
@@ -376,7 +423,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
     ap.add_argument("--block", type=int, nargs=2, default=(0, 0))
-    ap.add_argument("--workload", default="headline", choices=["headline", "ensemble", "adjoint", "tile"])
+    ap.add_argument("--workload", default="headline", choices=["headline", "ensemble", "adjoint", "tile", "q19"])
     ap.add_argument("--tiles", type=int, default=8)
     ap.add_argument("--split", action="store_true",
                     help="tile workload: split steps (edge rows, pack, inner rows during the transfer)")
@@ -396,6 +443,9 @@ def main():
         return
     if a.workload == "tile":
         bench_tile(a)
+        return
+    if a.workload == "q19":
+        bench_q19(a)
         return
 
     from lbm_amd.domain import Domain
