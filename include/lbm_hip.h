@@ -159,9 +159,12 @@ int lbm_halo_unpack(lbm_ctx* ctx, int side, const void* dev_buf);
  * completes the step.  The inner cells read no halo and write nothing the pack reads, so the
  * caller's transfer (on its own stream, after an event recorded on lbm_stream following the pack)
  * overlaps the inner launch; lbm_halo_unpack then follows the inner part on lbm_stream.  The pair
- * gives lbm_step(ctx, it - 1, 1)'s result bit for bit.  Between the parts only lbm_halo_pack,
- * lbm_halo_bytes, lbm_stream and lbm_synchronize may be called; lbm_step, lbm_step_part(EDGE),
- * lbm_set_state, lbm_get_state and lbm_get_macros fail with LBM_ESTATE.  Needs the inflow/outflow
+ * gives lbm_step(ctx, it - 1, 1)'s result bit for bit.  Between the parts the calls that read or
+ * change the state or the step's inputs (lbm_step, lbm_step_part(EDGE), lbm_set_state,
+ * lbm_init_equilibrium, lbm_get_state, lbm_get_macros, lbm_get_tau, lbm_get_totals, lbm_set_* of
+ * inputs, lbm_generate_turbulence, averaging) fail with LBM_ESTATE: the inner part uses the
+ * inputs the edge part saw.  The halo calls, lbm_stream, lbm_synchronize, the timing calls and
+ * lbm_get_info remain available.  Needs the inflow/outflow
  * planes folded into the step kernel (the default) and j off the contiguous axis.  The
  * reference's MPI path exchanges between whole steps (src/main.F90:318,
  * src/m_mpi_halo_exchange_j.F90:55-120); the split has no counterpart there. */

@@ -501,6 +501,7 @@ int lbm_set_state(lbm_ctx* c, const void* f, int realkind) {
 int lbm_init_equilibrium(lbm_ctx* c, const void* rho, const void* u, const void* v, const void* w, int realkind,
                          int it0) {
     if (!c || !rho || !u || !v || !w) return fail(c, LBM_EINVAL, "null argument");
+    if (int rc = check_no_part(c)) return rc;
     if (int rc = check_kind(c, realkind)) return rc;
     if (c->d.jbnd == LBM_BND_HALO)
         return fail(c, LBM_EINVAL, "a halo domain starts from lbm_set_state (its ghost planes are the neighbours')");
@@ -662,6 +663,7 @@ int average_flush(lbm_ctx* c) {
 
 int lbm_set_averaging(lbm_ctx* c, int on) {
     if (!c) return fail(c, LBM_EINVAL, "null context");
+    if (int rc = check_no_part(c)) return rc;
     if (on) {
         const size_t n = 6 * (size_t)c->lay.cplane * c->ops->real_bytes;
         if (int rc = dev_alloc(c, c->avg, n)) return rc;
@@ -678,6 +680,7 @@ int lbm_set_averaging(lbm_ctx* c, int on) {
 
 int lbm_get_averages(lbm_ctx* c, void* uave, void* vave, void* wave, void* ti, int* count, int realkind) {
     if (!c) return fail(c, LBM_EINVAL, "null context");
+    if (int rc = check_no_part(c)) return rc;
     if (int rc = check_kind(c, realkind)) return rc;
     if (!c->avg.p) return fail(c, LBM_ESTATE, "no averages: call lbm_set_averaging first");
     if (int rc = average_flush(c)) return rc;
@@ -766,6 +769,7 @@ int lbm_get_macros(lbm_ctx* c, void* rho, void* u, void* v, void* w, int realkin
 
 int lbm_get_tau(lbm_ctx* c, void* tau, int realkind) {
     if (!c || !tau) return fail(c, LBM_EINVAL, "null argument");
+    if (int rc = check_no_part(c)) return rc;
     if (int rc = check_kind(c, realkind)) return rc;
     const Layout& L = c->lay;
     const size_t nc = ncell_padded(c);
@@ -791,6 +795,7 @@ int lbm_get_tau(lbm_ctx* c, void* tau, int realkind) {
 
 int lbm_set_blanking(lbm_ctx* c, const uint8_t* mask) {
     if (!c) return fail(c, LBM_EINVAL, "null context");
+    if (int rc = check_no_part(c)) return rc;
     if (!mask) {
         c->has_solid = false;
         return LBM_OK;
@@ -822,12 +827,14 @@ int lbm_set_blanking(lbm_ctx* c, const uint8_t* mask) {
 
 int lbm_set_ablvisc(lbm_ctx* c, const void* abl, int realkind) {
     if (!c || !abl) return fail(c, LBM_EINVAL, "null argument");
+    if (int rc = check_no_part(c)) return rc;
     if (int rc = check_kind(c, realkind)) return rc;
     return upload_reals(c, c->ablvisc, abl, c->d.nz, realkind);
 }
 
 int lbm_set_inflow(lbm_ctx* c, const void* shear, double uini, double udir, int realkind) {
     if (!c) return fail(c, LBM_EINVAL, "null context");
+    if (int rc = check_no_part(c)) return rc;
     if (int rc = check_kind(c, realkind)) return rc;
     c->uini = uini;
     c->udir = udir;
@@ -838,6 +845,7 @@ int lbm_set_inflow(lbm_ctx* c, const void* shear, double uini, double udir, int 
 
 int lbm_set_inflow_series(lbm_ctx* c, int it_lo, int it_hi, const void* ut, const void* ud, int realkind) {
     if (!c) return fail(c, LBM_EINVAL, "null context");
+    if (int rc = check_no_part(c)) return rc;
     if (int rc = check_kind(c, realkind)) return rc;
     c->ut_series.clear();
     c->ud_series.clear();
@@ -855,6 +863,7 @@ int lbm_set_inflow_series(lbm_ctx* c, int it_lo, int it_hi, const void* ut, cons
 int lbm_set_turbulence(lbm_ctx* c, const void* uu, const void* vv, const void* ww, int nrturb, double ampl,
                        int realkind) {
     if (!c) return fail(c, LBM_EINVAL, "null context");
+    if (int rc = check_no_part(c)) return rc;
     if (!uu) {
         c->has_turb = false;
         return LBM_OK;
@@ -875,6 +884,7 @@ int lbm_set_turbulence(lbm_ctx* c, const void* uu, const void* vv, const void* w
 int lbm_generate_turbulence(lbm_ctx* c, int nrturb, double ampl, uint64_t seed, void* uu, void* vv, void* ww, void* rr,
                             int realkind) {
     if (!c) return fail(c, LBM_EINVAL, "null context");
+    if (int rc = check_no_part(c)) return rc;
     if (nrturb < 1) return fail(c, LBM_EINVAL, "nrturb must be >= 1");
     if ((uu || vv || ww || rr) && check_kind(c, realkind)) return LBM_EINVAL;
     const int rb = c->ops->real_bytes;
@@ -925,6 +935,7 @@ int lbm_generate_turbulence(lbm_ctx* c, int nrturb, double ampl, uint64_t seed, 
 
 int lbm_set_external_forcing(lbm_ctx* c, const void* force, int realkind) {
     if (!c) return fail(c, LBM_EINVAL, "null context");
+    if (int rc = check_no_part(c)) return rc;
     if (!force) {
         c->has_force = false;
         return LBM_OK;
@@ -1024,6 +1035,7 @@ int lbm_timing_read(lbm_ctx* c, double* step_ms, int* n_step, double* bc_ms, int
 
 int lbm_get_totals(lbm_ctx* c, double* mass, double* mom) {
     if (!c) return fail(c, LBM_EINVAL, "null context");
+    if (int rc = check_no_part(c)) return rc;
     if (!c->have_state) return fail(c, LBM_ESTATE, "no state");
     const int nb = 1024;
     if (int rc = dev_alloc(c, c->partial, (size_t)nb * 4 * sizeof(double))) return rc;

@@ -230,8 +230,10 @@ def test_split_step_errors():
     with pytest.raises(RuntimeError, match="part must be"):
         d.step_part(1, 3)
     d.step_part(1, PART_EDGE)
+    mask = np.zeros((8, 8, 14), dtype=np.uint8)   # (nz+2, ny+2, nx+2)
     for call in (lambda: d.step(1, 1), lambda: d.step_part(1, PART_EDGE), d.get_state,
-                 lambda: d.set_state(f)):
+                 lambda: d.set_state(f), lambda: d.set_blanking(mask), lambda: d.set_inflow(0.02, 0.0),
+                 d.totals, d.macrovars):
         with pytest.raises(RuntimeError, match="split step is open"):
             call()
     with pytest.raises(RuntimeError, match="differs"):
