@@ -619,11 +619,9 @@ __global__ void __launch_bounds__(sizeof(typename C::raw) >= 8 ? 256 : 1024, siz
     }
     int y = by * blockDim.y + threadIdx.y;
     if (a.part) {
-        // launch coordinate c (0-based) along the halo axis -> the part's cell, or n (the cell
-        // n + 1, which step_cell skips) past the part's rows
+        // launch coordinate along the halo axis -> the part's cell (part_row)
         const int n = a.lay.n(a.part_axis == 1 ? a.lay.ax1 : a.lay.ax2);
-        const int c = a.part_axis == 1 ? y : bz;
-        const int m = a.part == 1 ? (c < 2 ? c : c < 4 ? n - 4 + c : n) : (c < n - 4 ? c + 2 : n);
+        const int m = part_row(a.part_axis == 1 ? y : bz, a.part, n);
         if (a.part_axis == 1) y = m;
         else bz = m;
     }

@@ -61,6 +61,14 @@ struct Layout {
     }
 };
 
+// Split steps along the halo axis (n >= 5 cells): launch coordinate c (0-based) of part 1, which
+// covers 4 launch rows (cells 1, 2, n-1, n), or of part 2, which covers n - 4 (cells 3 .. n-2) ->
+// the 0-based coordinate of its cell, or n (cell n + 1, which the step kernel skips) past the
+// part's rows.
+__host__ __device__ inline int part_row(int c, int part, int n) {
+    return part == 1 ? (c < 2 ? c : c < 4 ? n - 4 + c : n) : (c < n - 4 ? c + 2 : n);
+}
+
 // fast < 0: choose (first periodic axis, else i).  q > 0 with rowblock: row-blocked buffers when a
 // buffer stays below 4 GiB (32-bit byte offsets across the whole buffer), planes otherwise.
 inline Layout make_layout(int nx, int ny, int nz, int raw_bytes, int per_i, int per_j, int per_k, int fast = -1,

@@ -3,7 +3,9 @@
 //   * every (population, cell) of the padded grid has its own element, inside the buffer;
 //   * the first interior cell of every row and population starts on a 128-byte boundary;
 //   * per-cell indices are distinct and inside cplane;
-//   * row-blocked buffers are chosen only below 4 GiB, where 32-bit byte offsets span them.
+//   * row-blocked buffers are chosen only below 4 GiB, where 32-bit byte offsets span them;
+//   * the two launches of a split step (part_row) cover every cell of the halo axis exactly once
+//     and map every other launch coordinate past the last cell, for any block height.
 // Prints one JSON line per case; exits non-zero on the first violation.
 #include <cstdio>
 #include <cstdlib>
@@ -61,7 +63,29 @@ int main() {
     const Layout big = make_layout(768, 256, 192, 4, 0, 1, 0, -1, 27, true);
     const Layout mid = make_layout(384, 192, 128, 4, 0, 1, 0, -1, 27, true);
     if (big.rowblock || !mid.rowblock) return 5;
-    std::printf("{\"cases\": %d, \"rowblock_768x256x192_f32\": %d, \"rowblock_384x192x128_f32\": %d}\n", n,
-                big.rowblock, mid.rowblock);
+    // split steps: launch heights rounded up to the block height, as the launcher's grid is
+    int nsplit = 0;
+    for (int len = 5; len <= 1024; ++len)
+        for (int by : {1, 2, 4, 8}) {
+            std::vector<int> cnt(len, 0);
+            for (int part = 1; part <= 2; ++part) {
+                const int rows = part == 1 ? 4 : len - 4;
+                const int launched = (rows + by - 1) / by * by;
+                for (int c = 0; c < launched; ++c) {
+                    const int m = part_row(c, part, len);
+                    if (m == len) {
+                        if (c < rows) return 6;   // a coordinate of the part mapped outside
+                        continue;
+                    }
+                    if (m < 0 || m > len || c >= rows) return 7;
+                    ++cnt[m];
+                }
+            }
+            for (int x = 0; x < len; ++x)
+                if (cnt[x] != 1) return 8;
+            ++nsplit;
+        }
+    std::printf("{\"cases\": %d, \"rowblock_768x256x192_f32\": %d, \"rowblock_384x192x128_f32\": %d, "
+                "\"split_heights\": %d}\n", n, big.rowblock, mid.rowblock, nsplit);
     return 0;
 }
