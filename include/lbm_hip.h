@@ -154,9 +154,9 @@ int lbm_halo_pack(lbm_ctx* ctx, int side, void* dev_buf);
 int lbm_halo_unpack(lbm_ctx* ctx, int side, const void* dev_buf);
 
 /* Step it of a halo domain in two launches, so that the halo transfer can run during the
- * second: LBM_PART_EDGE steps the cells with j = 1, 2, ny-1, ny (all of them when ny < 5), after
- * which lbm_halo_pack packs this step's boundary planes; LBM_PART_INNER steps j = 3 .. ny-2 and
- * completes the step.  The inner cells read no halo and write nothing the pack reads, so the
+ * second: LBM_PART_EDGE steps the cells with j = 1, 2, ny-1, ny (j = 1, ny when no closed face is
+ * folded into the step; all of them when no inner cell is left), after which lbm_halo_pack packs
+ * this step's boundary planes; LBM_PART_INNER steps the other cells and completes the step.  The inner cells read no halo and write nothing the pack reads, so the
  * caller's transfer (on its own stream, after an event recorded on lbm_stream following the pack)
  * overlaps the inner launch; lbm_halo_unpack then follows the inner part on lbm_stream.  The pair
  * gives lbm_step(ctx, it - 1, 1)'s result bit for bit.  Between the parts the calls that read or

@@ -639,8 +639,11 @@ int lbm_step_part(lbm_ctx* c, int it, int part) {
     if (!axis) return fail(c, LBM_EINVAL, "split steps need j off the contiguous axis (LBM_FAST_AXIS)");
     StepHost h = step_base(c);
     step_inputs(c, h, it);
-    // tiles thinner than 5 cells along j: the first part is the whole step
-    h.part = c->d.ny >= 5 ? LBM_PART_EDGE : 0;
+    // Edge rows per side: 2 with a folded face (the unpack completes face ghosts in columns 1
+    // and ny, which rows 2 and ny-1 pull; rows 2 and ny-1 push face ghosts into the columns the
+    // pack reads), else 1.  Tiles without inner rows: the first part is the whole step.
+    h.part_edge = c->fold_on && (c->fold_dir == 0 || c->fold_dir == 2) ? 2 : 1;
+    h.part = c->d.ny > 2 * h.part_edge ? LBM_PART_EDGE : 0;
     h.part_axis = axis;
     HIPCHK(c, c->ops->step(c->lay, h, c->stream));
     step_advance(c);

@@ -52,7 +52,7 @@ struct StepHost {
     double utime, cosd, sind, rho0;           // inflow values of the step
     int heavy_first;                          // bit 0 / 1: cells 1 and n of ax1 / ax2 first
     int halo_dir;                             // halo axis (-1: none)
-    int part, part_axis;                      // split step along the halo axis (StepArgs::part)
+    int part, part_axis, part_edge;           // split step along the halo axis (StepArgs::part)
     int bgk3, hrr, vre;
     int variant;      // step-kernel variant: bit 0 non-temporal stores, bit 2 two cells per lane
     int block_x, block_y;

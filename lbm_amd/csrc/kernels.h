@@ -62,10 +62,10 @@ struct StepArgs {
     int heavy_first;       // bit 0 / bit 1: cells 1 and n of ax1 / ax2 are dispatched first
     int halo_dir;          // axis whose ghost planes are halos filled by lbm_halo_unpack (-1: none)
     int xcd_groups;        // 1: the blocks of one row group run on one XCD (step_kernel)
-    // split step along the halo axis (step_kernel): 0 every cell; 1 the cells within two of a halo
-    // plane (coordinates 1, 2, n-1, n; the launch covers 4 along part_axis); 2 the others (3..n-2,
-    // the launch covers n-4).  part_axis: 1 the halo axis is ax1, 2 it is ax2.
-    int part, part_axis;
+    // split step along the halo axis (step_kernel): 0 every cell; 1 the part_edge cells next to
+    // each halo plane (the launch covers 2 * part_edge along part_axis); 2 the others (the launch
+    // covers n - 2 * part_edge).  part_axis: 1 the halo axis is ax1, 2 it is ax2.
+    int part, part_axis, part_edge;
 };
 
 // Signed sum  sx*a + sy*b + sz*c  for compile-time signs in {-1,0,1}: adds only.
@@ -621,7 +621,7 @@ __global__ void __launch_bounds__(sizeof(typename C::raw) >= 8 ? 256 : 1024, siz
     if (a.part) {
         // launch coordinate along the halo axis -> the part's cell (part_row)
         const int n = a.lay.n(a.part_axis == 1 ? a.lay.ax1 : a.lay.ax2);
-        const int m = part_row(a.part_axis == 1 ? y : bz, a.part, n);
+        const int m = part_row(a.part_axis == 1 ? y : bz, a.part, n, a.part_edge);
         if (a.part_axis == 1) y = m;
         else bz = m;
     }

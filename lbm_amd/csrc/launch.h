@@ -110,17 +110,18 @@ struct Launch {
         a.halo_dir = h.halo_dir;
         a.part = h.part;
         a.part_axis = h.part_axis;
+        a.part_edge = h.part_edge;
         return a;
     }
 
     static hipError_t step(const Layout& L, const StepHost& h, hipStream_t s) {
         StepArgs<R> a = args_of(L, h);
         const dim3 block(h.block_x, h.block_y, 1);
-        // a split step covers 4 (part 1) or n - 4 (part 2) along the halo axis (n >= 5)
+        // a split step covers 2e (part 1) or n - 2e (part 2) along the halo axis (n > 2e)
         int n1 = L.n(L.ax1), n2 = L.n(L.ax2);
         if (h.part) {
             int& n = h.part_axis == 1 ? n1 : n2;
-            n = h.part == 1 ? 4 : n - 4;
+            n = h.part == 1 ? 2 * h.part_edge : n - 2 * h.part_edge;
             // the kernel's part mapping replaces the dispatch order along the halo axis
             a.heavy_first &= h.part_axis == 1 ? ~1 : ~2;
         }

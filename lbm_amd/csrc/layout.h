@@ -61,12 +61,12 @@ struct Layout {
     }
 };
 
-// Split steps along the halo axis (n >= 5 cells): launch coordinate c (0-based) of part 1, which
-// covers 4 launch rows (cells 1, 2, n-1, n), or of part 2, which covers n - 4 (cells 3 .. n-2) ->
-// the 0-based coordinate of its cell, or n (cell n + 1, which the step kernel skips) past the
-// part's rows.
-__host__ __device__ inline int part_row(int c, int part, int n) {
-    return part == 1 ? (c < 2 ? c : c < 4 ? n - 4 + c : n) : (c < n - 4 ? c + 2 : n);
+// Split steps along the halo axis, e edge rows at each end (n >= 2e + 1 cells): launch coordinate
+// c (0-based) of part 1, which covers 2e launch rows (cells 1..e and n-e+1..n), or of part 2, which
+// covers n - 2e (cells e+1 .. n-e) -> the 0-based coordinate of its cell, or n (cell n + 1, which
+// the step kernel skips) past the part's rows.
+__host__ __device__ inline int part_row(int c, int part, int n, int e) {
+    return part == 1 ? (c < e ? c : c < 2 * e ? n - 2 * e + c : n) : (c < n - 2 * e ? c + e : n);
 }
 
 // fast < 0: choose (first periodic axis, else i).  q > 0 with rowblock: row-blocked buffers when a

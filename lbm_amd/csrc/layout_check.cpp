@@ -65,14 +65,15 @@ int main() {
     if (big.rowblock || !mid.rowblock) return 5;
     // split steps: launch heights rounded up to the block height, as the launcher's grid is
     int nsplit = 0;
-    for (int len = 5; len <= 1024; ++len)
+    for (int e = 1; e <= 2; ++e)
+    for (int len = 2 * e + 1; len <= 1024; ++len)
         for (int by : {1, 2, 4, 8}) {
             std::vector<int> cnt(len, 0);
             for (int part = 1; part <= 2; ++part) {
-                const int rows = part == 1 ? 4 : len - 4;
+                const int rows = part == 1 ? 2 * e : len - 2 * e;
                 const int launched = (rows + by - 1) / by * by;
                 for (int c = 0; c < launched; ++c) {
-                    const int m = part_row(c, part, len);
+                    const int m = part_row(c, part, len, e);
                     if (m == len) {
                         if (c < rows) return 6;   // a coordinate of the part mapped outside
                         continue;

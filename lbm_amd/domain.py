@@ -314,8 +314,9 @@ class Domain:
         self._chk(self.lib.lbm_step(self.h, int(it0), int(nsteps)), "lbm_step")
 
     def step_part(self, it, part):
-        """Step ``it`` of a halo domain in two launches: PART_EDGE (j = 1, 2, ny-1, ny), after which
-        halo_pack may run, then PART_INNER (the other cells), which completes the step."""
+        """Step ``it`` of a halo domain in two launches: PART_EDGE (j = 1, 2, ny-1, ny; j = 1, ny
+        without a closed face), after which halo_pack may run, then PART_INNER (the other cells),
+        which completes the step."""
         self._chk(self.lib.lbm_step_part(self.h, int(it), int(part)), "lbm_step_part")
 
     def synchronize(self):

@@ -189,8 +189,8 @@ def test_two_tiles_separate_inflow_pass(name, monkeypatch):
 @pytest.mark.parametrize("name", list(CASES))
 def test_split_steps_match_untiled(name, splits):
     """Edge rows, pack, inner rows with the transfer on another stream, unpack: the untiled run bit
-    for bit.  Tiles of 4 rows (one launch: the edge part is the whole tile), 5 (an inner part of
-    one row) and 8 (four inner rows)."""
+    for bit.  Two edge rows per side with a closed face, one without.  Tiles of 4 rows (with a
+    face: one launch, the edge part is the whole tile), 5 and 8."""
     run_case(name, splits, split=True)
 
 

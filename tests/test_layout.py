@@ -2,7 +2,8 @@
 (population, cell) of the padded grid gets its own element inside the buffer in both the
 row-blocked and the plane arrangement, rows start 128-byte aligned, per-cell indices are distinct,
 and row-blocked buffers are used only below 4 GiB; the two launches of a split step cover each cell
-of the halo axis once (tile heights 5..1024, block heights 1, 2, 4, 8)."""
+of the halo axis once (one or two edge rows per side, tile heights up to 1024, block heights 1, 2,
+4, 8)."""
 import json
 import os
 import subprocess
@@ -22,6 +23,6 @@ def test_layouts_are_bijective_aligned_and_bounded():
     summary = rows[-1]
     assert summary["cases"] == 6 * 4 * 2 * 5 * 2 * 4
     assert summary["rowblock_768x256x192_f32"] == 0 and summary["rowblock_384x192x128_f32"] == 1
-    assert summary["split_heights"] == (1024 - 5 + 1) * 4
+    assert summary["split_heights"] == ((1024 - 3 + 1) + (1024 - 5 + 1)) * 4
     cases = rows[:-1]
     assert any(r["rowblock"] for r in cases) and any(not r["rowblock"] for r in cases)
