@@ -432,8 +432,8 @@ def main():
     if a.steps is None:
         a.steps = {"ensemble": 500, "adjoint": 400}.get(a.workload, 1000)
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
-                                   "described in DESIGN.md, not built)"}))
+        print(json.dumps({"error": "the benchmark runs on one GPU; --gpus must be 1 (a j-tiled domain "
+                                   "measures one GPU's share: --workload tile)"}))
         sys.exit(2)
     if a.workload == "ensemble":
         bench_ensemble(a)
