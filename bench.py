@@ -176,7 +176,8 @@ def bench_ensemble(a):
                                "no-slip k=1 / free-slip k=nz, inflow forcing at i=10, nu=1e-4",
                    "cells": cells, "members": a.members, "storage": a.storage, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_from_profiles("ensemble_step_kernel_batch_q19_64members"),
                      "kernel": "step_kernel_batch (graph; faces and inflow planes folded)", "bytes_per_cell": bpc_bytes,
                      "kernel_ms": round(step_ms, 5)},
         "tuned_launch": {"block": block, "blocks_per_cu": bpc, "ms_per_step": round(tune_ms, 5)},
