@@ -28,6 +28,15 @@ sweep) of D3Q19 regularized + Vreman, FP32, 192x96x96, inflow/outflow i, periodi
 k=1, with 64 Gaussian body-force blobs and an 8-coefficient power-law inflow profile, objective
 the time-mean kinetic energy over i in [140,180]; the gradient is checked against FP64 central
 differences (h = 1e-6) on a subset of the parameters and against the FP64 adjoint on all.
+
+    python bench.py --workload tile [--tiles 8] [--split]
+
+times one GPU's j-tile of the headline workload (tile 4 of 8, 384x24x128) with its halo exchange
+every step; --split steps it in two launches with the transfer during the second.
+
+    python bench.py --workload q19 [--storage f32|f16s]
+
+times D3Q19 BGK (tau = 0.6, periodic i and j, no-slip k) at 768x256x192.
 """
 from __future__ import annotations
 
@@ -177,7 +186,9 @@ def bench_ensemble(a):
                    "cells": cells, "members": a.members, "storage": a.storage, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_from_profiles("ensemble_step_kernel_batch_q19_64members"),
+                     # measured for the default ensemble (64 FP32 members) only
+                     "traffic": (traffic_from_profiles("ensemble_step_kernel_batch_q19_64members")
+                                 if a.storage == "f32" and a.members == 64 else None),
                      "kernel": "step_kernel_batch (graph; faces and inflow planes folded)", "bytes_per_cell": bpc_bytes,
                      "kernel_ms": round(step_ms, 5)},
         "tuned_launch": {"block": block, "blocks_per_cu": bpc, "ms_per_step": round(tune_ms, 5)},
