@@ -50,7 +50,8 @@ int copy_dev(lbm_ctx* c, void* dst, const void* src, size_t n) {
 extern "C" {
 
 int lbm_objective_run(lbm_ctx* c, int it0, int nsteps, const int32_t* box, double* J) {
-    if (!c || !J) return fail(c, LBM_EINVAL, "null argument");
+    if (int rc_ = check_ctx(c)) return rc_;
+    if (!J) return fail(c, LBM_EINVAL, "null argument");
     if (nsteps < 1) return fail(c, LBM_EINVAL, "nsteps must be >= 1");
     AdjBox b;
     if (int rc = check_box(c, box, &b)) return rc;
@@ -86,14 +87,16 @@ int lbm_objective_run(lbm_ctx* c, int it0, int nsteps, const int32_t* box, doubl
 }
 
 int lbm_objective_tangent(lbm_ctx* c, double* dJ) {
-    if (!c || !dJ) return fail(c, LBM_EINVAL, "null argument");
+    if (int rc_ = check_ctx(c)) return rc_;
+    if (!dJ) return fail(c, LBM_EINVAL, "null argument");
     if (c->d.storage != LBM_STORE_F64T) return fail(c, LBM_EINVAL, "tangents need LBM_STORE_F64T storage");
     *dJ = c->obj_tangent;
     return LBM_OK;
 }
 
 int lbm_set_tangent(lbm_ctx* c, const double* dforce, const double* dshear) {
-    if (!c) return fail(c, LBM_EINVAL, "null context");
+    if (int rc_ = check_ctx(c)) return rc_;
+    if (int rc = check_no_part(c)) return rc;
     const lbm_desc& d = c->d;
     if (d.storage != LBM_STORE_F64T) return fail(c, LBM_EINVAL, "tangents need LBM_STORE_F64T storage");
     if (dforce && !c->has_force) return fail(c, LBM_ESTATE, "set the external forcing before its tangent");
@@ -124,7 +127,8 @@ int lbm_set_tangent(lbm_ctx* c, const double* dforce, const double* dshear) {
 
 int lbm_adjoint_run(lbm_ctx* c, int it0, int nsteps, int every, const int32_t* box, double* J, double* grad_force,
                     double* grad_shear) {
-    if (!c || !J) return fail(c, LBM_EINVAL, "null argument");
+    if (int rc_ = check_ctx(c)) return rc_;
+    if (!J) return fail(c, LBM_EINVAL, "null argument");
     const lbm_desc& d = c->d;
     if (!c->have_state) return fail(c, LBM_ESTATE, "no state");
     if (c->pending || c->use_ext)

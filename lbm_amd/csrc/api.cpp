@@ -30,6 +30,13 @@ int fail(lbm_ctx* c, int code, const std::string& msg) {
     return code;
 }
 
+int check_ctx(const lbm_ctx* c) {
+    if (!c) return fail(nullptr, LBM_EINVAL, "null context");
+    if (c->tag != kCtxTag)
+        return fail(nullptr, LBM_EINVAL, "not a live lbm_ctx handle (an ensemble handle, or a destroyed context)");
+    return LBM_OK;
+}
+
 int hipfail(lbm_ctx* c, hipError_t e, const char* where) {
     return fail(c, LBM_EHIP, std::string(where) + ": " + hipGetErrorString(e));
 }
@@ -334,7 +341,7 @@ int check_no_part(lbm_ctx* c) {
 extern "C" {
 
 const char* lbm_last_error(const lbm_ctx* ctx) {
-    return ctx ? ctx->err.c_str() : g_create_error.c_str();
+    return ctx && ctx->tag == kCtxTag ? ctx->err.c_str() : g_create_error.c_str();
 }
 
 int lbm_create(const lbm_desc* desc, lbm_ctx** out) {
@@ -467,6 +474,8 @@ int lbm_create(const lbm_desc* desc, lbm_ctx** out) {
 
 int lbm_destroy(lbm_ctx* c) {
     if (!c) return LBM_OK;
+    if (int rc = check_ctx(c)) return rc;
+    c->tag = 0;
     if (c->stream) hipStreamSynchronize(c->stream);
     for (int b = 0; b < 2; ++b)
         if (c->buf[b]) hipFree(c->buf[b]);
@@ -482,7 +491,8 @@ int lbm_destroy(lbm_ctx* c) {
 }
 
 int lbm_set_state(lbm_ctx* c, const void* f, int realkind) {
-    if (!c || !f) return fail(c, LBM_EINVAL, "null argument");
+    if (int rc_ = check_ctx(c)) return rc_;
+    if (!f) return fail(c, LBM_EINVAL, "null argument");
     if (int rc = check_no_part(c)) return rc;
     if (int rc = check_kind(c, realkind)) return rc;
     const size_t n = ncell_padded(c) * c->d.q;
@@ -500,7 +510,8 @@ int lbm_set_state(lbm_ctx* c, const void* f, int realkind) {
 
 int lbm_init_equilibrium(lbm_ctx* c, const void* rho, const void* u, const void* v, const void* w, int realkind,
                          int it0) {
-    if (!c || !rho || !u || !v || !w) return fail(c, LBM_EINVAL, "null argument");
+    if (int rc_ = check_ctx(c)) return rc_;
+    if (!rho || !u || !v || !w) return fail(c, LBM_EINVAL, "null argument");
     if (int rc = check_no_part(c)) return rc;
     if (int rc = check_kind(c, realkind)) return rc;
     if (c->d.jbnd == LBM_BND_HALO)
@@ -593,7 +604,7 @@ int step_complete(lbm_ctx* c, int it, bool timed) {
 }  // namespace
 
 int lbm_step(lbm_ctx* c, int it0, int nsteps) {
-    if (!c) return fail(c, LBM_EINVAL, "null context");
+    if (int rc_ = check_ctx(c)) return rc_;
     if (!c->have_state) return fail(c, LBM_ESTATE, "no state: call lbm_set_state or lbm_init_equilibrium first");
     if (nsteps < 0) return fail(c, LBM_EINVAL, "nsteps < 0");
     if (int rc = check_no_part(c)) return rc;
@@ -616,7 +627,7 @@ int lbm_step(lbm_ctx* c, int it0, int nsteps) {
 }
 
 int lbm_step_part(lbm_ctx* c, int it, int part) {
-    if (!c) return fail(c, LBM_EINVAL, "null context");
+    if (int rc_ = check_ctx(c)) return rc_;
     if (part != LBM_PART_EDGE && part != LBM_PART_INNER)
         return fail(c, LBM_EINVAL, "part must be LBM_PART_EDGE or LBM_PART_INNER");
     if (part == LBM_PART_INNER) {
@@ -665,7 +676,7 @@ int average_flush(lbm_ctx* c) {
 }  // namespace
 
 int lbm_set_averaging(lbm_ctx* c, int on) {
-    if (!c) return fail(c, LBM_EINVAL, "null context");
+    if (int rc_ = check_ctx(c)) return rc_;
     if (int rc = check_no_part(c)) return rc;
     if (on) {
         const size_t n = 6 * (size_t)c->lay.cplane * c->ops->real_bytes;
@@ -682,7 +693,7 @@ int lbm_set_averaging(lbm_ctx* c, int on) {
 }
 
 int lbm_get_averages(lbm_ctx* c, void* uave, void* vave, void* wave, void* ti, int* count, int realkind) {
-    if (!c) return fail(c, LBM_EINVAL, "null context");
+    if (int rc_ = check_ctx(c)) return rc_;
     if (int rc = check_no_part(c)) return rc;
     if (int rc = check_kind(c, realkind)) return rc;
     if (!c->avg.p) return fail(c, LBM_ESTATE, "no averages: call lbm_set_averaging first");
@@ -735,7 +746,8 @@ int lbm_get_averages(lbm_ctx* c, void* uave, void* vave, void* wave, void* ti, i
 }
 
 int lbm_get_state(lbm_ctx* c, void* f, int realkind) {
-    if (!c || !f) return fail(c, LBM_EINVAL, "null argument");
+    if (int rc_ = check_ctx(c)) return rc_;
+    if (!f) return fail(c, LBM_EINVAL, "null argument");
     if (int rc = check_no_part(c)) return rc;
     if (!c->have_state) return fail(c, LBM_ESTATE, "no state");
     if (int rc = check_kind(c, realkind)) return rc;
@@ -751,7 +763,8 @@ int lbm_get_state(lbm_ctx* c, void* f, int realkind) {
 }
 
 int lbm_get_macros(lbm_ctx* c, void* rho, void* u, void* v, void* w, int realkind) {
-    if (!c || !rho || !u || !v || !w) return fail(c, LBM_EINVAL, "null argument");
+    if (int rc_ = check_ctx(c)) return rc_;
+    if (!rho || !u || !v || !w) return fail(c, LBM_EINVAL, "null argument");
     if (int rc = check_no_part(c)) return rc;
     if (!c->have_state) return fail(c, LBM_ESTATE, "no state");
     if (int rc = check_kind(c, realkind)) return rc;
@@ -771,7 +784,8 @@ int lbm_get_macros(lbm_ctx* c, void* rho, void* u, void* v, void* w, int realkin
 }
 
 int lbm_get_tau(lbm_ctx* c, void* tau, int realkind) {
-    if (!c || !tau) return fail(c, LBM_EINVAL, "null argument");
+    if (int rc_ = check_ctx(c)) return rc_;
+    if (!tau) return fail(c, LBM_EINVAL, "null argument");
     if (int rc = check_no_part(c)) return rc;
     if (int rc = check_kind(c, realkind)) return rc;
     const Layout& L = c->lay;
@@ -797,7 +811,7 @@ int lbm_get_tau(lbm_ctx* c, void* tau, int realkind) {
 }
 
 int lbm_set_blanking(lbm_ctx* c, const uint8_t* mask) {
-    if (!c) return fail(c, LBM_EINVAL, "null context");
+    if (int rc_ = check_ctx(c)) return rc_;
     if (int rc = check_no_part(c)) return rc;
     if (!mask) {
         c->has_solid = false;
@@ -829,14 +843,15 @@ int lbm_set_blanking(lbm_ctx* c, const uint8_t* mask) {
 }
 
 int lbm_set_ablvisc(lbm_ctx* c, const void* abl, int realkind) {
-    if (!c || !abl) return fail(c, LBM_EINVAL, "null argument");
+    if (int rc_ = check_ctx(c)) return rc_;
+    if (!abl) return fail(c, LBM_EINVAL, "null argument");
     if (int rc = check_no_part(c)) return rc;
     if (int rc = check_kind(c, realkind)) return rc;
     return upload_reals(c, c->ablvisc, abl, c->d.nz, realkind);
 }
 
 int lbm_set_inflow(lbm_ctx* c, const void* shear, double uini, double udir, int realkind) {
-    if (!c) return fail(c, LBM_EINVAL, "null context");
+    if (int rc_ = check_ctx(c)) return rc_;
     if (int rc = check_no_part(c)) return rc;
     if (int rc = check_kind(c, realkind)) return rc;
     c->uini = uini;
@@ -847,7 +862,7 @@ int lbm_set_inflow(lbm_ctx* c, const void* shear, double uini, double udir, int 
 }
 
 int lbm_set_inflow_series(lbm_ctx* c, int it_lo, int it_hi, const void* ut, const void* ud, int realkind) {
-    if (!c) return fail(c, LBM_EINVAL, "null context");
+    if (int rc_ = check_ctx(c)) return rc_;
     if (int rc = check_no_part(c)) return rc;
     if (int rc = check_kind(c, realkind)) return rc;
     c->ut_series.clear();
@@ -865,7 +880,7 @@ int lbm_set_inflow_series(lbm_ctx* c, int it_lo, int it_hi, const void* ut, cons
 
 int lbm_set_turbulence(lbm_ctx* c, const void* uu, const void* vv, const void* ww, int nrturb, double ampl,
                        int realkind) {
-    if (!c) return fail(c, LBM_EINVAL, "null context");
+    if (int rc_ = check_ctx(c)) return rc_;
     if (int rc = check_no_part(c)) return rc;
     if (!uu) {
         c->has_turb = false;
@@ -886,7 +901,7 @@ int lbm_set_turbulence(lbm_ctx* c, const void* uu, const void* vv, const void* w
 
 int lbm_generate_turbulence(lbm_ctx* c, int nrturb, double ampl, uint64_t seed, void* uu, void* vv, void* ww, void* rr,
                             int realkind) {
-    if (!c) return fail(c, LBM_EINVAL, "null context");
+    if (int rc_ = check_ctx(c)) return rc_;
     if (int rc = check_no_part(c)) return rc;
     if (nrturb < 1) return fail(c, LBM_EINVAL, "nrturb must be >= 1");
     if ((uu || vv || ww || rr) && check_kind(c, realkind)) return LBM_EINVAL;
@@ -937,7 +952,7 @@ int lbm_generate_turbulence(lbm_ctx* c, int nrturb, double ampl, uint64_t seed, 
 }
 
 int lbm_set_external_forcing(lbm_ctx* c, const void* force, int realkind) {
-    if (!c) return fail(c, LBM_EINVAL, "null context");
+    if (int rc_ = check_ctx(c)) return rc_;
     if (int rc = check_no_part(c)) return rc;
     if (!force) {
         c->has_force = false;
@@ -967,15 +982,16 @@ int lbm_set_external_forcing(lbm_ctx* c, const void* force, int realkind) {
 }
 
 int lbm_synchronize(lbm_ctx* c) {
-    if (!c) return fail(c, LBM_EINVAL, "null context");
+    if (int rc_ = check_ctx(c)) return rc_;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return LBM_OK;
 }
 
-void* lbm_stream(lbm_ctx* c) { return c ? (void*)c->stream : nullptr; }
+void* lbm_stream(lbm_ctx* c) { return check_ctx(c) == LBM_OK ? (void*)c->stream : nullptr; }
 
 int lbm_halo_bytes(lbm_ctx* c, size_t* bytes) {
-    if (!c || !bytes) return fail(c, LBM_EINVAL, "null argument");
+    if (int rc_ = check_ctx(c)) return rc_;
+    if (!bytes) return fail(c, LBM_EINVAL, "null argument");
     if (c->d.jbnd != LBM_BND_HALO) return fail(c, LBM_EINVAL, "not a halo domain (jbnd != LBM_BND_HALO)");
     *bytes = (size_t)(c->d.q == 27 ? 9 : 5) * (c->d.nx + 2) * (c->d.nz + 2) * c->ops->raw_bytes;
     return LBM_OK;
@@ -983,7 +999,7 @@ int lbm_halo_bytes(lbm_ctx* c, size_t* bytes) {
 
 namespace {
 int halo_check(lbm_ctx* c, int side, const void* buf) {
-    if (!c || !buf) return fail(c, LBM_EINVAL, "null argument");
+    if (!buf) return fail(c, LBM_EINVAL, "null argument");
     if (c->d.jbnd != LBM_BND_HALO) return fail(c, LBM_EINVAL, "not a halo domain (jbnd != LBM_BND_HALO)");
     if (side < 0 || side > 2) return fail(c, LBM_EINVAL, "side must be 0, 1 or 2 (both)");
     if (!c->have_state) return fail(c, LBM_ESTATE, "no state: call lbm_set_state first");
@@ -992,6 +1008,7 @@ int halo_check(lbm_ctx* c, int side, const void* buf) {
 }  // namespace
 
 int lbm_halo_pack(lbm_ctx* c, int side, void* dev_buf) {
+    if (int rc_ = check_ctx(c)) return rc_;
     if (int rc = halo_check(c, side, dev_buf)) return rc;
     // the state of the last step (or the uploaded state, which both buffers hold)
     HIPCHK(c, c->ops->halo_pack(c->lay, c->buf[c->cur], side, dev_buf, c->stream));
@@ -999,13 +1016,14 @@ int lbm_halo_pack(lbm_ctx* c, int side, void* dev_buf) {
 }
 
 int lbm_halo_unpack(lbm_ctx* c, int side, const void* dev_buf) {
+    if (int rc_ = check_ctx(c)) return rc_;
     if (int rc = halo_check(c, side, dev_buf)) return rc;
     HIPCHK(c, c->ops->halo_unpack(c->lay, c->buf[c->cur], side, dev_buf, c->fold_on ? c->fold_dir : -1, c->stream));
     return LBM_OK;
 }
 
 int lbm_set_timing(lbm_ctx* c, int on) {
-    if (!c) return fail(c, LBM_EINVAL, "null context");
+    if (int rc_ = check_ctx(c)) return rc_;
     if (on < 0) return fail(c, LBM_EINVAL, "timing interval < 0");
     c->timing = on != 0;
     c->timing_every = on > 1 ? on : 1;
@@ -1015,7 +1033,7 @@ int lbm_set_timing(lbm_ctx* c, int on) {
 }
 
 int lbm_timing_read(lbm_ctx* c, double* step_ms, int* n_step, double* bc_ms, int* n_bc) {
-    if (!c) return fail(c, LBM_EINVAL, "null context");
+    if (int rc_ = check_ctx(c)) return rc_;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     double s = 0, b = 0;
     for (size_t q = 0; q + 1 < c->n_step_ev; q += 2) {
@@ -1037,7 +1055,7 @@ int lbm_timing_read(lbm_ctx* c, double* step_ms, int* n_step, double* bc_ms, int
 }
 
 int lbm_get_totals(lbm_ctx* c, double* mass, double* mom) {
-    if (!c) return fail(c, LBM_EINVAL, "null context");
+    if (int rc_ = check_ctx(c)) return rc_;
     if (int rc = check_no_part(c)) return rc;
     if (!c->have_state) return fail(c, LBM_ESTATE, "no state");
     const int nb = 1024;
@@ -1056,7 +1074,7 @@ int lbm_get_totals(lbm_ctx* c, double* mass, double* mom) {
 }
 
 int lbm_get_info(lbm_ctx* c, int64_t* bytes, int32_t* pitch, int64_t* plane) {
-    if (!c) return fail(c, LBM_EINVAL, "null context");
+    if (int rc_ = check_ctx(c)) return rc_;
     if (bytes) *bytes = (int64_t)(2 * c->buf_bytes);
     if (pitch) *pitch = c->lay.pitch;
     if (plane) *plane = c->lay.cplane;

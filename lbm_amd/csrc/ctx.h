@@ -18,9 +18,16 @@ struct DevBuf {
     size_t n = 0;
 };
 
+// Tag in the first field of every handle, checked by each export before it reads anything else: a
+// handle of the other kind (an lbm_ens* passed where an lbm_ctx* is expected, or the reverse) or a
+// destroyed one (the tag is cleared on destroy) fails with LBM_EINVAL instead of being used.
+constexpr uint32_t kCtxTag = 0x4c424d43u;   // "CMBL"
+constexpr uint32_t kEnsTag = 0x4c424d45u;   // "EMBL"
+
 }  // namespace lbm_host
 
 struct lbm_ctx {
+    uint32_t tag = lbm_host::kCtxTag;
     lbm_desc d{};
     lbm::Layout lay{};
     const lbm::Ops* ops = nullptr;
@@ -74,6 +81,11 @@ struct lbm_ctx {
 namespace lbm_host {
 
 int fail(lbm_ctx* c, int code, const std::string& msg);
+// LBM_OK for a live context handle; otherwise LBM_EINVAL with the message kept for
+// lbm_last_error(NULL)
+int check_ctx(const lbm_ctx* c);
+// LBM_ESTATE while a split step (lbm_step_part) is open
+int check_no_part(lbm_ctx* c);
 int hipfail(lbm_ctx* c, hipError_t e, const char* where);
 int dev_alloc(lbm_ctx* c, DevBuf& b, size_t n);
 void dev_free(DevBuf& b);

@@ -24,6 +24,7 @@ constexpr int kChunk = 1024;       // steps per upload of the per-step inflow va
 }  // namespace
 
 struct lbm_ens {
+    uint32_t tag = kEnsTag;
     std::vector<lbm_ctx*> mem;
     const Ops* ops = nullptr;
     int bgk3 = 0, hrr = 0, vre = 0;
@@ -52,6 +53,15 @@ struct lbm_ens {
 };
 
 namespace {
+
+// LBM_OK for a live ensemble handle; otherwise LBM_EINVAL with the message kept for
+// lbm_ensemble_last_error(NULL) (and lbm_last_error(NULL))
+int check_ens(const lbm_ens* e) {
+    if (!e) return fail(nullptr, LBM_EINVAL, "null ensemble");
+    if (e->tag != kEnsTag)
+        return fail(nullptr, LBM_EINVAL, "not a live lbm_ens handle (a context handle, or a destroyed ensemble)");
+    return LBM_OK;
+}
 
 int efail(lbm_ens* e, int code, const std::string& msg) {
     e->err = msg;
@@ -233,13 +243,16 @@ int batched(lbm_ens* e, int it0, int nsteps) {
 extern "C" {
 
 const char* lbm_ensemble_last_error(const lbm_ens* e) {
-    return e ? e->err.c_str() : "null ensemble";
+    return e && e->tag == kEnsTag ? e->err.c_str() : lbm_last_error(nullptr);
 }
 
 int lbm_ensemble_create(lbm_ctx* const* members, int n, lbm_ens** out) {
-    if (!out) return LBM_EINVAL;
+    if (!out) return fail(nullptr, LBM_EINVAL, "null argument");
     *out = nullptr;
-    if (!members || n < 1) return LBM_EINVAL;
+    if (!members || n < 1) return fail(nullptr, LBM_EINVAL, "no members");
+    for (int m = 0; m < n; ++m)
+        if (check_ctx(members[m]) != LBM_OK)
+            return fail(nullptr, LBM_EINVAL, "member " + std::to_string(m) + " is not a live lbm_ctx handle");
     lbm_ens* e = new lbm_ens();
     auto bad = [&](int code, const std::string& msg) {
         // keep the message reachable: the ensemble is returned so lbm_ensemble_last_error works
@@ -249,7 +262,6 @@ int lbm_ensemble_create(lbm_ctx* const* members, int n, lbm_ens** out) {
     };
     for (int m = 0; m < n; ++m) {
         lbm_ctx* c = members[m];
-        if (!c) return bad(LBM_EINVAL, "member " + std::to_string(m) + " is null");
         e->mem.push_back(c);
         const lbm_desc& d = c->d;
         const lbm_desc& d0 = members[0]->d;
@@ -286,6 +298,8 @@ int lbm_ensemble_create(lbm_ctx* const* members, int n, lbm_ens** out) {
 
 int lbm_ensemble_destroy(lbm_ens* e) {
     if (!e) return LBM_OK;
+    if (int rc = check_ens(e)) return rc;
+    e->tag = 0;
     if (e->stream) hipStreamSynchronize(e->stream);
     drop_graphs(e);
     for (DevBuf* b : {&e->tab, &e->start, &e->vals, &e->clock, &e->backup}) dev_free(*b);
@@ -299,7 +313,7 @@ int lbm_ensemble_destroy(lbm_ens* e) {
 }
 
 int lbm_ensemble_set_launch(lbm_ens* e, int block, int blocks_per_cu) {
-    if (!e) return LBM_EINVAL;
+    if (int rc_ = check_ens(e)) return rc_;
     if (block < 64 || block > 1024 || block % 64 != 0) return efail(e, LBM_EINVAL, "block must be a multiple of 64 in [64, 1024]");
     if (blocks_per_cu < 0 || blocks_per_cu > 64) return efail(e, LBM_EINVAL, "blocks_per_cu must be in [0, 64]");
     e->block = block;
@@ -308,14 +322,15 @@ int lbm_ensemble_set_launch(lbm_ens* e, int block, int blocks_per_cu) {
 }
 
 int lbm_ensemble_get_launch(lbm_ens* e, int* block, int* blocks_per_cu) {
-    if (!e || !block || !blocks_per_cu) return LBM_EINVAL;
+    if (int rc_ = check_ens(e)) return rc_;
+    if (!block || !blocks_per_cu) return efail(e, LBM_EINVAL, "null argument");
     *block = e->block;
     *blocks_per_cu = e->bpc;
     return LBM_OK;
 }
 
 int lbm_ensemble_step(lbm_ens* e, int it0, int nsteps) {
-    if (!e) return LBM_EINVAL;
+    if (int rc_ = check_ens(e)) return rc_;
     if (nsteps < 0) return efail(e, LBM_EINVAL, "nsteps < 0");
     if (nsteps == 0) return LBM_OK;
     const int n = (int)e->mem.size();
@@ -344,19 +359,20 @@ int lbm_ensemble_step(lbm_ens* e, int it0, int nsteps) {
 }
 
 int lbm_ensemble_synchronize(lbm_ens* e) {
-    if (!e) return LBM_EINVAL;
+    if (int rc_ = check_ens(e)) return rc_;
     EHIP(e, hipStreamSynchronize(e->stream));
     return LBM_OK;
 }
 
 int lbm_ensemble_set_timing(lbm_ens* e, int on) {
-    if (!e) return LBM_EINVAL;
+    if (int rc_ = check_ens(e)) return rc_;
     e->timing = on != 0;
     return LBM_OK;
 }
 
 int lbm_ensemble_timing_read(lbm_ens* e, double* ms, int* nsteps) {
-    if (!e || !ms || !nsteps) return LBM_EINVAL;
+    if (int rc_ = check_ens(e)) return rc_;
+    if (!ms || !nsteps) return efail(e, LBM_EINVAL, "null argument");
     EHIP(e, hipStreamSynchronize(e->stream));
     double t = 0.0;
     for (size_t q = 0; q + 1 < e->n_ev; q += 2) {
@@ -375,7 +391,7 @@ int lbm_ensemble_timing_read(lbm_ens* e, double* ms, int* nsteps) {
 // `trial_steps` timed steps from the same saved state; the fastest is kept and the state and the
 // members' bookkeeping are restored afterwards.
 int lbm_ensemble_tune(lbm_ens* e, int it0, int trial_steps, int* best_block, int* best_bpc, double* best_ms) {
-    if (!e) return LBM_EINVAL;
+    if (int rc_ = check_ens(e)) return rc_;
     if (trial_steps < 1) return efail(e, LBM_EINVAL, "trial_steps must be >= 1");
     const int n = (int)e->mem.size();
     struct Saved {

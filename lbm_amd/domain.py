@@ -364,7 +364,7 @@ class Ensemble:
         h = C.c_void_p()
         rc = self.lib.lbm_ensemble_create(arr, len(self.members), C.byref(h))
         if rc != 0:
-            msg = self.lib.lbm_ensemble_last_error(h).decode() if h.value else "invalid arguments"
+            msg = self.lib.lbm_ensemble_last_error(h).decode()   # h NULL: the thread's last message
             if h.value:
                 self.lib.lbm_ensemble_destroy(h)
             raise LBMError(f"lbm_ensemble_create failed ({rc}): {msg}")
@@ -384,41 +384,6 @@ class Ensemble:
             self.close()
         except Exception:
             pass
-
-    # -- objective and adjoint -------------------------------------------------------------
-    @staticmethod
-    def _box(box):
-        (i0, i1), (j0, j1), (k0, k1) = box
-        return (C.c_int32 * 6)(i0, i1, j0, j1, k0, k1)
-
-    def objective(self, it0, nsteps, box):
-        """Steps it0+1 .. it0+nsteps; returns the time-mean kinetic energy 0.5 rho|u|^2 over box
-        ((i0,i1),(j0,j1),(k0,k1)), 1-based inclusive (lbm_objective_run)."""
-        J = C.c_double()
-        self._chk(self.lib.lbm_objective_run(self.h, it0, nsteps, self._box(box), C.byref(J)), "lbm_objective_run")
-        return J.value
-
-    def adjoint(self, it0, nsteps, every, box, grad_force=True):
-        """As objective(), plus the reverse sweep: returns (J, dJ/d external forcing with the
-        forcing's shape (nz+2, ny+2, nx+2, 3) or None, dJ/d uvel_shear (nz,))."""
-        nx, ny, nz = self.dims
-        J = C.c_double()
-        gF = np.zeros((nz + 2, ny + 2, nx + 2, 3)) if grad_force else None
-        gS = np.zeros(nz)
-        self._chk(self.lib.lbm_adjoint_run(self.h, it0, nsteps, every, self._box(box), C.byref(J), _p(gF), _p(gS)),
-                  "lbm_adjoint_run")
-        return J.value, gF, gS
-
-    def set_tangent(self, dforce=None, dshear=None):
-        """Direction of the tangent run (storage "f64t"): d forcing (forcing's shape) and d shear."""
-        F = None if dforce is None else self._arr(dforce, self.shape_cell + (3,), np.float64)
-        S = None if dshear is None else np.ascontiguousarray(dshear, dtype=np.float64)
-        self._chk(self.lib.lbm_set_tangent(self.h, _p(F), _p(S)), "lbm_set_tangent")
-
-    def objective_tangent(self):
-        dJ = C.c_double()
-        self._chk(self.lib.lbm_objective_tangent(self.h, C.byref(dJ)), "lbm_objective_tangent")
-        return dJ.value
 
     # -- time stepping -------------------------------------------------------------------
     def step(self, it0, nsteps=1):
