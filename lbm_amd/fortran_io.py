@@ -165,3 +165,59 @@ def read_testing(path: str, shape) -> np.ndarray:
     with open(path, "rb") as fh:
         data = read_record(fh, 4 * int(np.prod(shape)))
     return np.frombuffer(data, dtype=np.float32).reshape(shape).copy()
+
+
+# ---------------------------------------------------------------------------------------
+# uvel_shear.dat / uvel_time.dat : the inflow profile and its time series
+# ---------------------------------------------------------------------------------------
+def _list_directed_rows(path: str):
+    """Rows of whitespace- or comma-separated values, blank lines skipped (list-directed input)."""
+    with open(path) as fh:
+        return [r.replace(",", " ").split() for r in fh if r.strip()]
+
+
+def read_inflow(directory: str, nz: int, nt0: int, nt1: int, uini: float, udir: float, vel: float = 1.0,
+                time: float = 1.0, dtype=np.float32):
+    """The inflow profile and time series the reference reads at start-up (src/m_inflow.F90:27-86),
+    in the REAL kind ``dtype``: returns (uvel_shear (nz,), uvel_time (nt1-nt0+1,), udir_time (nt1-nt0+1,)),
+    the series for steps nt0..nt1, ready for Domain.set_inflow / Domain.set_inflow_series.
+
+    * uvel_shear.dat (optional): nz rows "k z u"; uvel_shear(k) = u(k) / u(nz), else 1.
+    * uvel_time.dat (optional): rows "t u dir"; u is divided by ``vel`` (p2l%vel, m/s per lattice
+      unit); step i is at t = (i - 1) * ``time`` (p2l%time); the series is u, dir of the first row
+      up to its t, of the last row from its t on, linear in between.  Without the file every step
+      takes uini and udir.
+    All arithmetic rounds in ``dtype`` as the reference's REAL does."""
+    R = np.dtype(dtype).type
+    uh = np.ones(nz, dtype)
+    p = os.path.join(directory, "uvel_shear.dat")
+    if os.path.exists(p):
+        rows = _list_directed_rows(p)
+        if len(rows) < nz:
+            raise ValueError(f"{p}: {len(rows)} rows for nz = {nz}")
+        for k in range(nz):
+            uh[k] = R(rows[k][2])
+    shear = (uh / uh[nz - 1]).astype(dtype)
+    n = nt1 - nt0 + 1
+    ut = np.full(n, R(uini), dtype)
+    ud = np.full(n, R(udir), dtype)
+    p = os.path.join(directory, "uvel_time.dat")
+    if os.path.exists(p):
+        rows = _list_directed_rows(p)
+        td = np.array([R(r[0]) for r in rows], dtype)
+        uv = np.array([R(r[1]) for r in rows], dtype) / R(vel)
+        dv = np.array([R(r[2]) for r in rows], dtype)
+        for q, i in enumerate(range(nt0, nt1 + 1)):
+            t = R(i - 1) * R(time)
+            if t <= td[0]:
+                ut[q], ud[q] = uv[0], dv[0]
+            elif t >= td[-1]:
+                ut[q], ud[q] = uv[-1], dv[-1]
+            else:
+                for m in range(len(td) - 1):
+                    if td[m] <= t <= td[m + 1]:
+                        s = (t - td[m]) / (td[m + 1] - td[m])
+                        ut[q] = uv[m] + s * (uv[m + 1] - uv[m])
+                        ud[q] = dv[m] + s * (dv[m + 1] - dv[m])
+                        break
+    return shear, ut, ud

@@ -144,6 +144,40 @@ class OracleSim:
         return self.f1
 
 
+def inflow_series(files, nz, nt0, nt1, uini, udir, prec="f32"):
+    """src/m_inflow.F90:27-86 on the texts of uvel_shear.dat / uvel_time.dat (``files``: name ->
+    text, either may be absent) with p2l%vel = p2l%time = 1 (C_l = C_u = 1 in infile_text).
+    Returns (uvel_shear[nz], uvel_time[0..nt1], udir_time[0..nt1]) indexed by absolute step, the
+    form OracleSim.run takes; every operation rounds in the build's REAL kind."""
+    R = np.float32 if prec == "f32" else np.float64
+    vals = lambda txt: [[R(x) for x in line.replace(",", " ").split()] for line in txt.splitlines() if line.strip()]
+    uvel_h = np.ones(nz, R)
+    if "uvel_shear.dat" in files:
+        for k, row in enumerate(vals(files["uvel_shear.dat"])[:nz]):
+            uvel_h[k] = row[2]                        # read(10,*) kk, z(k), uvel_h(k)
+    shear = np.array([uvel_h[k] / uvel_h[nz - 1] for k in range(nz)], R)
+    ut = np.full(nt1 + 1, R(uini), R)
+    ud = np.full(nt1 + 1, R(udir), R)
+    if "uvel_time.dat" in files:
+        rows = vals(files["uvel_time.dat"])
+        tdata = [r[0] for r in rows]
+        utd = [r[1] / R(1.0) for r in rows]            # uvel_tdata(k) / p2l%vel
+        udd = [r[2] for r in rows]
+        last = len(rows) - 1
+        for i in range(nt0, nt1 + 1):
+            t = R(i - 1) * R(1.0)                       # real(i-1)*p2l%time
+            if t <= tdata[0]:
+                ut[i], ud[i] = utd[0], udd[0]
+            elif t >= tdata[last]:
+                ut[i], ud[i] = utd[last], udd[last]
+            else:
+                k = next(k for k in range(last) if tdata[k] <= t <= tdata[k + 1])
+                tmp = (t - tdata[k]) / (tdata[k + 1] - tdata[k])
+                ut[i] = utd[k] + tmp * (utd[k + 1] - utd[k])
+                ud[i] = udd[k] + tmp * (udd[k + 1] - udd[k])
+    return shear, ut, ud
+
+
 # ------------------------------------------------------------------------------------------
 # Reference build runner
 # ------------------------------------------------------------------------------------------
@@ -179,10 +213,11 @@ def infile_text(*, ibgk=2, ihrr=0, ivreman=0, smagorinsky=0.0, nt0=1, nt1=2, ibn
 
 
 def run_reference(exe, f_in, *, nt0, nt1, turb_fields=None, workdir=None, threads=1, timeout=3600,
-                  read_output=True, **opts):
+                  read_output=True, files=None, **opts):
     """Run a reference executable (built by oracle/build_ref.py) from restart state ``f_in``
     at step nt0 to nt1.  Returns (f_out, stdout).  f_out is the restart record written at
-    nt1 (post-streaming f1 with ghosts, src/main.F90:398)."""
+    nt1 (post-streaming f1 with ghosts, src/main.F90:398).  ``files``: name -> text of input
+    files written into the working directory first (uvel_shear.dat, uvel_time.dat)."""
     import sys
     sys.path.insert(0, os.path.dirname(HERE))
     from lbm_amd import fortran_io as fio
@@ -195,6 +230,9 @@ def run_reference(exe, f_in, *, nt0, nt1, turb_fields=None, workdir=None, thread
             uu, vv, ww, rr = turb_fields
             fio.write_turbulence(os.path.join(wd, "restart", fio.restart_name(nt0, prefix="turbulence")),
                                  uu, vv, ww, rr)
+        for fname, text in (files or {}).items():
+            with open(os.path.join(wd, fname), "w") as fh:
+                fh.write(text)
         with open(os.path.join(wd, "infile.in"), "w") as fh:
             fh.write(infile_text(nt0=nt0, nt1=nt1, turb=turb_fields is not None, **opts))
         env = dict(os.environ, OMP_NUM_THREADS=str(threads))

@@ -52,7 +52,25 @@ CASES = {
     # inflow at 20 degrees, free-slip j, no-slip k
     "q19_bgk_inflow_dir": (19, "sp", (12, 8, 6), 20, 6, "none", 0,
                            dict(ibgk=2, ibnd=1, jbnd=22, kbnd=11, uini=0.03, udir=20.0, visc=1.0 / 30.0)),
+    # inflow profile and time series read from uvel_shear.dat / uvel_time.dat (FILES below)
+    "q27_bgk3_inflow_series": (27, "sp", (16, 8, 6), 20, 8, "none", 0,
+                               dict(ibgk=3, ihrr=0, ivreman=0, ibnd=1, jbnd=0, kbnd=11, uini=0.05, udir=0.0,
+                                    visc=1.0 / 30.0)),
 }
+
+# Input files of a case, written into the reference's working directory (src/m_inflow.F90:27-86
+# reads them): a power-law profile over the nz = 6 cells, and a series of speeds and directions
+# whose rows start after step 1 (t = 0), interpolate in between and end before the last step.
+FILES = {
+    "q27_bgk3_inflow_series": {
+        "uvel_shear.dat": "".join(f"{k:4d} {k - 0.5:8.3f} {((k - 0.5) / 6) ** 0.2:.7f}\n" for k in range(1, 7)),
+        "uvel_time.dat": "   3.0  0.020   5.0\n   9.5  0.050 -10.0\n  14.0  0.035  30.0\n  17.0  0.040  30.0\n",
+    },
+}
+
+
+def case_files(name):
+    return FILES.get(name, {})
 
 
 def case_inputs(name):

@@ -17,7 +17,7 @@ sys.path.insert(0, HERE)
 
 import build_ref  # noqa: E402
 import oracle  # noqa: E402
-from cases import CASES, case_inputs, fixture_path, sha  # noqa: E402
+from cases import CASES, case_files, case_inputs, fixture_path, sha  # noqa: E402
 
 
 def make(name):
@@ -29,9 +29,9 @@ def make(name):
         opts["nrturb"] = nrturb
         opts.setdefault("ampl", 0.0)
     f_out, _ = oracle.run_reference(exe, f0, nt0=1, nt1=1 + steps, turb_fields=turb,
-                                    experiment=exp, **opts)
+                                    experiment=exp, files=case_files(name), **opts)
     meta = dict(name=name, q=q, prec=prec, dims=list(dims), steps=steps, seed=seed, experiment=exp,
-                nrturb=nrturb, options=o, nt0=1, input_sha256=sha(f0),
+                nrturb=nrturb, options=o, nt0=1, input_sha256=sha(f0), files=case_files(name),
                 producer="reference build (amdflang) via oracle/build_ref.py")
     arrays = dict(meta=np.array(json.dumps(meta)), f_out=f_out)
     if turb is not None:

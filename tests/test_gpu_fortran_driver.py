@@ -21,7 +21,7 @@ BUILD = os.path.join(ROOT, "lbm_amd", "_build")
 
 
 @pytest.mark.parametrize("name", ["q27_regvre_inflow_cyl", "q27_regvre_inflow_dp", "q19_bgk_inflow_dir",
-                                  "q27_regvre_closed"])
+                                  "q27_regvre_closed", "q27_bgk3_inflow_series"])
 def test_fortran_driver_matches_reference_build(name, tmp_path):
     q, prec, dims, steps, seed, exp, nrturb, o = cases.CASES[name]
     exe = os.path.join(BUILD, "lbm_driver_" + ("f32" if prec == "sp" else "f64"))
@@ -37,6 +37,8 @@ def test_fortran_driver_matches_reference_build(name, tmp_path):
         opts["nrturb"] = nrturb
         opts.setdefault("ampl", 0.0)
         fio.write_turbulence(str(tmp_path / "restart" / fio.restart_name(nt0, prefix="turbulence")), *turb)
+    for fname, text in cases.case_files(name).items():   # uvel_shear.dat, uvel_time.dat
+        (tmp_path / fname).write_text(text)
     (tmp_path / "infile.in").write_text(
         oracle.infile_text(nt0=nt0, nt1=nt1, turb=bool(nrturb), experiment=exp, **opts))
     print(f"driver {os.path.basename(exe)} case {name} dims={dims} steps={steps}", flush=True)

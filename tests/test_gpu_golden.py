@@ -5,6 +5,8 @@ multiply-adds, so its rounding differs from the reference's dense Hermite contra
 the 12-100 steps of a fixture the populations must agree to 2e-6 absolute (f is O(0.3)) and the
 moments to a relative 1e-5 (density by max|rho|, velocities by the largest speed).  FP64: 1e-12.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -33,6 +35,8 @@ def rel_errors(f, ref, q):
 
 
 def run_case(name):
+    import tempfile
+    from lbm_amd import fortran_io
     from lbm_amd.domain import Domain
     meta, fx = cases.load(name)
     q, prec, dims, steps, seed, exp, nrturb, o = cases.CASES[name]
@@ -42,7 +46,16 @@ def run_case(name):
                  ivreman=o.get("ivreman", 0), smagorinsky=o.get("smagorinsky", 0.0),
                  kinevisc=o.get("visc", 1 / 30.0), ibnd=o["ibnd"], jbnd=o["jbnd"], kbnd=o["kbnd"])
     dom.set_state(f0)
-    dom.set_inflow(o.get("uini", 0.0), o.get("udir", 0.0))
+    # the inflow files of the case read by the library's reader of src/m_inflow.F90
+    with tempfile.TemporaryDirectory() as wd:
+        for fname, text in cases.case_files(name).items():
+            with open(os.path.join(wd, fname), "w") as fh:
+                fh.write(text)
+        dt = np.float32 if prec == "sp" else np.float64
+        shear, ut, ud = fortran_io.read_inflow(wd, dims[2], 1, 1 + steps, o.get("uini", 0.0), o.get("udir", 0.0),
+                                               dtype=dt)
+    dom.set_inflow(o.get("uini", 0.0), o.get("udir", 0.0), shear=shear)
+    dom.set_inflow_series(1, ut, ud)
     bl = cases.blanking(name)
     if bl is not None:
         dom.set_blanking(bl)

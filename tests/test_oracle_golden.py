@@ -24,8 +24,33 @@ def test_oracle_matches_reference_build(name):
     tb = None
     if nrturb:
         tb = (fx["uu"], fx["vv"], fx["ww"], nrturb, float(dt(o.get("ampl", 0.0))))
-    f = sim.run(1, 1 + steps, uini=float(dt(o.get("uini", 0.0))), udir=float(dt(o.get("udir", 0.0))),
-                blank=cases.blanking(name), turb=tb)
+    uini, udir = float(dt(o.get("uini", 0.0))), float(dt(o.get("udir", 0.0)))
+    shear, ut, ud = oracle.inflow_series(cases.case_files(name), dims[2], 1, 1 + steps, uini, udir,
+                                         "f32" if prec == "sp" else "f64")
+    f = sim.run(1, 1 + steps, uvel_shear=shear, uvel_time=ut, udir_time=ud, blank=cases.blanking(name), turb=tb)
     ref = fx["f_out"]
     assert f.dtype == ref.dtype and f.shape == ref.shape
     assert np.array_equal(f, ref), f"max diff {np.abs(f - ref).max()}"
+
+
+def test_inflow_files_drive_the_golden_case(tmp_path):
+    """The profile and series of q27_bgk3_inflow_series reach the result: the same run with the
+    constant inflow of infile.in does not reproduce the golden record.  The product's reader of
+    the two files (lbm_amd.fortran_io.read_inflow) gives the oracle's values bit for bit."""
+    from lbm_amd import fortran_io
+    name = "q27_bgk3_inflow_series"
+    meta, fx = cases.load(name)
+    q, prec, dims, steps, seed, exp, nrturb, o = cases.CASES[name]
+    files = cases.case_files(name)
+    assert meta["files"] == files
+    f0, _ = cases.case_inputs(name)
+    shear, ut, ud = oracle.inflow_series(files, dims[2], 1, 1 + steps, o["uini"], o["udir"])
+    assert len(set(ut[1:].tolist())) > 5 and len(set(ud[1:].tolist())) > 5 and shear[-1] == 1 and shear[0] < 0.8
+    for fname, text in files.items():
+        (tmp_path / fname).write_text(text)
+    ps, pt, pd = fortran_io.read_inflow(str(tmp_path), dims[2], 1, 1 + steps, o["uini"], o["udir"])
+    assert np.array_equal(ps, shear) and np.array_equal(pt, ut[1:]) and np.array_equal(pd, ud[1:])
+    sim = oracle.OracleSim(q, dims, "f32", ibgk=3, kinevisc=float(np.float32(o["visc"])), ibnd=1, jbnd=0, kbnd=11)
+    sim.restart(f0)
+    f = sim.run(1, 1 + steps, uini=float(np.float32(o["uini"])))
+    assert np.abs(f - fx["f_out"]).max() > 1e-4

@@ -251,3 +251,76 @@ def test_taylor_green_viscosity():
     e1 = energy()
     expect = np.exp(-4 * nu * k * k * steps)
     assert abs(e1 / e0 / expect - 1) < 1e-2   # O(k^2) lattice dispersion; a wrong nu(tau) is off by far more
+
+
+def test_fillghosts_is_the_clamped_copy():
+    """or_fillghosts (src/m_fillghosts.F90:19-26) on a ragged tiny grid of distinct values: every
+    ghost holds its clamped interior cell, i.e. numpy's edge padding of the interior, and the
+    interior is untouched."""
+    for q, dims in ((19, (3, 2, 4)), (27, (2, 5, 3))):
+        sim = oracle.OracleSim(q, dims, "f64")
+        rng = np.random.default_rng(q)
+        f = rng.random(sim.fA.shape)
+        inner = f[1:-1, 1:-1, 1:-1].copy()
+        sim.lib.or_fillghosts(C.byref(sim.p), oracle._ptr(f))
+        expect = np.pad(inner, ((1, 1), (1, 1), (1, 1), (0, 0)), mode="edge")
+        assert np.array_equal(f, expect)
+
+
+def feq2_np(lat, rho, u):
+    """Second-order equilibrium w rho (1 + 3 c.u + 4.5 (c.u)^2 - 1.5 u.u), FP64, per cell."""
+    cu = np.einsum("...p,lp->...l", u, lat.c.astype(float))
+    usq = (u * u).sum(-1)[..., None]
+    return lat.w * rho[..., None] * (1 + 3 * cu + 4.5 * cu * cu - 1.5 * usq)
+
+
+def feq3_np(lat, rho, u):
+    """feq2 plus the third-order term 4.5 w rho c.u ((c.u)^2 - u.u) (src/m_postcoll_kernel.F90)."""
+    cu = np.einsum("...p,lp->...l", u, lat.c.astype(float))
+    usq = (u * u).sum(-1)[..., None]
+    return feq2_np(lat, rho, u) + 4.5 * lat.w * rho[..., None] * cu * (cu * cu - usq)
+
+
+def forcing_moment_errors(df, rho, u, F, lat):
+    """Deviations of the increment df from what feq2(rho, v) - feq2(rho, u) with v = u - F/rho
+    fixes (src/m_forcings_apply_kernel.F90:142-207): second moments rho (v v - u u), the third
+    moments m_xxy = rho (v_y - u_y)/3 = -F_y/3 (any third-order term adds rho (v_x^2 v_y - u_x^2 u_y))
+    and, on D3Q27, m_xyz = 0 (a third-order term adds rho (v_x v_y v_z - u_x u_y u_z))."""
+    c = lat.c.astype(float)
+    v = u - F / rho[..., None]
+    pi = np.einsum("...l,lp,lq->...pq", df, c, c)
+    pi_expect = rho[..., None, None] * (np.einsum("...p,...q->...pq", v, v) - np.einsum("...p,...q->...pq", u, u))
+    m_xxy = np.einsum("...l,l->...", df, c[:, 0] ** 2 * c[:, 1])
+    m_xyz = np.einsum("...l,l->...", df, c[:, 0] * c[:, 1] * c[:, 2])
+    return (np.abs(pi - pi_expect).max() / np.abs(pi_expect).max(),
+            np.abs(m_xxy + F[..., 1] / 3).max() / np.abs(F[..., 1] / 3).max(),
+            np.abs(m_xyz).max() / np.abs(F).max())
+
+
+@pytest.mark.parametrize("q", [19, 27])
+def test_forcing_uses_the_second_order_equilibrium(q):
+    """The moments of the forcing increment pin the second-order equilibrium (also with ibgk = 3,
+    README.md:81): a slip to the third-order form, planted here in numpy, fails the same check."""
+    lat = lattice(q)
+    sim = oracle.OracleSim(q, (3, 2, 2), "f64", ibgk=3)
+    rng = np.random.default_rng(5)
+    sim.fA[...] = lat.w * (1 + 0.01 * rng.standard_normal(sim.fA.shape))
+    sim.rho[...] = 1.0 + 0.05 * rng.random(sim.rho.shape)
+    sim.u[...], sim.v[...], sim.w[...] = (0.05 * rng.standard_normal(sim.u.shape) for _ in range(3))
+    F = 1e-4 * rng.standard_normal(sim.rho.shape + (3,))
+    before = sim.fA.copy()
+    sim.lib.or_forcings_apply(C.byref(sim.p), oracle._ptr(sim.fA), oracle._ptr(F), oracle._ptr(sim.rho),
+                              oracle._ptr(sim.u), oracle._ptr(sim.v), oracle._ptr(sim.w))
+    inner = (slice(1, -1),) * 3
+    df = (sim.fA - before)[inner]
+    rho = sim.rho[inner]
+    u = np.stack([sim.u[inner], sim.v[inner], sim.w[inner]], -1)
+    Fi = F[inner]
+    e_pi, e_xxy, e_xyz = forcing_moment_errors(df, rho, u, Fi, lat)
+    assert e_pi < 1e-9 and e_xxy < 1e-8 and e_xyz < 1e-12
+    # the planted slip: feq3 in place of feq2
+    v = u - Fi / rho[..., None]
+    e3 = forcing_moment_errors(feq3_np(lat, rho, v) - feq3_np(lat, rho, u), rho, u, Fi, lat)
+    assert e3[1] > 1e-4 or e3[2] > 1e-4
+    # and the restated increment itself agrees with numpy's feq2 difference
+    assert np.allclose(df, feq2_np(lat, rho, v) - feq2_np(lat, rho, u), rtol=0, atol=1e-15)
