@@ -221,3 +221,15 @@ def read_inflow(directory: str, nz: int, nt0: int, nt1: int, uini: float, udir: 
                         ud[q] = dv[m] + s * (dv[m + 1] - dv[m])
                         break
     return shear, ut, ud
+
+
+def write_theta(path: str, theta, dtype=np.float32, subrecords: bool = False) -> None:
+    """restart/theta_<tile>_<it>.uf: the rotor azimuths theta(nturbines) [rad], one record
+    (src/m_saverestart.F90:118-122, read back by src/m_readrestart.F90:103-116)."""
+    with open(path, "wb") as fh:
+        write_record(fh, np.ascontiguousarray(theta, dtype=dtype).tobytes(), subrecords)
+
+
+def read_theta(path: str, dtype=np.float32) -> np.ndarray:
+    with open(path, "rb") as fh:
+        return np.frombuffer(read_record(fh), dtype=dtype).copy()

@@ -16,6 +16,7 @@
  *   inflow_turbulence_forcing  src/m_inflow_turbulence_forcing.F90:5-135
  *   fequil3_block_kernel       src/m_fequil3_block_kernel.F90:7-117
  *   inflow_turbulence_apply    src/m_inflow_turbulence_apply.F90:4-31
+ *   turbine_forcing (CPU path) src/m_turbine_forcing.F90:8-140 and the routines it calls
  *   fequil3_kernel             src/m_fequil3_kernel.F90:7-124 (initialisation)
  *   fillghosts                 src/m_fillghosts.F90:4-28 (initialisation)
  * Array layout is the reference's: f(nl,0:nx+1,0:ny+1,0:nz+1), column major, so element
@@ -347,6 +348,163 @@ OR_EXPORT void or_forcings_apply(const or_params *P, REAL *f, const REAL *F, con
  * slice i=ip from the pre-collision moments (src/m_inflow_turbulence_forcing.F90:57-128).
  * uu,vv,ww here are the (ny,nz) slices for lit = mod(it,nrturb) (0 -> nrturb), column major.
  * df is turbulence_df(nl,ny,nz). */
+/* ---------------------------------- turbines ---------------------------------- */
+/* The actuator-line turbines of src/m_turbine_forcing.F90:8-140 on the CPU path the reference's
+ * serial/OpenMP build takes (no MPI, no _CUDA): azimuth update, point distribution
+ * (src/m_turbine_distribute_points.F90:6-64, basis src/m_turbine_rotor_basis.F90:13-23), point
+ * forces (src/m_turbine_point_forces_kernel.F90:5-68 with src/m_turbine_interpolation.F90:25-114,
+ * src/m_nrelliftdrag.F90:7-35, src/m_turbine_compute_bladeforce.F90:20-65) and the Gaussian deposit
+ * (src/m_turbine_deposit.F90:7-91), every operation in the reference's order. */
+typedef struct {
+    int nturb, nblades, nchords, nfoils, nang_max;
+    const REAL *xhub, *yhub, *zhub, *yaw, *tilt, *omegand, *pitch;   /* [nturb] */
+    REAL *theta;                                                      /* [nturb], advanced per step */
+    const REAL *relm, *dc, *chord, *twist;                            /* [nchords] */
+    const int *nfoil;                                                 /* [nchords], 1-based */
+    const int *nang;                                                  /* [nfoils] */
+    const REAL *foil;                                                 /* [nfoils][nang_max][3]: deg, cl, cd */
+} or_turbines;
+
+/* math in the REAL kind (cosf for float, as the FP32 build calls it; cos for double) */
+#define RCOS(x) _Generic((x), float: cosf, default: cos)(x)
+#define RSIN(x) _Generic((x), float: sinf, default: sin)(x)
+#define REXP(x) _Generic((x), float: expf, default: exp)(x)
+#define RFLOOR(x) _Generic((x), float: floorf, default: floor)(x)
+#define RCEIL(x) _Generic((x), float: ceilf, default: ceil)(x)
+#define RSQRT(x) _Generic((x), float: sqrtf, default: sqrt)(x)
+#define RATAN2(y, x) _Generic((y), float: atan2f, default: atan2)(y, x)
+
+static const REAL OR_PI = (REAL)3.14159265358979323846;   /* src/mod_turbines.F90:10-11 */
+
+static void rotor_basis(REAL yaw, REAL tilt, REAL ea[3], REAL e1[3], REAL e2[3])
+{
+    ea[0] = RCOS(yaw) * RCOS(tilt); ea[1] = RSIN(yaw) * RCOS(tilt); ea[2] = -RSIN(tilt);
+    e1[0] = -RSIN(yaw);            e1[1] = RCOS(yaw);             e1[2] = (REAL)0.0;
+    e2[0] = -RCOS(yaw) * RSIN(tilt); e2[1] = -RSIN(yaw) * RSIN(tilt); e2[2] = -RCOS(tilt);
+}
+
+static REAL trilinear(REAL c000, REAL c100, REAL c010, REAL c110, REAL c001, REAL c101, REAL c011, REAL c111,
+                      REAL fx, REAL fy, REAL fz)
+{
+    REAL one = (REAL)1;
+    return c000 * (one - fx) * (one - fy) * (one - fz) + c100 * fx * (one - fy) * (one - fz) +
+           c010 * (one - fx) * fy * (one - fz) + c110 * fx * fy * (one - fz) +
+           c001 * (one - fx) * (one - fy) * fz + c101 * fx * (one - fy) * fz +
+           c011 * (one - fx) * fy * fz + c111 * fx * fy * fz;
+}
+
+static REAL interp(const or_params *P, const REAL *a, int i, int j, int k, REAL fx, REAL fy, REAL fz)
+{
+    return trilinear(a[CIDX(P, i, j, k)], a[CIDX(P, i + 1, j, k)], a[CIDX(P, i, j + 1, k)], a[CIDX(P, i + 1, j + 1, k)],
+                     a[CIDX(P, i, j, k + 1)], a[CIDX(P, i + 1, j, k + 1)], a[CIDX(P, i, j + 1, k + 1)],
+                     a[CIDX(P, i + 1, j + 1, k + 1)], fx, fy, fz);
+}
+
+static int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+/* F (3,0:nx+1,0:ny+1,0:nz+1) receives the turbine forcing of this step (the deposit; zero elsewhere) */
+OR_EXPORT void or_turbine_forcing(const or_params *P, const or_turbines *T, const REAL *rho, const REAL *u,
+                                  const REAL *v, const REAL *w, REAL *F)
+{
+    const REAL pi2 = (REAL)2.0 * OR_PI;
+    for (int t = 0; t < T->nturb; t++) T->theta[t] = T->theta[t] + T->omegand[t];
+    int np = T->nturb * T->nblades * T->nchords;
+    REAL *pts = (REAL *)malloc(sizeof(REAL) * 6 * (size_t)(np > 0 ? np : 1));   /* xg, yg, zg, F */
+    int p = 0;
+    for (int t = 0; t < T->nturb; t++) {
+        REAL ea[3], e1[3], e2[3];
+        rotor_basis(T->yaw[t], T->tilt[t], ea, e1, e2);
+        for (int ib = 1; ib <= T->nblades; ib++) {
+            REAL theta = T->theta[t] + (REAL)(ib - 1) * pi2 / (REAL)T->nblades;
+            for (int ic = 0; ic < T->nchords; ic++, p++) {
+                REAL er[3];
+                for (int d = 0; d < 3; d++) er[d] = RCOS(theta) * e1[d] + RSIN(theta) * e2[d];
+                REAL xg = T->xhub[t] + T->relm[ic] * er[0];
+                REAL yg = T->yhub[t] + T->relm[ic] * er[1];
+                REAL zg = T->zhub[t] + T->relm[ic] * er[2];
+                /* turbine_interpolate_velocity, j_start = 1 */
+                int ig = (int)RFLOOR(xg), jg = (int)RFLOOR(yg), kg = (int)RFLOOR(zg);
+                REAL fx = xg - (REAL)ig, fy = yg - (REAL)jg, fz = zg - (REAL)kg;
+                int i = clampi(ig, 1, P->nx - 1), j = clampi(jg, 1, P->ny - 1), k = clampi(kg, 1, P->nz - 1);
+                REAL ux = interp(P, u, i, j, k, fx, fy, fz), uy = interp(P, v, i, j, k, fx, fy, fz);
+                REAL uz = interp(P, w, i, j, k, fx, fy, fz), dens = interp(P, rho, i, j, k, fx, fy, fz);
+                REAL costheta = RCOS(theta), sintheta = RSIN(theta);
+                REAL utheta = T->omegand[t] * T->relm[ic] - uz * costheta - uy * sintheta;
+                REAL phi = RATAN2(ux, utheta);
+                REAL ang = phi * (REAL)180.0 / OR_PI - T->twist[ic] - T->pitch[t];
+                /* nrelliftdrag */
+                REAL cl = 0, cd = 0;
+                int kf = T->nfoil[ic] - 1;
+                const REAL *tab = T->foil + (size_t)kf * T->nang_max * 3;
+                for (int a = 0; a < T->nang[kf] - 1; a++) {
+                    if (tab[3 * a] < ang && ang <= tab[3 * (a + 1)]) {
+                        REAL slope = (tab[3 * (a + 1) + 1] - tab[3 * a + 1]) / (tab[3 * (a + 1)] - tab[3 * a]);
+                        cl = tab[3 * a + 1] + slope * (ang - tab[3 * a]);
+                        slope = (tab[3 * (a + 1) + 2] - tab[3 * a + 2]) / (tab[3 * (a + 1)] - tab[3 * a]);
+                        cd = tab[3 * a + 2] + slope * (ang - tab[3 * a]);
+                        break;
+                    }
+                }
+                /* turbine_compute_bladeforce */
+                REAL et[3];
+                for (int d = 0; d < 3; d++) { er[d] = RCOS(theta) * e1[d] + RSIN(theta) * e2[d]; et[d] = -RSIN(theta) * e1[d] + RCOS(theta) * e2[d]; }
+                REAL speed2 = ux * ux + utheta * utheta;
+                REAL area = T->chord[ic] * T->dc[ic];
+                REAL tiploss = (REAL)1.0;
+                REAL Lf = (REAL)0.5 * dens * speed2 * cl * area * tiploss;
+                REAL Df = (REAL)0.5 * dens * speed2 * cd * area * tiploss;
+                REAL ph = RATAN2(ux, utheta), sinphi = RSIN(ph), cosphi = RCOS(ph);
+                pts[6 * p] = xg; pts[6 * p + 1] = yg; pts[6 * p + 2] = zg;
+                for (int d = 0; d < 3; d++)
+                    pts[6 * p + 3 + d] = Lf * (cosphi * ea[d] + sinphi * et[d]) - Df * (sinphi * ea[d] - cosphi * et[d]);
+            }
+        }
+    }
+    /* turbine_deposit into F_turb = 0, then external_forcing = F_turb */
+    size_t ncell = (size_t)(P->nx + 2) * (P->ny + 2) * (P->nz + 2);
+    memset(F, 0, sizeof(REAL) * 3 * ncell);
+    REAL epsilon = (REAL)2.0;
+    REAL sigma = epsilon / (REAL)RSQRT((REAL)2.0);
+    int krad = (int)RCEIL((REAL)3.0 * sigma);
+    if (krad > 5) krad = 5;
+    for (p = 0; p < np; p++) {
+        REAL xg = pts[6 * p], yg = pts[6 * p + 1], zg = pts[6 * p + 2];
+        REAL Fp[3] = {pts[6 * p + 3], pts[6 * p + 4], pts[6 * p + 5]};
+        if (Fp[0] == (REAL)0 && Fp[1] == (REAL)0 && Fp[2] == (REAL)0) continue;
+        int i0 = (int)RFLOOR(xg), j0 = (int)RFLOOR(yg), k0 = (int)RFLOOR(zg);
+        int ii1 = i0 - krad > 1 ? i0 - krad : 1, ii2 = i0 + krad < P->nx ? i0 + krad : P->nx;
+        int jj1 = j0 - krad > 1 ? j0 - krad : 1, jj2 = j0 + krad < P->ny ? j0 + krad : P->ny;
+        int kk1 = k0 - krad > 1 ? k0 - krad : 1, kk2 = k0 + krad < P->nz ? k0 + krad : P->nz;
+        REAL sumW = 0;
+        for (int kk = kk1; kk <= kk2; kk++) {
+            REAL dz = (REAL)kk - zg;
+            for (int jj = jj1; jj <= jj2; jj++) {
+                REAL dy = (REAL)jj - yg;
+                for (int ii = ii1; ii <= ii2; ii++) {
+                    REAL dx = (REAL)ii - xg;
+                    sumW = sumW + REXP(-(dx * dx + dy * dy + dz * dz) / ((REAL)2.0 * sigma * sigma));
+                }
+            }
+        }
+        if (sumW < (REAL)1.0e-6) continue;
+        for (int kk = kk1; kk <= kk2; kk++) {
+            REAL dz = (REAL)kk - zg;
+            for (int jj = jj1; jj <= jj2; jj++) {
+                REAL dy = (REAL)jj - yg;
+                for (int ii = ii1; ii <= ii2; ii++) {
+                    REAL dx = (REAL)ii - xg;
+                    REAL weight = REXP(-(dx * dx + dy * dy + dz * dz) / ((REAL)2.0 * sigma * sigma)) / sumW;
+                    size_t c = CIDX(P, ii, jj, kk);
+                    F[3 * c] = F[3 * c] + weight * Fp[0];
+                    F[3 * c + 1] = F[3 * c + 1] + weight * Fp[1];
+                    F[3 * c + 2] = F[3 * c + 2] + weight * Fp[2];
+                }
+            }
+        }
+    }
+    free(pts);
+}
+
 OR_EXPORT void or_turbulence_forcing(const or_params *P, const REAL *rho, const REAL *u, const REAL *v,
                                      const REAL *w, const REAL *uu, const REAL *vv, const REAL *ww,
                                      REAL ampl, REAL *df)
@@ -666,15 +824,22 @@ OR_EXPORT int or_run(const or_params *P, REAL *fA, REAL *fB, REAL *tau, REAL *rh
                      const REAL *ablvisc, const uint8_t *blank,
                      const REAL *uvel_shear, const REAL *uvel_time, const REAL *udir_time,
                      const REAL *uu, const REAL *vv, const REAL *ww, int nrturb, REAL ampl,
-                     const REAL *F, int apply_forcing, int it0, int it1)
+                     const REAL *F, int apply_forcing, int it0, int it1, const or_turbines *T)
 {
     REAL *f1 = fA, *f2 = fB;
+    REAL *Ft = NULL;   /* external_forcing of the turbines (src/main.F90:291-292) */
+    if (T) {
+        Ft = (REAL *)malloc(sizeof(REAL) * 3 * (size_t)(P->nx + 2) * (P->ny + 2) * (P->nz + 2));
+        F = Ft;
+        apply_forcing = 1;
+    }
     REAL *uvel = (REAL *)malloc(sizeof(REAL) * (size_t)P->nz);
     REAL *df = NULL;
     size_t sl = (size_t)P->ny * (size_t)P->nz;
     if (uu) df = (REAL *)malloc(sizeof(REAL) * sl * (size_t)P->nl);
     int swapped = 0;
     for (int it = it0 + 1; it <= it1; it++) {
+        if (T) or_turbine_forcing(P, T, rho, u, v, w, Ft);
         if (uu) {
             int lit = it % nrturb;
             if (lit == 0) lit = nrturb;
@@ -693,5 +858,6 @@ OR_EXPORT int or_run(const or_params *P, REAL *fA, REAL *fB, REAL *tau, REAL *rh
     }
     free(uvel);
     free(df);
+    free(Ft);
     return swapped;
 }

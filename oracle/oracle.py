@@ -37,6 +37,53 @@ def _params_type(real):
 
 _LIBS = {}
 
+# NREL 5-MW blade sections (src/mod_nrel5mw.F90:24-48): radial position, segment length and chord
+# [m], twist [deg], airfoil index (1-based into NREL5MW_FOILS); rotor and hub radius [m].
+NREL5MW_RELM = [2.86670, 5.60000, 8.33330, 11.75000, 15.85000, 19.95000, 24.05000, 28.15000, 32.25000,
+                36.35000, 40.45000, 44.55000, 48.65000, 52.75000, 56.16670, 58.90000, 61.63330]
+NREL5MW_DC = [2.73330, 2.73330, 2.73330] + [4.10000] * 11 + [2.73330, 2.73330, 2.73330]
+NREL5MW_CHORD = [3.54200, 3.85400, 4.16700, 4.55700, 4.65200, 4.45800, 4.24900, 4.00700, 3.74800, 3.50200,
+                 3.25600, 3.01000, 2.76400, 2.51800, 2.31300, 2.08600, 1.41900]
+NREL5MW_TWIST = [13.3080, 13.3080, 13.3080, 13.3080, 11.4800, 10.1620, 9.0110, 7.7950, 6.5440, 5.3610, 4.1880,
+                 3.1250, 2.3190, 1.5260, 0.8630, 0.3700, 0.1060]
+NREL5MW_NFOIL = [1, 1, 2, 3, 4, 4, 5, 6, 6, 7, 7, 8, 8, 8, 8, 8, 8]
+NREL5MW_FOILS = ["Cylinder1.dat", "Cylinder2.dat", "DU40_A17.dat", "DU35_A17.dat", "DU30_A17.dat", "DU25_A17.dat",
+                 "DU21_A17.dat", "NACA64_A17.dat"]   # src/m_nrelreadfoil.F90:13-20
+
+
+def read_foil_text(text, prec="f32"):
+    """One airfoil table as src/m_nrelreadfoil.F90:33-47 reads it: 50 lines skipped, the row count,
+    2 lines skipped, then rows alpha [deg], cl, cd, cm.  Returns an (n, 3) array of deg, cl, cd."""
+    R = np.float32 if prec == "f32" else np.float64
+    lines = text.splitlines()
+    n = int(lines[50].split()[0])
+    rows = [[R(x) for x in lines[53 + r].replace(",", " ").split()[:3]] for r in range(n)]
+    return np.array(rows, R)
+
+
+def turbines_from_infile(pos, rpm, pitch, length, vel, foils, prec="f32", theta=None, nblades=3):
+    """turbine_initialize (src/m_turbine_initialize.F90:29-70) on infile values: pos = [(ipos, jpos,
+    kpos, yaw_deg, tilt_deg)], rpm, pitch [deg], C_l = length [m], C_u = vel [m/s]; foils = list of
+    (n, 3) tables in NREL5MW_FOILS order.  Every conversion rounds in the REAL kind.  Returns the dict
+    OracleSim.run(turbines=...) takes; theta (rad) defaults to zeros."""
+    R = np.float32 if prec == "f32" else np.float64
+    pi = R(3.14159265358979323846)
+    pi2 = R(2.0) * pi
+    length, vel = R(length), R(vel)
+    p2l_time = length / vel
+    omega = pi2 * R(rpm) / R(60.0)
+    n = len(pos)
+    return dict(
+        xhub=np.array([R(p[0]) for p in pos], R), yhub=np.array([R(p[1]) for p in pos], R),
+        zhub=np.array([R(p[2]) for p in pos], R),
+        yaw=np.array([(R(p[3]) / R(360.0)) * pi2 for p in pos], R),
+        tilt=np.array([(R(p[4]) / R(360.0)) * pi2 for p in pos], R),
+        omegand=np.full(n, omega * p2l_time, R), pitch=np.full(n, R(pitch), R),
+        theta=np.zeros(n, R) if theta is None else np.asarray(theta, R).copy(),
+        relm=np.array([R(x) / length for x in NREL5MW_RELM], R), dc=np.array([R(x) / length for x in NREL5MW_DC], R),
+        chord=np.array([R(x) / length for x in NREL5MW_CHORD], R), twist=np.array(NREL5MW_TWIST, R),
+        nfoil=np.array(NREL5MW_NFOIL, np.int32), foils=[np.asarray(f, R) for f in foils], nblades=nblades)
+
 
 def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
@@ -53,6 +100,32 @@ def _lib(prec):
 
 def _ptr(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _turbines_struct(real, T, keep):
+    """ctypes image of or_turbines for the dict T (arrays kept alive in ``keep``)."""
+    class S(C.Structure):
+        _fields_ = [(n, C.c_int) for n in ("nturb", "nblades", "nchords", "nfoils", "nang_max")] + \
+                   [(n, C.c_void_p) for n in ("xhub", "yhub", "zhub", "yaw", "tilt", "omegand", "pitch", "theta",
+                                              "relm", "dc", "chord", "twist", "nfoil", "nang", "foil")]
+    dt = np.float32 if real is C.c_float else np.float64
+    nmax = max(len(f) for f in T["foils"])
+    foil = np.zeros((len(T["foils"]), nmax, 3), dt)
+    for k, f in enumerate(T["foils"]):
+        foil[k, :len(f)] = f
+    arr = {k: np.ascontiguousarray(T[k], dt) for k in ("xhub", "yhub", "zhub", "yaw", "tilt", "omegand", "pitch",
+                                                       "relm", "dc", "chord", "twist")}
+    arr["theta"] = T["theta"]            # advanced in place
+    assert arr["theta"].dtype == dt and arr["theta"].flags.c_contiguous
+    arr["nfoil"] = np.ascontiguousarray(T["nfoil"], np.int32)
+    arr["nang"] = np.array([len(f) for f in T["foils"]], np.int32)
+    arr["foil"] = foil
+    keep.extend(arr.values())
+    s = S(len(T["xhub"]), int(T.get("nblades", 3)), len(T["relm"]), len(T["foils"]), nmax,
+          *[arr[k].ctypes.data for k in ("xhub", "yhub", "zhub", "yaw", "tilt", "omegand", "pitch", "theta",
+                                          "relm", "dc", "chord", "twist", "nfoil", "nang", "foil")])
+    keep.append(s)
+    return s
 
 
 class OracleSim:
@@ -120,7 +193,8 @@ class OracleSim:
 
     # -- time loop --------------------------------------------------------------------
     def run(self, it0, it1, uini=0.0, udir=0.0, uvel_shear=None, uvel_time=None, udir_time=None,
-            blank=None, turb=None, forcing=None, ablvisc=None):
+            blank=None, turb=None, forcing=None, ablvisc=None, turbines=None):
+        """Steps it0+1..it1.  turbines: the dict of turbines_from_infile; its theta advances."""
         nx, ny, nz = self.dims
         dt = self.dt
         shear = np.ones(nz, dt) if uvel_shear is None else np.asarray(uvel_shear, dt)
@@ -135,11 +209,13 @@ class OracleSim:
         F = None if forcing is None else np.ascontiguousarray(forcing, dt)
         bl = None if blank is None else np.ascontiguousarray(blank, np.uint8)
         real = C.c_float if self.prec == "f32" else C.c_double
+        keep = []
+        T = None if turbines is None else C.byref(_turbines_struct(real, turbines, keep))
         sw = self.lib.or_run(C.byref(self.p), _ptr(self.f1), _ptr(self.f2), _ptr(self.tau),
                              _ptr(self.rho), _ptr(self.u), _ptr(self.v), _ptr(self.w), _ptr(abl),
                              _ptr(bl), _ptr(shear), _ptr(ut), _ptr(dtm), _ptr(uu), _ptr(vv), _ptr(ww),
                              C.c_int(nrturb), real(ampl), _ptr(F), C.c_int(F is not None), C.c_int(it0),
-                             C.c_int(it1))
+                             C.c_int(it1), T)
         self.cur ^= sw
         return self.f1
 
@@ -183,9 +259,14 @@ def inflow_series(files, nz, nt0, nt1, uini, udir, prec="f32"):
 # ------------------------------------------------------------------------------------------
 def infile_text(*, ibgk=2, ihrr=0, ivreman=0, smagorinsky=0.0, nt0=1, nt1=2, ibnd=0, jbnd=0, kbnd=0,
                 uini=0.0, udir=0.0, turb=False, ampl=0.0, nrturb=100, visc=1.0 / 30.0,
-                experiment="none", ltesting=True, ldump=True, ltiming=False, iout=None):
-    """An infile.in in the line order src/m_readinfile.F90:84-160 reads.  C_l = C_u = 1 so the
-    viscosity line is the non-dimensional kinevisc (src/m_readinfile.F90:182-186)."""
+                experiment="none", ltesting=True, ldump=True, ltiming=False, iout=None,
+                length=1.0, vel=1.0, turbines=None):
+    """An infile.in in the line order src/m_readinfile.F90:84-160 reads.  With the default
+    C_l = C_u = 1 the viscosity line is the non-dimensional kinevisc (src/m_readinfile.F90:182-186)
+    and uini the lattice speed; ``length``/``vel`` set C_l [m] and C_u [m/s] (then the caller
+    converts: kinevisc = visc / (C_l * C_u), lattice uini = uini / C_u).  ``turbines``: dict with
+    pitch [deg], rpm, tsr, tiploss and pos = [(ipos, jpos, kpos, yaw, tilt), ...]
+    (src/m_readinfile.F90:143-161)."""
     iout = iout if iout is not None else 10 ** 7
     L = lambda b: "T" if b else "F"
     smagorinsky, uini, udir, ampl, visc = (float(x) for x in (smagorinsky, uini, udir, ampl, visc))
@@ -203,21 +284,30 @@ def infile_text(*, ibgk=2, ihrr=0, ivreman=0, smagorinsky=0.0, nt0=1, nt1=2, ibn
         "# Inflow variables",
         f" {uini!r} {udir!r}", f" {L(turb)} {ampl!r} {nrturb}",
         "# Physical variables",
-        f" {visc!r}", " 1.225", " 1.0", " 1.0",
+        f" {visc!r}", " 1.225", f" {float(length)!r}", f" {float(vel)!r}",
         "# Averaging variables",
         " F F", " 20000", " 40000",
         "# Turbine-definitions",
-        " 0",
     ]
+    if turbines:
+        t = turbines
+        rows += [f" {len(t['pos'])}", f" {float(t.get('pitch', 0.0))!r}", f" {float(t['rpm'])!r}",
+                 f" {float(t.get('tsr', 0.0))!r}", f" {int(t.get('tiploss', 0))}"]
+        for n, (ip, jp, kp, yw, tl) in enumerate(t["pos"]):
+            rows += [f"# turbine {n + 1}", f" {ip} {jp} {kp} {float(yw)!r} {float(tl)!r}"]
+    else:
+        rows += [" 0"]
     return "\n".join(rows) + "\n"
 
 
 def run_reference(exe, f_in, *, nt0, nt1, turb_fields=None, workdir=None, threads=1, timeout=3600,
-                  read_output=True, files=None, **opts):
+                  read_output=True, files=None, theta=None, **opts):
     """Run a reference executable (built by oracle/build_ref.py) from restart state ``f_in``
     at step nt0 to nt1.  Returns (f_out, stdout).  f_out is the restart record written at
     nt1 (post-streaming f1 with ghosts, src/main.F90:398).  ``files``: name -> text of input
-    files written into the working directory first (uvel_shear.dat, uvel_time.dat)."""
+    files written into the working directory first (uvel_shear.dat, uvel_time.dat, Airfoils/*.dat).
+    ``theta``: the turbines' azimuths at nt0 (runs with turbines need that restart record); with it
+    the result is (f_out, theta_out, stdout)."""
     import sys
     sys.path.insert(0, os.path.dirname(HERE))
     from lbm_amd import fortran_io as fio
@@ -226,11 +316,15 @@ def run_reference(exe, f_in, *, nt0, nt1, turb_fields=None, workdir=None, thread
     try:
         os.makedirs(os.path.join(wd, "restart"), exist_ok=True)
         fio.write_restart(os.path.join(wd, "restart", fio.restart_name(nt0)), f_in)
+        if theta is not None:
+            fio.write_theta(os.path.join(wd, "restart", fio.restart_name(nt0, prefix="theta")), theta,
+                            np.asarray(f_in).dtype)
         if turb_fields is not None:
             uu, vv, ww, rr = turb_fields
             fio.write_turbulence(os.path.join(wd, "restart", fio.restart_name(nt0, prefix="turbulence")),
                                  uu, vv, ww, rr)
         for fname, text in (files or {}).items():
+            os.makedirs(os.path.dirname(os.path.join(wd, fname)), exist_ok=True)
             with open(os.path.join(wd, fname), "w") as fh:
                 fh.write(text)
         with open(os.path.join(wd, "infile.in"), "w") as fh:
@@ -243,6 +337,9 @@ def run_reference(exe, f_in, *, nt0, nt1, turb_fields=None, workdir=None, thread
         if not read_output:
             return None, p.stdout
         f_out = fio.read_restart(os.path.join(wd, "restart", fio.restart_name(nt1)))
+        if theta is not None:
+            th = fio.read_theta(os.path.join(wd, "restart", fio.restart_name(nt1, prefix="theta")), f_out.dtype)
+            return f_out, th, p.stdout
         return f_out, p.stdout
     finally:
         if own:

@@ -56,7 +56,36 @@ CASES = {
     "q27_bgk3_inflow_series": (27, "sp", (16, 8, 6), 20, 8, "none", 0,
                                dict(ibgk=3, ihrr=0, ivreman=0, ibnd=1, jbnd=0, kbnd=11, uini=0.05, udir=0.0,
                                     visc=1.0 / 30.0)),
+    # actuator-line turbines (TURBINES below), D3Q27 reg + Vreman, inflow/outflow, one rotor
+    "q27_regvre_turbine": (27, "sp", (16, 20, 20), 20, 9, "none", 0,
+                           dict(ibgk=3, ihrr=1, ivreman=1, smagorinsky=0.16733, ibnd=1, jbnd=0, kbnd=12,
+                                uini=0.05, udir=0.0, visc=0.01)),
+    # two yawed and tilted rotors whose deposits overlap, D3Q19 BGK, periodic i and j, no-slip k
+    "q19_bgk_turbines2": (19, "sp", (28, 20, 20), 20, 10, "none", 0,
+                          dict(ibgk=2, ibnd=0, jbnd=0, kbnd=11, uini=0.04, udir=0.0, visc=0.02)),
 }
+
+# Turbines of a case (infile section src/m_readinfile.F90:143-161): (ipos, jpos, kpos, yaw, tilt [deg]),
+# RPM, pitch [deg], and the azimuths theta [rad] of the restart record.  C_l = 16 m and C_u = 256 m/s
+# (rotor radius 64.4 m = 4.03 cells): every conversion of src/m_readinfile.F90:130-186 is then an exact
+# power-of-two scaling, so the infile's physical values give the case's lattice uini and kinevisc exactly.
+TURBINES = {
+    "q27_regvre_turbine": dict(pos=[(6, 10, 10, 0.0, 0.0)], rpm=9.0, pitch=0.0, theta=[0.0]),
+    "q19_bgk_turbines2": dict(pos=[(9, 10, 10, 15.0, 5.0), (16, 10, 11, -20.0, 0.0)], rpm=12.0, pitch=2.0,
+                              theta=[0.3, 1.1]),
+}
+TURBINE_LENGTH, TURBINE_VEL = 16.0, 256.0
+
+
+def infile_units(name):
+    """Extra infile_text arguments of a case: physical units and the turbine section."""
+    if name not in TURBINES:
+        return {}
+    q, prec, dims, steps, seed, exp, nrturb, o = CASES[name]
+    t = TURBINES[name]
+    return dict(length=TURBINE_LENGTH, vel=TURBINE_VEL, uini=o.get("uini", 0.0) * TURBINE_VEL,
+                visc=o["visc"] * TURBINE_LENGTH * TURBINE_VEL,
+                turbines=dict(pos=t["pos"], rpm=t["rpm"], pitch=t["pitch"], tsr=0.0, tiploss=0))
 
 # Input files of a case, written into the reference's working directory (src/m_inflow.F90:27-86
 # reads them): a power-law profile over the nz = 6 cells, and a series of speeds and directions

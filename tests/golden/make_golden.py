@@ -17,7 +17,9 @@ sys.path.insert(0, HERE)
 
 import build_ref  # noqa: E402
 import oracle  # noqa: E402
-from cases import CASES, case_files, case_inputs, fixture_path, sha  # noqa: E402
+from cases import CASES, TURBINES, case_files, case_inputs, fixture_path, infile_units, sha  # noqa: E402
+
+AIRFOILS = "/root/reference/airfoils"   # the reference's airfoil tables, read at generation time only
 
 
 def make(name):
@@ -28,14 +30,31 @@ def make(name):
     if nrturb:
         opts["nrturb"] = nrturb
         opts.setdefault("ampl", 0.0)
-    f_out, _ = oracle.run_reference(exe, f0, nt0=1, nt1=1 + steps, turb_fields=turb,
-                                    experiment=exp, files=case_files(name), **opts)
+    files = dict(case_files(name))
+    theta, foils = None, None
+    if name in TURBINES:
+        opts.update(infile_units(name))
+        theta = np.array(TURBINES[name]["theta"], np.float32 if prec == "sp" else np.float64)
+        texts = [open(os.path.join(AIRFOILS, fn)).read() for fn in oracle.NREL5MW_FOILS]
+        files.update({"Airfoils/" + fn: t for fn, t in zip(oracle.NREL5MW_FOILS, texts)})
+        foils = [oracle.read_foil_text(t, "f32" if prec == "sp" else "f64") for t in texts]
+    res = oracle.run_reference(exe, f0, nt0=1, nt1=1 + steps, turb_fields=turb, experiment=exp, files=files,
+                               theta=theta, **opts)
+    f_out = res[0]
     meta = dict(name=name, q=q, prec=prec, dims=list(dims), steps=steps, seed=seed, experiment=exp,
                 nrturb=nrturb, options=o, nt0=1, input_sha256=sha(f0), files=case_files(name),
+                turbines=TURBINES.get(name), infile_units=infile_units(name),
                 producer="reference build (amdflang) via oracle/build_ref.py")
     arrays = dict(meta=np.array(json.dumps(meta)), f_out=f_out)
     if turb is not None:
         arrays.update(uu=turb[0], vv=turb[1], ww=turb[2])
+    if foils is not None:
+        # the airfoil tables as read (deg, cl, cd per row), padded; the azimuths after the run
+        nmax = max(len(f) for f in foils)
+        tab = np.zeros((len(foils), nmax, 3), foils[0].dtype)
+        for k, f in enumerate(foils):
+            tab[k, :len(f)] = f
+        arrays.update(foil=tab, nang=np.array([len(f) for f in foils], np.int32), theta_out=res[1])
     np.savez_compressed(fixture_path(name), **arrays)
     print(f"{name}: {os.path.getsize(fixture_path(name)) / 1024:.0f} KiB")
 
