@@ -126,8 +126,49 @@ int lbm_generate_turbulence(lbm_ctx* ctx, int nrturb, double ampl, uint64_t seed
  * (src/m_forcings_apply.F90:3); NULL switches it off. */
 int lbm_set_external_forcing(lbm_ctx* ctx, const void* force, int realkind);
 
-/* Time steps it0+1 .. it0+nsteps: the sequence of src/main.F90:296-329 (turbulence df,
- * postcoll, turbulence apply, forcings_apply, solids, boundarycond, drift, swap, macrovars). */
+/* Actuator-line turbines: the external-force producer of src/main.F90:291-292, replacing
+ * turbine_forcing (src/m_turbine_forcing.F90:8-140) with turbine_distribute_points
+ * (src/m_turbine_distribute_points.F90:6-64), turbine_point_forces (src/m_turbine_point_forces.F90:20-82,
+ * src/m_turbine_interpolation.F90:25-114, src/m_nrelliftdrag.F90:7-35,
+ * src/m_turbine_compute_bladeforce.F90:20-65) and turbine_deposit (src/m_turbine_deposit.F90:7-91).
+ * Every step, before the collision, the rotor azimuths advance by omegand, each blade section's point
+ * takes the moments of the step's input (rho, u, v, w as macrovars leaves them) by trilinear
+ * interpolation, its angle of attack selects cl, cd from its airfoil table, and its lift and drag
+ * are spread with a Gaussian stencil (sigma = 2/sqrt(2), at most 5 cells out) into the external
+ * forcing, which forcings_apply then adds (src/main.F90:306).  The values are those of
+ * turbine_initialize (src/m_turbine_initialize.F90:29-70): positions in grid-index units, lengths in
+ * cells (m / p2l%length), angles of the rotor in radians, pitch, twist and table angles in degrees,
+ * omegand = 2 pi rpm / 60 * p2l%time.  Exclusive with lbm_set_external_forcing; not in ensembles,
+ * adjoint runs or halo domains.  t == NULL removes the turbines.  Host arrays are copied. */
+typedef struct {
+    int32_t nturbines;       /* rotors                                                        */
+    int32_t nblades;         /* blades per rotor (the reference: 3)                           */
+    int32_t nchords;         /* sections per blade (17 for the NREL 5-MW blade)               */
+    int32_t nfoils;          /* airfoil tables                                                */
+    int32_t nang_max;        /* row stride of the airfoil tables                              */
+    int32_t reserved;
+    const double* hub;       /* [3*nturbines] hub x, y, z (ipos, jpos, kpos)                  */
+    const double* yaw;       /* [nturbines] rad                                               */
+    const double* tilt;      /* [nturbines] rad                                               */
+    const double* omegand;   /* [nturbines] azimuth advance per step, rad                     */
+    const double* pitch;     /* [nturbines] collective pitch, deg                             */
+    const double* theta;     /* [nturbines] azimuth of the state's step, rad (the restart's)  */
+    const double* relm;      /* [nchords] radial position of each section, cells              */
+    const double* dc;        /* [nchords] radial length of each section, cells                */
+    const double* chord;     /* [nchords] chord, cells                                        */
+    const double* twist;     /* [nchords] deg                                                 */
+    const int32_t* nfoil;    /* [nchords] airfoil table of each section, 1-based              */
+    const int32_t* nang;     /* [nfoils] rows of each table                                   */
+    const double* foil;      /* [nfoils][nang_max][3]: angle of attack (deg), cl, cd          */
+} lbm_turbines;
+int lbm_set_turbines(lbm_ctx* ctx, const lbm_turbines* t);
+/* The rotors' azimuths after the last step (the restart record theta_<tile>_<it>.uf,
+ * src/m_saverestart.F90:118-122). */
+int lbm_get_turbine_theta(lbm_ctx* ctx, double* theta);
+
+/* Time steps it0+1 .. it0+nsteps: the sequence of src/main.F90:291-329 (turbine forcing,
+ * turbulence df, postcoll, turbulence apply, forcings_apply, solids, boundarycond, drift, swap,
+ * macrovars). */
 int lbm_step(lbm_ctx* ctx, int it0, int nsteps);
 
 /* Wait for the context's stream.  The stream itself (a hipStream_t) for callers that order

@@ -23,7 +23,7 @@ FORTRAN = os.path.join(HERE, "fortran")
 ARCH = "gfx950"
 
 SOURCES = ["kernels_f32.hip", "kernels_f64.hip", "kernels_f16s.hip", "kernels_bf16s.hip", "kernels_f64t.hip", "api.cpp",
-           "ensemble.cpp", "adjoint.cpp", "turbulence.hip"]
+           "ensemble.cpp", "adjoint.cpp", "turbulence.hip", "turbines.cpp"]
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wall", "-Wno-unused-function", "-I", CSRC, "-I", os.path.join(os.path.dirname(HERE), "include")]

@@ -53,6 +53,7 @@ int lbm_objective_run(lbm_ctx* c, int it0, int nsteps, const int32_t* box, doubl
     if (int rc_ = check_ctx(c)) return rc_;
     if (!J) return fail(c, LBM_EINVAL, "null argument");
     if (nsteps < 1) return fail(c, LBM_EINVAL, "nsteps must be >= 1");
+    if (c->turb_on) return fail(c, LBM_EINVAL, "objective runs take no turbines");
     AdjBox b;
     if (int rc = check_box(c, box, &b)) return rc;
     DevBuf acc;
@@ -135,6 +136,7 @@ int lbm_adjoint_run(lbm_ctx* c, int it0, int nsteps, int every, const int32_t* b
         return fail(c, LBM_ESTATE, "the adjoint starts from a stepped state: call lbm_step once after set_state / init_equilibrium");
     if (!c->ops->adj_step) return fail(c, LBM_EINVAL, "the adjoint needs FP32 or FP64 storage");
     if (c->has_turb) return fail(c, LBM_EINVAL, "the adjoint does not differentiate the inflow-turbulence forcing: switch it off");
+    if (c->turb_on) return fail(c, LBM_EINVAL, "the adjoint does not differentiate the turbines: remove them");
     if (c->avg_on) return fail(c, LBM_ESTATE, "switch time averaging off before an adjoint run");
     const int closed = (d.ibnd > 10) + (d.jbnd > 10) + (d.kbnd > 10);
     if (c->d.jbnd == LBM_BND_HALO) return fail(c, LBM_EINVAL, "the adjoint takes untiled domains (no halo planes)");

@@ -47,6 +47,10 @@ struct Cell {
         a.Fx = force ? F[0].data() : nullptr;
         a.Fy = force ? F[1].data() : nullptr;
         a.Fz = force ? F[2].data() : nullptr;
+        for (int d = 0; d < 3; ++d) {   // the force box: every interior cell
+            a.flo[d] = 1;
+            a.fhi[d] = 3;
+        }
         solid[0] = solid_cell ? 1u : 0u;
         a.solid = solid;
         a.solid_wpr = 1;

@@ -253,7 +253,13 @@ StepHost step_host(const lbm_ctx* c, bool fold) {
     h.solid_wpr = c->solid_wpr;
     h.ip = d.iturb_pos;
     h.ampl = c->ampl;
-    for (int q = 0; q < 3; ++q) h.F[q] = c->has_force ? c->force[q].p : nullptr;
+    // forcing: the caller's field (every interior cell) or the turbines' (their deposit box)
+    for (int q = 0; q < 3; ++q) h.F[q] = (c->has_force || c->turb_on) ? c->force[q].p : nullptr;
+    const int n[3] = {d.nx, d.ny, d.nz};
+    for (int a = 0; a < 3; ++a) {
+        h.flo[a] = c->turb_on ? c->th.lo[a] : 1;
+        h.fhi[a] = c->turb_on ? c->th.hi[a] : n[a];
+    }
     h.tau_out = (d.store_tau && d.ivreman) ? c->tau.p : nullptr;
     h.bgk3 = d.ibgk == 3;
     h.hrr = d.ihrr == 1;
@@ -481,7 +487,8 @@ int lbm_destroy(lbm_ctx* c) {
         if (c->buf[b]) hipFree(c->buf[b]);
     DevBuf* all[] = {&c->ablvisc, &c->shear, &c->solid, &c->turb[0], &c->turb[1], &c->turb[2], &c->force[0],
                      &c->force[1], &c->force[2], &c->tau, &c->ext[0], &c->ext[1], &c->ext[2], &c->ext[3],
-                     &c->stage, &c->partial, &c->snap};
+                     &c->stage, &c->partial, &c->snap, &c->t_rot, &c->t_theta, &c->t_blade, &c->t_nfoil,
+                     &c->t_nang, &c->t_foil, &c->t_pts, &c->t_box};
     for (DevBuf* b : all) dev_free(*b);
     for (hipEvent_t ev : c->ev_step) hipEventDestroy(ev);
     for (hipEvent_t ev : c->ev_bc) hipEventDestroy(ev);
@@ -613,6 +620,8 @@ int lbm_step(lbm_ctx* c, int it0, int nsteps) {
         step_inputs(c, h, it);
         // events on every timing_every-th step only: each record costs the stream a few us
         const bool timed = c->timing && c->timing_seq++ % c->timing_every == 0;
+        // turbine forcing of this step from its input's moments (src/main.F90:291-292)
+        if (c->turb_on) HIPCHK(c, c->ops->turb_step(c->lay, c->th, h.src, h.stream, h.ext, c->stream));
         hipEvent_t e0 = nullptr;
         if (timed && (e0 = next_event(c, c->ev_step, c->n_step_ev))) hipEventRecord(e0, c->stream);
         HIPCHK(c, c->ops->step(c->lay, h, c->stream));
@@ -644,6 +653,7 @@ int lbm_step_part(lbm_ctx* c, int it, int part) {
     if (!c->have_state) return fail(c, LBM_ESTATE, "no state: call lbm_set_state first");
     if (int rc = check_no_part(c)) return rc;
     if (c->d.jbnd != LBM_BND_HALO) return fail(c, LBM_EINVAL, "split steps need a halo domain (jbnd = LBM_BND_HALO)");
+    if (c->turb_on) return fail(c, LBM_EINVAL, "split steps take no turbines");
     if (c->d.ibnd == 1 && !c->inflow_fold_on)
         return fail(c, LBM_EINVAL, "split steps need the inflow/outflow planes in the step kernel (LBM_INFLOW_FOLD)");
     const int axis = c->lay.ax1 == 1 ? 1 : c->lay.ax2 == 1 ? 2 : 0;
@@ -954,6 +964,7 @@ int lbm_generate_turbulence(lbm_ctx* c, int nrturb, double ampl, uint64_t seed, 
 int lbm_set_external_forcing(lbm_ctx* c, const void* force, int realkind) {
     if (int rc_ = check_ctx(c)) return rc_;
     if (int rc = check_no_part(c)) return rc;
+    if (c->turb_on) return fail(c, LBM_EINVAL, "the turbines write the external forcing: remove them first");
     if (!force) {
         c->has_force = false;
         return LBM_OK;

@@ -53,6 +53,10 @@ struct lbm_ctx {
     std::vector<double> ut_series, ud_series;
     double tau0 = 0.0, fac0 = 0.0;
     double obj_tangent = 0.0;      // directional derivative of the last objective (tangent storage)
+    // actuator-line turbines (lbm_set_turbines): their forcing is written into force[] each step
+    bool turb_on = false;
+    lbm::TurbHost th{};
+    lbm_host::DevBuf t_rot, t_theta, t_blade, t_nfoil, t_nang, t_foil, t_pts, t_box;
     // time averaging: sums of u, v, w, u^2, v^2, w^2 (6 cell planes of the compute type)
     lbm_host::DevBuf avg;
     bool avg_on = false;
@@ -90,6 +94,11 @@ int hipfail(lbm_ctx* c, hipError_t e, const char* where);
 int dev_alloc(lbm_ctx* c, DevBuf& b, size_t n);
 void dev_free(DevBuf& b);
 double round_real(const lbm_ctx* c, double v);
+// element q of a host image of a device array of the compute type (rb bytes per value)
+void put_real(void* img, size_t q, double v, int rb);
+double get_real(const void* img, size_t q, int rb);
+// host reals of either kind (realkind 4 or 8) -> device array of the compute type
+int upload_reals(lbm_ctx* c, DevBuf& b, const void* host, size_t n, int realkind);
 void opts_of(int code, int* o1, int* oN);
 double utime_at(const lbm_ctx* c, int it);
 double udir_at(const lbm_ctx* c, int it);

@@ -44,6 +44,7 @@ struct StepHost {
     const void* turb[3];
     double ampl;
     const void* F[3];
+    int flo[3], fhi[3];   // cells (i, j, k; inclusive) whose force is read; the others have none
     void* avg;            // averaging sums (6 cell planes of the compute type) or nullptr
     void* tau_out;
     int fold_dir, fold_opt_lo, fold_opt_hi;   // closed face performed by the step kernel
@@ -81,6 +82,24 @@ struct EnsLaunch {
     const void* vals;                // device (utime, cos, sin)[steps of the table][nmem]
     int* clock;                      // device {it, steps since set, table's first step}
     int bgk3, hrr, vre;
+};
+
+// Actuator-line turbines (turbines.h): device tables of the compute type and the deposit boxes.
+constexpr int kTurbRot = 16;   // per-rotor table: hub x,y,z | e_axis | e1 | e2 | omegand, pitch | pad
+constexpr int kTurbPt = 8;     // per-point record: xg, yg, zg, F(3), sum of stencil weights, pad
+struct TurbHost {
+    int nturb, nblades, nchords, nang_max, npts, krad;
+    double sigma2;
+    const void* rot;      // [nturb][16] (compute type)
+    void* theta;          // [nturb]
+    const void* blade;    // [nchords][4]
+    const int* nfoil;     // [nchords], 0-based
+    const int* nang;      // [nfoils]
+    const void* foil;     // [nfoils][nang_max][3]
+    void* pts;            // [npts][8]
+    const int* tbox;      // [nturb][6]
+    int lo[3], hi[3];     // union of the rotors' deposit boxes (i, j, k; 1-based, inclusive)
+    void* F[3];           // force planes
 };
 
 // Objective box (inclusive, 1-based interior indices i, j, k).
@@ -125,6 +144,10 @@ struct Ops {
     hipError_t (*halo_pack)(const Layout&, const void* f, int side, void* buf, hipStream_t);
     hipError_t (*halo_unpack)(const Layout&, void* f, int side, const void* buf, int fold_dir, hipStream_t);
     hipError_t (*ghost_copy)(const Layout&, const void* src, void* dst, hipStream_t);
+    // turbine forcing of one step from the step's input (src, stream as the step kernel's; ext: the
+    // moments of an equilibrium start or nullptr)
+    hipError_t (*turb_step)(const Layout&, const TurbHost&, const void* src, int stream, const void* const ext[4],
+                            hipStream_t);
     int real_bytes;   // compute type size
     int raw_bytes;    // storage size
 };

@@ -52,6 +52,7 @@ struct StepArgs {
     const R* Fx;           // external forcing (cell layout); nullptr: off
     const R* Fy;
     const R* Fz;
+    int flo[3], fhi[3];    // cells (i, j, k; inclusive) whose force is read (the others have none)
     R* tau_out;            // cell layout; nullptr: tau not stored
     int fold_dir;          // closed axis whose face pass this kernel performs (-1: none)
     int fold_opt_lo, fold_opt_hi;
@@ -309,7 +310,12 @@ __host__ __device__ __forceinline__ void collide_cell(const StepArgs<typename C:
         add_feq_diff(ux + a.ampl * a.tu[sl], uy + a.ampl * a.tv[sl], uz + a.ampl * a.tw[sl],
                      std::integral_constant<bool, BGK3>());
     }
-    if (a.Fx != nullptr && rh > R(0)) {
+    // force box: every interior cell for the caller's field, the deposit box for turbines (which the
+    // tangent storage never has; the test is compiled out there, where it trips the register allocator)
+    bool in_fbox = true;
+    if constexpr (sizeof(R) <= 8)
+        in_fbox = i >= a.flo[0] && i <= a.fhi[0] && j >= a.flo[1] && j <= a.fhi[1] && k >= a.flo[2] && k <= a.fhi[2];
+    if (a.Fx != nullptr && rh > R(0) && in_fbox) {
         const R ir = recip(rh);
         add_feq_diff(ux - a.Fx[oc] * ir, uy - a.Fy[oc] * ir, uz - a.Fz[oc] * ir, std::integral_constant<bool, false>());
     }

@@ -9,6 +9,7 @@
 #include "impl.h"
 #include "kernels.h"
 #include "adjoint.h"
+#include "turbines.h"
 
 namespace lbm {
 
@@ -96,6 +97,10 @@ struct Launch {
         a.Fx = static_cast<const R*>(h.F[0]);
         a.Fy = static_cast<const R*>(h.F[1]);
         a.Fz = static_cast<const R*>(h.F[2]);
+        for (int d = 0; d < 3; ++d) {
+            a.flo[d] = h.flo[d];
+            a.fhi[d] = h.fhi[d];
+        }
         a.tau_out = static_cast<R*>(h.tau_out);
         a.fold_dir = h.fold_dir;
         a.fold_opt_lo = h.fold_opt_lo;
@@ -158,6 +163,44 @@ struct Launch {
         if (blocks == 0) return hipSuccess;
         hipLaunchKernelGGL((ghost_copy_kernel<Q, C>), dim3((unsigned)blocks), dim3(256), 0, s, L, src, dst, nr1, nr2);
         return hipGetLastError();
+    }
+
+    static hipError_t turb_step(const Layout& L, const TurbHost& h, const void* src, int stream,
+                                const void* const ext[4], hipStream_t s) {
+        if constexpr (sizeof(R) > 8) {
+            return hipErrorNotSupported;   // the tangent storage has no turbines
+        } else {
+            TurbArgs<R> a;
+            a.lay = L;
+            a.src = src;
+            a.stream = stream;
+            for (int q = 0; q < 4; ++q) a.ext[q] = static_cast<const R*>(ext[q]);
+            a.nturb = h.nturb;
+            a.nblades = h.nblades;
+            a.nchords = h.nchords;
+            a.nang_max = h.nang_max;
+            a.krad = h.krad;
+            a.sigma2 = (R)h.sigma2;
+            a.rot = static_cast<const R*>(h.rot);
+            a.theta = static_cast<R*>(h.theta);
+            a.blade = static_cast<const R*>(h.blade);
+            a.nfoil = h.nfoil;
+            a.nang = h.nang;
+            a.foil = static_cast<const R*>(h.foil);
+            a.pts = static_cast<R*>(h.pts);
+            a.tbox = h.tbox;
+            for (int d = 0; d < 3; ++d) {
+                a.lo[d] = h.lo[d];
+                a.hi[d] = h.hi[d];
+            }
+            for (int q = 0; q < 3; ++q) a.F[q] = static_cast<R*>(h.F[q]);
+            hipLaunchKernelGGL((turbine_point_kernel<Q, C>), dim3(h.npts), dim3(256), 0, s, a);
+            const long long cells = (long long)(h.hi[0] - h.lo[0] + 1) * (h.hi[1] - h.lo[1] + 1) * (h.hi[2] - h.lo[2] + 1);
+            // at least one block: block 0 also stores the advanced azimuths
+            const long long blocks = cells > 0 ? (cells + 255) / 256 : 1;
+            hipLaunchKernelGGL((turbine_deposit_kernel<R>), dim3((unsigned)blocks), dim3(256), 0, s, a);
+            return hipGetLastError();
+        }
     }
 
     static dim3 halo_grid(const Layout& L, int nside) {
@@ -386,7 +429,7 @@ struct Launch {
         static const Ops o = {&step, &closed, &snapshot, &edges, &inflow, &aos_in, &state_out, &feq_init, &totals, &average,
                               sizeof(MemberTab<R>), &ens_pack, &ens_step,
                               kAdj ? &adj_step : nullptr, kAdj ? &adj_inflow : nullptr, kAdj ? &adj_face : nullptr,
-                              &objective, &halo_pack, &halo_unpack, &ghost_copy, (int)sizeof(R),
+                              &objective, &halo_pack, &halo_unpack, &ghost_copy, &turb_step, (int)sizeof(R),
                               (int)sizeof(typename C::raw)};
         return &o;
     }

@@ -35,7 +35,8 @@ EXPORTS = ["lbm_create", "lbm_destroy", "lbm_last_error", "lbm_set_state", "lbm_
            "lbm_ensemble_tune", "lbm_ensemble_set_timing", "lbm_ensemble_timing_read",
            "lbm_objective_run", "lbm_adjoint_run", "lbm_set_tangent", "lbm_objective_tangent",
            "lbm_generate_turbulence", "lbm_set_averaging", "lbm_get_averages",
-           "lbm_halo_bytes", "lbm_halo_pack", "lbm_halo_unpack", "lbm_step_part"]
+           "lbm_halo_bytes", "lbm_halo_pack", "lbm_halo_unpack", "lbm_step_part",
+           "lbm_set_turbines", "lbm_get_turbine_theta"]
 
 
 class Desc(C.Structure):
@@ -44,6 +45,14 @@ class Desc(C.Structure):
                 ("ibnd", C.c_int32), ("jbnd", C.c_int32), ("kbnd", C.c_int32), ("iturb_pos", C.c_int32),
                 ("store_tau", C.c_int32), ("block_x", C.c_int32), ("block_y", C.c_int32),
                 ("kinevisc", C.c_double), ("smagorinsky", C.c_double), ("rho0", C.c_double)]
+
+
+class Turbines(C.Structure):
+    """lbm_turbines (include/lbm_hip.h)."""
+    _fields_ = [("nturbines", C.c_int32), ("nblades", C.c_int32), ("nchords", C.c_int32), ("nfoils", C.c_int32),
+                ("nang_max", C.c_int32), ("reserved", C.c_int32)] + \
+               [(n, C.c_void_p) for n in ("hub", "yaw", "tilt", "omegand", "pitch", "theta", "relm", "dc", "chord",
+                                          "twist", "nfoil", "nang", "foil")]
 
 
 class LBMError(RuntimeError):
@@ -105,6 +114,8 @@ def load_library(path: str = LIB_PATH):
         "lbm_get_averages": ([vp, vp, vp, vp, vp, C.POINTER(i32), i32], i32),
         "lbm_generate_turbulence": ([vp, i32, dbl, C.c_uint64, vp, vp, vp, vp, i32], i32),
         "lbm_objective_tangent": ([vp, C.POINTER(dbl)], i32),
+        "lbm_set_turbines": ([vp, C.POINTER(Turbines)], i32),
+        "lbm_get_turbine_theta": ([vp, C.POINTER(dbl)], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -273,6 +284,34 @@ class Domain:
             return
         F = self._arr(force, self.shape_cell + (3,))
         self._chk(self.lib.lbm_set_external_forcing(self.h, _p(F), F.itemsize), "lbm_set_external_forcing")
+
+    def set_turbines(self, t):
+        """Actuator-line turbines (lbm_set_turbines) from the dict of lbm_amd.turbines.from_infile;
+        None removes them."""
+        if t is None:
+            self._chk(self.lib.lbm_set_turbines(self.h, None), "lbm_set_turbines")
+            return
+        f64 = lambda a: np.ascontiguousarray(a, np.float64)
+        foils = t["foils"]
+        nmax = max(len(f) for f in foils)
+        foil = np.zeros((len(foils), nmax, 3))
+        for k, f in enumerate(foils):
+            foil[k, :len(f)] = f
+        arr = dict(hub=f64(t["hub"]).reshape(-1), yaw=f64(t["yaw"]), tilt=f64(t["tilt"]), omegand=f64(t["omegand"]),
+                   pitch=f64(t["pitch"]), theta=f64(t["theta"]), relm=f64(t["relm"]), dc=f64(t["dc"]),
+                   chord=f64(t["chord"]), twist=f64(t["twist"]), nfoil=np.ascontiguousarray(t["nfoil"], np.int32),
+                   nang=np.array([len(f) for f in foils], np.int32), foil=foil)
+        d = Turbines(len(arr["yaw"]), int(t.get("nblades", 3)), len(arr["relm"]), len(foils), nmax, 0,
+                     *[arr[k].ctypes.data for k in ("hub", "yaw", "tilt", "omegand", "pitch", "theta", "relm", "dc",
+                                                     "chord", "twist", "nfoil", "nang", "foil")])
+        self._chk(self.lib.lbm_set_turbines(self.h, C.byref(d)), "lbm_set_turbines")
+        self._nturb = len(arr["yaw"])
+
+    def turbine_theta(self):
+        """The rotors' azimuths [rad] after the last step."""
+        th = (C.c_double * self._nturb)()
+        self._chk(self.lib.lbm_get_turbine_theta(self.h, th), "lbm_get_turbine_theta")
+        return np.array(th[:])
 
     # -- objective and adjoint -------------------------------------------------------------
     @staticmethod

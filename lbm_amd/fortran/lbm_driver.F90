@@ -9,9 +9,11 @@
 ! Inflow-turbulence fields come from the turbulence restart file when it exists, otherwise (and
 ! every nrturb steps, as src/main.F90:381-383) from the device generator lbm_generate_turbulence.
 ! Time averages (laveraging, avestart, avesave) are accumulated on the device and written to
-! averages/averages_0000_<avesave>.uf.  Not carried over (outside the lattice update): turbines,
-! buoyancy/ABL, tracers, Tecplot and NetCDF output, the random perturbation of inipert (nt0 = 0
-! starts from the inflow profile).  Geometry: 'none' or the reference's 'cylinder' (src/m_cylinder.F90).
+! averages/averages_0000_<avesave>.uf.  Actuator-line turbines (the infile's turbine section, the
+! NREL 5-MW blade, ./Airfoils/*.dat, restart/theta_0000_<it6>.uf) run on the device.  Not carried
+! over (outside the lattice update): buoyancy/ABL, tracers, Tecplot and NetCDF output, the random
+! perturbation of inipert (nt0 = 0 starts from the inflow profile).  Geometry: 'none' or the
+! reference's 'cylinder' (src/m_cylinder.F90).
 program lbm_driver
    use, intrinsic :: iso_c_binding
    use lbm_hip
@@ -45,6 +47,31 @@ program lbm_driver
    integer(c_int) :: rc
    integer(8) :: t0, t1, rate
    real(8) :: wall, mass, mom(3)
+   ! turbines: infile values (src/m_readinfile.F90:143-161) and the device description
+   real(rk) :: pitchangle, turbrpm, tipspeedratio
+   integer :: itiploss, nang_max
+   integer, allocatable :: ipos(:), jpos(:), kpos(:)
+   real(rk), allocatable :: yawd(:), tiltd(:)
+   real(c_double), allocatable, target :: t_hub(:,:), t_yaw(:), t_tilt(:), t_omegand(:), t_pitch(:), t_theta(:)
+   real(c_double), allocatable, target :: t_relm(:), t_dc(:), t_chord(:), t_twist(:), t_foil(:,:,:)
+   integer(c_int32_t), allocatable, target :: t_nfoil(:), t_nang(:)
+   type(lbm_turbines) :: tdesc
+   ! NREL 5-MW blade (src/mod_nrel5mw.F90:24-48) and its airfoil tables (src/m_nrelreadfoil.F90:13-20)
+   real(rk), parameter :: nrel_relm(17) = [2.86670_rk, 5.60000_rk, 8.33330_rk, 11.75000_rk, 15.85000_rk, &
+      19.95000_rk, 24.05000_rk, 28.15000_rk, 32.25000_rk, 36.35000_rk, 40.45000_rk, 44.55000_rk, 48.65000_rk, &
+      52.75000_rk, 56.16670_rk, 58.90000_rk, 61.63330_rk]
+   real(rk), parameter :: nrel_dc(17) = [2.73330_rk, 2.73330_rk, 2.73330_rk, 4.10000_rk, 4.10000_rk, 4.10000_rk, &
+      4.10000_rk, 4.10000_rk, 4.10000_rk, 4.10000_rk, 4.10000_rk, 4.10000_rk, 4.10000_rk, 4.10000_rk, 2.73330_rk, &
+      2.73330_rk, 2.73330_rk]
+   real(rk), parameter :: nrel_chord(17) = [3.54200_rk, 3.85400_rk, 4.16700_rk, 4.55700_rk, 4.65200_rk, 4.45800_rk, &
+      4.24900_rk, 4.00700_rk, 3.74800_rk, 3.50200_rk, 3.25600_rk, 3.01000_rk, 2.76400_rk, 2.51800_rk, 2.31300_rk, &
+      2.08600_rk, 1.41900_rk]
+   real(rk), parameter :: nrel_twist(17) = [13.3080_rk, 13.3080_rk, 13.3080_rk, 13.3080_rk, 11.4800_rk, 10.1620_rk, &
+      9.0110_rk, 7.7950_rk, 6.5440_rk, 5.3610_rk, 4.1880_rk, 3.1250_rk, 2.3190_rk, 1.5260_rk, 0.8630_rk, 0.3700_rk, &
+      0.1060_rk]
+   integer, parameter :: nrel_nfoil(17) = [1, 1, 2, 3, 4, 4, 5, 6, 6, 7, 7, 8, 8, 8, 8, 8, 8]
+   character(len=14), parameter :: foil_files(8) = [character(len=14) :: 'Cylinder1.dat', 'Cylinder2.dat', &
+      'DU40_A17.dat', 'DU35_A17.dat', 'DU30_A17.dat', 'DU25_A17.dat', 'DU21_A17.dat', 'NACA64_A17.dat']
 
    ! ---------------------------------------------------------------- infile.in
    inquire(file='infile.in', exist=ex)
@@ -91,8 +118,18 @@ program lbm_driver
    read(iunit,*) avesave
    read(iunit,'(a)') ver
    read(iunit,*) nturbines
+   if (nturbines > 0) then
+      allocate(ipos(nturbines), jpos(nturbines), kpos(nturbines), yawd(nturbines), tiltd(nturbines))
+      read(iunit,*) pitchangle
+      read(iunit,*) turbrpm
+      read(iunit,*) tipspeedratio
+      read(iunit,*) itiploss
+      do i = 1, nturbines
+         read(iunit,'(a)') ver
+         read(iunit,*) ipos(i), jpos(i), kpos(i), yawd(i), tiltd(i)
+      enddo
+   endif
    close(iunit)
-   if (nturbines > 0) stop 'lbm_driver: turbines are not part of this build'
    if (iablvisc == 2) stop 'lbm_driver: the buoyancy scheme is not part of this build'
    ! conversions as src/m_readinfile.F90:133,171-186
    uini = uini / p_vel
@@ -189,6 +226,8 @@ program lbm_driver
       call lbm_check(ctx, rc, 'lbm_set_state')
    endif
 
+   if (nturbines > 0) call setup_turbines()
+
    if (inflowturbulence) then
       allocate(uu(ny,nz,0:nrturb), vv(ny,nz,0:nrturb), ww(ny,nz,0:nrturb), rr(ny,nz,0:nrturb))
       write(fname,'(a,i6.6,a)') 'restart/turbulence_0000_', nt0, '.uf'
@@ -253,6 +292,83 @@ program lbm_driver
 
 contains
 
+   subroutine setup_turbines()
+      ! turbine_initialize (src/m_turbine_initialize.F90:29-70) in the REAL kind, the airfoil tables
+      ! (src/m_nrelreadfoil.F90:33-47) and the azimuths of the restart (src/m_readrestart.F90:103-116)
+      real(rk), parameter :: pi = 3.14159265358979323846_rk, pi2 = 2.0_rk * pi
+      real(rk) :: omega, row(4)
+      real(rk), allocatable :: th(:)
+      integer :: iu, kf, ia, na(8)
+      character(len=100) :: fn
+      omega = pi2 * turbrpm / 60.0_rk
+      allocate(t_hub(3, nturbines), t_yaw(nturbines), t_tilt(nturbines), t_omegand(nturbines), t_pitch(nturbines), &
+               t_theta(nturbines), th(nturbines))
+      do i = 1, nturbines
+         t_hub(:, i) = [real(real(ipos(i), rk), c_double), real(real(jpos(i), rk), c_double), &
+                        real(real(kpos(i), rk), c_double)]
+         t_yaw(i) = real((yawd(i) / 360.0_rk) * pi2, c_double)
+         t_tilt(i) = real((tiltd(i) / 360.0_rk) * pi2, c_double)
+         t_omegand(i) = real(omega * p_time, c_double)
+         t_pitch(i) = real(pitchangle, c_double)
+      enddo
+      allocate(t_relm(17), t_dc(17), t_chord(17), t_twist(17), t_nfoil(17))
+      t_relm = real(nrel_relm / p_len, c_double)
+      t_dc = real(nrel_dc / p_len, c_double)
+      t_chord = real(nrel_chord / p_len, c_double)
+      t_twist = real(nrel_twist, c_double)
+      t_nfoil = nrel_nfoil
+      ! airfoil tables: 50 lines, the row count, 2 lines, rows alpha cl cd cm
+      nang_max = 0
+      do kf = 1, 8
+         open(newunit=iu, file='./Airfoils/'//trim(foil_files(kf)), status='old', iostat=ios)
+         if (ios /= 0) then
+            print '(2a)', 'lbm_driver: airfoil table missing: ./Airfoils/', trim(foil_files(kf))
+            stop 1
+         endif
+         do ia = 1, 50
+            read(iu,*)
+         enddo
+         read(iu,*) na(kf)
+         close(iu)
+         nang_max = max(nang_max, na(kf))
+      enddo
+      allocate(t_foil(3, nang_max, 8), t_nang(8))
+      t_foil = 0.0d0
+      do kf = 1, 8
+         open(newunit=iu, file='./Airfoils/'//trim(foil_files(kf)), status='old')
+         do ia = 1, 53
+            read(iu,*)
+         enddo
+         do ia = 1, na(kf)
+            read(iu,*) row
+            t_foil(:, ia, kf) = real(row(1:3), c_double)
+         enddo
+         close(iu)
+         t_nang(kf) = na(kf)
+      enddo
+      th = 0.0_rk
+      if (nt0 > 0) then
+         write(fn,'(a,i6.6,a)') 'restart/theta_0000_', nt0, '.uf'
+         inquire(file=trim(fn), exist=ex)
+         if (.not. ex) then
+            print '(2a)', 'lbm_driver: no restart file for theta: ', trim(fn)
+            stop 1
+         endif
+         open(newunit=iu, file=trim(fn), form='unformatted', status='old')
+         read(iu) th
+         close(iu)
+      endif
+      t_theta = real(th, c_double)
+      tdesc%nturbines = nturbines; tdesc%nblades = 3; tdesc%nchords = 17; tdesc%nfoils = 8
+      tdesc%nang_max = nang_max; tdesc%reserved = 0
+      tdesc%hub = c_loc(t_hub); tdesc%yaw = c_loc(t_yaw); tdesc%tilt = c_loc(t_tilt)
+      tdesc%omegand = c_loc(t_omegand); tdesc%pitch = c_loc(t_pitch); tdesc%theta = c_loc(t_theta)
+      tdesc%relm = c_loc(t_relm); tdesc%dc = c_loc(t_dc); tdesc%chord = c_loc(t_chord); tdesc%twist = c_loc(t_twist)
+      tdesc%nfoil = c_loc(t_nfoil); tdesc%nang = c_loc(t_nang); tdesc%foil = c_loc(t_foil)
+      rc = lbm_set_turbines(ctx, tdesc)
+      call lbm_check(ctx, rc, 'lbm_set_turbines')
+   end subroutine
+
    subroutine save_averages(itw)
       ! uave, vave, wave and Ti of averaging_full (src/m_averaging_full.F90:82-95), written as an
       ! unformatted record averages/averages_0000_<it>.uf instead of the Tecplot diag file
@@ -297,6 +413,15 @@ contains
          write(fn,'(a,i6.6,a)') 'restart/turbulence_0000_', itw, '.uf'
          open(newunit=iu, file=trim(fn), form='unformatted', status='replace')
          write(iu) ny, nz, nrturb, uu, vv, ww, rr
+         close(iu)
+      endif
+      if (nturbines > 0) then
+         ! src/m_saverestart.F90:118-122
+         rc = lbm_get_turbine_theta(ctx, t_theta)
+         call lbm_check(ctx, rc, 'lbm_get_turbine_theta')
+         write(fn,'(a,i6.6,a)') 'restart/theta_0000_', itw, '.uf'
+         open(newunit=iu, file=trim(fn), form='unformatted', status='replace')
+         write(iu) real(t_theta, rk)
          close(iu)
       endif
    end subroutine

@@ -25,6 +25,13 @@ module lbm_hip
       real(c_double)     :: kinevisc, smagorinsky, rho0
    end type
 
+   ! actuator-line turbines (lbm_set_turbines): sizes and the addresses of the tables
+   type, bind(C), public :: lbm_turbines
+      integer(c_int32_t) :: nturbines, nblades, nchords, nfoils, nang_max, reserved
+      type(c_ptr) :: hub, yaw, tilt, omegand, pitch, theta, relm, dc, chord, twist, nfoil, nang, foil
+   end type
+
+   public :: lbm_set_turbines, lbm_get_turbine_theta
    public :: lbm_create, lbm_destroy, lbm_last_error, lbm_set_state, lbm_get_state, lbm_get_macros, &
              lbm_init_equilibrium, lbm_set_blanking, lbm_set_ablvisc, lbm_set_inflow, &
              lbm_set_inflow_series, lbm_set_turbulence, lbm_set_external_forcing, lbm_step, &
@@ -111,6 +118,17 @@ module lbm_hip
          import :: c_int, c_ptr
          type(c_ptr), value :: ctx, force
          integer(c_int), value :: realkind
+      end function
+      ! turbine_forcing (src/m_turbine_forcing.F90:8-140) on the device, every step
+      integer(c_int) function lbm_set_turbines(ctx, t) bind(C, name='lbm_set_turbines')
+         import :: c_int, c_ptr, lbm_turbines
+         type(c_ptr), value :: ctx
+         type(lbm_turbines), intent(in) :: t
+      end function
+      integer(c_int) function lbm_get_turbine_theta(ctx, theta) bind(C, name='lbm_get_turbine_theta')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(out) :: theta(*)
       end function
       integer(c_int) function lbm_step(ctx, it0, nsteps) bind(C, name='lbm_step')
          import :: c_int, c_ptr

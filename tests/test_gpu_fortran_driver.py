@@ -20,8 +20,23 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BUILD = os.path.join(ROOT, "lbm_amd", "_build")
 
 
+def write_airfoils(directory, fx):
+    """The fixture's airfoil tables as ./Airfoils/*.dat in the layout src/m_nrelreadfoil.F90:33-47
+    reads: 50 header lines, the row count, 2 lines, rows "alpha cl cd cm" (values printed exactly)."""
+    from lbm_amd.turbines import FOIL_FILES
+    os.makedirs(directory, exist_ok=True)
+    for k, fn in enumerate(FOIL_FILES):
+        n = int(fx["nang"][k])
+        rows = ["! header line %d" % (r + 1) for r in range(50)]
+        rows += [f"        {n}   NumAlf", "!    Alpha      Cl      Cd        Cm", "!    (deg)      (-)     (-)       (-)"]
+        rows += ["  %.9g  %.9g  %.9g  0.0" % tuple(float(x) for x in fx["foil"][k, r]) for r in range(n)]
+        with open(os.path.join(directory, fn), "w") as fh:
+            fh.write("\n".join(rows) + "\n")
+
+
 @pytest.mark.parametrize("name", ["q27_regvre_inflow_cyl", "q27_regvre_inflow_dp", "q19_bgk_inflow_dir",
-                                  "q27_regvre_closed", "q27_bgk3_inflow_series"])
+                                  "q27_regvre_closed", "q27_bgk3_inflow_series", "q27_regvre_turbine",
+                                  "q19_bgk_turbines2"])
 def test_fortran_driver_matches_reference_build(name, tmp_path):
     q, prec, dims, steps, seed, exp, nrturb, o = cases.CASES[name]
     exe = os.path.join(BUILD, "lbm_driver_" + ("f32" if prec == "sp" else "f64"))
@@ -39,6 +54,11 @@ def test_fortran_driver_matches_reference_build(name, tmp_path):
         fio.write_turbulence(str(tmp_path / "restart" / fio.restart_name(nt0, prefix="turbulence")), *turb)
     for fname, text in cases.case_files(name).items():   # uvel_shear.dat, uvel_time.dat
         (tmp_path / fname).write_text(text)
+    if name in cases.TURBINES:
+        opts.update(cases.infile_units(name))
+        write_airfoils(str(tmp_path / "Airfoils"), fx)
+        fio.write_theta(str(tmp_path / "restart" / fio.restart_name(nt0, prefix="theta")),
+                        np.array(cases.TURBINES[name]["theta"], np.float32))
     (tmp_path / "infile.in").write_text(
         oracle.infile_text(nt0=nt0, nt1=nt1, turb=bool(nrturb), experiment=exp, **opts))
     print(f"driver {os.path.basename(exe)} case {name} dims={dims} steps={steps}", flush=True)
@@ -52,6 +72,9 @@ def test_fortran_driver_matches_reference_build(name, tmp_path):
     tol = 2e-6 if ref.dtype == np.float32 else 1e-12
     print(f"max |df| vs reference build {d.max():.3e}", flush=True)
     assert np.isfinite(f).all() and d.max() <= tol
+    if name in cases.TURBINES:
+        th = fio.read_theta(str(tmp_path / "restart" / fio.restart_name(nt1, prefix="theta")))
+        assert np.array_equal(th, fx["theta_out"]), (th, fx["theta_out"])
     # the testing file (src/m_testing.F90) is written on the first run and compared on a second
     assert (tmp_path / "testing" / fio.restart_name(nt1, prefix="testing")).exists()
 

@@ -61,8 +61,19 @@ def run_case(name):
         dom.set_blanking(bl)
     if nrturb:
         dom.set_turbulence(fx["uu"], fx["vv"], fx["ww"], nrturb, o.get("ampl", 0.0))
+    theta = None
+    if name in cases.TURBINES:
+        from lbm_amd import turbines
+        t = cases.TURBINES[name]
+        foils = [fx["foil"][k, :n] for k, n in enumerate(fx["nang"])]
+        dom.set_turbines(turbines.from_infile(t["pos"], t["rpm"], t["pitch"], cases.TURBINE_LENGTH, cases.TURBINE_VEL,
+                                              foils, dt, theta=t["theta"]))
     dom.step(1, steps)
     f = dom.get_state()
+    if name in cases.TURBINES:
+        theta = dom.turbine_theta()
+        # the azimuths accumulate omegand in the REAL kind, as the reference's
+        assert np.array_equal(theta.astype(dt), fx["theta_out"]), (theta, fx["theta_out"])
     dom.close()
     return f, fx["f_out"]
 
