@@ -85,7 +85,9 @@ int lbm_init_equilibrium(lbm_ctx* ctx, const void* rho, const void* u, const voi
  * layers), as saverestart/testing write it (src/m_saverestart.F90:125-136). */
 int lbm_get_state(lbm_ctx* ctx, void* f, int realkind);
 
-/* Moments of that state for every cell including ghosts: macrovars (src/m_macrovars.F90:4). */
+/* Moments of that state for every cell including ghosts: macrovars (src/m_macrovars.F90:4).
+ * Returns LBM_EUNSTABLE (after filling the arrays) when a moment is not finite or a cell's density
+ * is <= 0: the test diag makes before it stops the run (src/m_diag.F90:169-173). */
 int lbm_get_macros(lbm_ctx* ctx, void* rho, void* u, void* v, void* w, int realkind);
 
 /* Per-cell relaxation time of the last step (needs store_tau and ivreman; interior cells,
@@ -232,7 +234,9 @@ int lbm_timing_read(lbm_ctx* ctx, double* step_ms, int* n_step, double* bc_ms, i
 int lbm_set_averaging(lbm_ctx* ctx, int on);
 int lbm_get_averages(lbm_ctx* ctx, void* uave, void* vave, void* wave, void* ti, int* count, int realkind);
 
-/* Global mass and momentum of the interior state in FP64 (accumulated on the device). */
+/* Global mass and momentum of the interior state in FP64 (accumulated on the device).  Returns
+ * LBM_EUNSTABLE (after setting the outputs) when a population is not finite or an interior cell's
+ * density is <= 0 (src/m_diag.F90:169-173); the message gives the count and the smallest density. */
 int lbm_get_totals(lbm_ctx* ctx, double* mass, double* momentum3);
 
 /* Bytes of HBM the two population buffers hold, the device layout pitch (elements per row) and

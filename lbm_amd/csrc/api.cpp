@@ -336,6 +336,25 @@ int boundary(lbm_ctx* c, void* f1, const void* f2, int it, int skip_dir, bool sk
     return LBM_OK;
 }
 
+int finish_totals(lbm_ctx* c, const double* h, int nb, double* mass, double* mom) {
+    double t[4] = {0, 0, 0, 0}, rmin = INFINITY, bad = 0;
+    for (int b = 0; b < nb; ++b) {
+        const double* p = h + (size_t)kTotalsSlots * b;
+        for (int q = 0; q < 4; ++q) t[q] += p[q];
+        rmin = std::min(rmin, p[4]);
+        bad += p[5];
+    }
+    if (mass) *mass = t[0];
+    if (mom) { mom[0] = t[1]; mom[1] = t[2]; mom[2] = t[3]; }
+    // src/m_diag.F90:169-173 stops on a negative density
+    if (bad > 0 || !(rmin > 0.0)) {
+        char msg[160];
+        std::snprintf(msg, sizeof msg, "unstable state: %.0f non-finite populations, smallest cell density %g", bad, rmin);
+        return fail(c, LBM_EUNSTABLE, msg);
+    }
+    return LBM_OK;
+}
+
 int check_no_part(lbm_ctx* c) {
     if (c->part_open)
         return fail(c, LBM_ESTATE, "a split step is open: finish it with lbm_step_part(ctx, it, LBM_PART_INNER)");
@@ -790,6 +809,25 @@ int lbm_get_macros(lbm_ctx* c, void* rho, void* u, void* v, void* w, int realkin
     HIPCHK(c, hipMemcpyAsync(v, st + 2 * cb, cb, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(w, st + 3 * cb, cb, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    // the instability test of src/m_diag.F90:169-173 on the moments just returned (all cells)
+    size_t bad = 0, nonpos = 0;
+    double rmin = INFINITY;
+    const void* arr[4] = {rho, u, v, w};
+    for (int q = 0; q < 4; ++q)
+        for (size_t e = 0; e < nc; ++e) {
+            const double x = realkind == 8 ? static_cast<const double*>(arr[q])[e] : static_cast<const float*>(arr[q])[e];
+            if (!std::isfinite(x)) ++bad;
+            else if (q == 0) {
+                rmin = std::min(rmin, x);
+                if (x <= 0.0) ++nonpos;
+            }
+        }
+    if (bad || nonpos) {
+        char msg[160];
+        std::snprintf(msg, sizeof msg, "unstable state: %zu non-finite moments, %zu cells with density <= 0 (smallest %g)",
+                      bad, nonpos, rmin);
+        return fail(c, LBM_EUNSTABLE, msg);
+    }
     return LBM_OK;
 }
 
@@ -1069,19 +1107,14 @@ int lbm_get_totals(lbm_ctx* c, double* mass, double* mom) {
     if (int rc_ = check_ctx(c)) return rc_;
     if (int rc = check_no_part(c)) return rc;
     if (!c->have_state) return fail(c, LBM_ESTATE, "no state");
-    const int nb = 1024;
-    if (int rc = dev_alloc(c, c->partial, (size_t)nb * 4 * sizeof(double))) return rc;
+    const int nb = totals_blocks((long long)c->d.nx * c->d.ny * c->d.nz);
+    if (int rc = dev_alloc(c, c->partial, (size_t)nb * kTotalsSlots * sizeof(double))) return rc;
     HIPCHK(c, c->ops->totals(c->lay, c->buf[c->cur], c->pending ? 0 : 1, static_cast<double*>(c->partial.p), nb,
                              c->stream));
-    std::vector<double> h((size_t)nb * 4);
+    std::vector<double> h((size_t)nb * kTotalsSlots);
     HIPCHK(c, hipMemcpyAsync(h.data(), c->partial.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    double t[4] = {0, 0, 0, 0};
-    for (int b = 0; b < nb; ++b)
-        for (int q = 0; q < 4; ++q) t[q] += h[(size_t)4 * b + q];
-    if (mass) *mass = t[0];
-    if (mom) { mom[0] = t[1]; mom[1] = t[2]; mom[2] = t[3]; }
-    return LBM_OK;
+    return finish_totals(c, h.data(), nb, mass, mom);
 }
 
 int lbm_get_info(lbm_ctx* c, int64_t* bytes, int32_t* pitch, int64_t* plane) {

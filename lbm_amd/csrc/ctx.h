@@ -88,6 +88,9 @@ int fail(lbm_ctx* c, int code, const std::string& msg);
 // LBM_OK for a live context handle; otherwise LBM_EINVAL with the message kept for
 // lbm_last_error(NULL)
 int check_ctx(const lbm_ctx* c);
+// mass, momentum and the instability test from the totals kernel's nb per-block partial results
+// (LBM_EUNSTABLE on a non-finite population or a cell density <= 0; the outputs are set either way)
+int finish_totals(lbm_ctx* c, const double* partial, int nb, double* mass, double* mom);
 // LBM_ESTATE while a split step (lbm_step_part) is open
 int check_no_part(lbm_ctx* c);
 int hipfail(lbm_ctx* c, hipError_t e, const char* where);

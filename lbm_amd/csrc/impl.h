@@ -102,6 +102,15 @@ struct TurbHost {
     void* F[3];           // force planes
 };
 
+// Totals (totals_kernel): partial results per block, and the block count for a grid of n interior
+// cells (one 256-thread block per 256 cells, at most kTotalsMaxBlocks, grid-stride beyond).
+constexpr int kTotalsSlots = 6;
+constexpr int kTotalsMaxBlocks = 1024;
+inline int totals_blocks(long long ncell) {
+    const long long b = (ncell + 255) / 256;
+    return (int)(b < 1 ? 1 : (b > kTotalsMaxBlocks ? kTotalsMaxBlocks : b));
+}
+
 // Objective box (inclusive, 1-based interior indices i, j, k).
 struct AdjBox {
     int lo[3], hi[3];

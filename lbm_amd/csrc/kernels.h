@@ -1255,36 +1255,47 @@ __global__ void average_kernel(Layout L, const void* cur, int stream_interior, t
     p[5 * P] += uz * uz;
 }
 
+// Global mass and momentum of the state (interior cells) in FP64, with the smallest cell density
+// and the number of non-finite populations (the instability test of src/m_diag.F90:169-173):
+// per-block partial results partial[kTotalsSlots*block + {0..5}] = (mass, mx, my, mz, min rho,
+// non-finite count), finished on the host.  Every block writes all of its slots.
 template <int Q, class C>
 __global__ void __launch_bounds__(256) totals_kernel(Layout L, const void* cur, int stream_interior, double* partial) {
     using Lt = Lattice<Q>;
-    __shared__ double sh[4][256];
+    __shared__ double sh[kTotalsSlots][256];
     const long long ncell = (long long)L.nx * L.ny * L.nz;
-    double m = 0, mx = 0, my = 0, mz = 0;
+    double m = 0, mx = 0, my = 0, mz = 0, rmin = INFINITY, bad = 0;
     const int s = stream_interior;
     for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += (long long)gridDim.x * blockDim.x) {
         const int i = (int)(c % L.nx) + 1;
         const int j = (int)((c / L.nx) % L.ny) + 1;
         const int k = (int)(c / ((long long)L.nx * L.ny)) + 1;
+        double r = 0;
 #pragma unroll
         for (int l = 0; l < Q; ++l) {
             const double f = (double)ld<Q, C>(L, cur, l, L.cell_w(i - s * Lt::cx(l), j - s * Lt::cy(l), k - s * Lt::cz(l)));
-            m += f;
+            if (!isfinite(f)) bad += 1;
+            r += f;
             mx += Lt::cx(l) * f;
             my += Lt::cy(l) * f;
             mz += Lt::cz(l) * f;
         }
+        m += r;
+        rmin = r < rmin ? r : rmin;
     }
     const int t = threadIdx.x;
-    sh[0][t] = m; sh[1][t] = mx; sh[2][t] = my; sh[3][t] = mz;
+    sh[0][t] = m; sh[1][t] = mx; sh[2][t] = my; sh[3][t] = mz; sh[4][t] = rmin; sh[5][t] = bad;
     __syncthreads();
     for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if (t < w)
+        if (t < w) {
             for (int q = 0; q < 4; ++q) sh[q][t] += sh[q][t + w];
+            sh[4][t] = sh[4][t + w] < sh[4][t] ? sh[4][t + w] : sh[4][t];
+            sh[5][t] += sh[5][t + w];
+        }
         __syncthreads();
     }
     if (t == 0)
-        for (int q = 0; q < 4; ++q) partial[4 * blockIdx.x + q] = sh[q][0];
+        for (int q = 0; q < kTotalsSlots; ++q) partial[kTotalsSlots * blockIdx.x + q] = sh[q][0];
 }
 
 }  // namespace lbm
