@@ -128,14 +128,19 @@ def read_restart(path: str) -> np.ndarray:
         nx, ny, nz, nl = _peek_ints(fh, 4)
         ncell = nl * (nx + 2) * (ny + 2) * (nz + 2)
         size = os.path.getsize(path)
+        # the file holds one record (8 bytes of markers: flang at any size, gfortran below 2 GiB) or
+        # a gfortran chain of ceil(payload / (2^31 - 9)) sub-records (8 bytes of markers each)
         for dt in (np.float32, np.float64):
             expected = 16 + ncell * np.dtype(dt).itemsize
-            framing = size - expected
-            if framing == 8 or (framing > 8 and framing % 8 == 0 and expected > _GF_MAX):
+            if size == expected + 8:
+                chain = False
+                break
+            if size == expected + 8 * (-(-expected // _GF_MAX)):
+                chain = True
                 break
         else:
             raise ValueError(f"{path}: size {size} fits neither FP32 nor FP64 for {nx}x{ny}x{nz}x{nl}")
-        data = read_record(fh, expected if framing == 8 else None)
+        data = read_record(fh, None if chain else expected)
     return np.frombuffer(data[16:], dtype=dt).reshape(nz + 2, ny + 2, nx + 2, nl).copy()
 
 
