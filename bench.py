@@ -94,39 +94,59 @@ def traffic_from_profiles(name):
         return None
 
 
-def cpu_baseline(w, f0, turb, sample_steps, threads):
+def default_cpu_threads():
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return min(n, int(env)) if env.isdigit() and int(env) > 0 else n
+
+
+def cpu_baseline(w, f0, turb, sample_steps, threads, sample_steps_1t=0):
     """The reference's own OpenMP build (oracle/_ref, amdflang) timed on this host on a bounded
-    sample of the same workload; its "Total TS routines" timer (src/m_wtime.F90:75)."""
+    sample of the same workload, by its "Total TS routines" timer (src/m_wtime.F90:75).  The build
+    carries the workload's cylinder (r=12 at (96,96)) in place of the reference's hard-coded one;
+    with sample_steps_1t > 0 a second, shorter run on one thread."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import build_ref  # noqa: E402
     import oracle  # noqa: E402
-    exe = build_ref.exe_path(w["q"], "sp", w["dims"], omp=True)
+    ic, jc, r = w["cyl"]
+    exe = build_ref.exe_path(w["q"], "sp", w["dims"], omp=True, cylinder=(ic, jc, r))
     if not os.path.exists(exe):
         try:
-            exe = build_ref.build(w["q"], "sp", w["dims"], omp=True)
+            exe = build_ref.build(w["q"], "sp", w["dims"], omp=True, cylinder=(ic, jc, r))
         except Exception:
             return None
     nx, ny, nz = w["dims"]
-    nr = sample_steps + 5
-    tf = tuple(a[: nr + 1] for a in turb)
-    wd = tempfile.mkdtemp(prefix="lbm_cpu_")
-    try:
-        _, out = oracle.run_reference(exe, f0, nt0=1, nt1=1 + sample_steps, turb_fields=tf, workdir=wd, read_output=False,
-                                      threads=threads, ibgk=3, ihrr=1, ivreman=1, smagorinsky=float(np.float32(SMAG)),
-                                      ibnd=1, jbnd=0, kbnd=12, uini=w["uini"], udir=0.0, visc=w["kinevisc"],
-                                      ampl=w["ampl"], nrturb=nr, experiment="cylinder", ltiming=True,
-                                      ltesting=False, ldump=False)
-    finally:
-        shutil.rmtree(wd, ignore_errors=True)
-    m = re.search(r"Total TS routines\s+time\s*=\s*([0-9.Ee+-]+)", out)
-    if not m:
+
+    def run(steps, nthreads):
+        nr = steps + 5
+        tf = tuple(x[: nr + 1] for x in turb)
+        wd = tempfile.mkdtemp(prefix="lbm_cpu_")
+        try:
+            _, out = oracle.run_reference(exe, f0, nt0=1, nt1=1 + steps, turb_fields=tf, workdir=wd,
+                                          read_output=False, threads=nthreads, ibgk=3, ihrr=1, ivreman=1,
+                                          smagorinsky=float(np.float32(SMAG)), ibnd=1, jbnd=0, kbnd=12,
+                                          uini=w["uini"], udir=0.0, visc=w["kinevisc"], ampl=w["ampl"], nrturb=nr,
+                                          experiment="cylinder", ltiming=True, ltesting=False, ldump=False)
+        finally:
+            shutil.rmtree(wd, ignore_errors=True)
+        m = re.search(r"Total TS routines\s+time\s*=\s*([0-9.Ee+-]+)", out)
+        return float(m.group(1)) if m else None
+
+    secs = run(sample_steps, threads)
+    if secs is None:
         return None
-    secs = float(m.group(1))
-    mlups = nx * ny * nz * sample_steps / secs / 1e6
-    return {"value": round(mlups, 3), "unit": "MLUPS", "cores": threads, "kind": "reference",
-            "sample": f"{sample_steps} steps of the same 384x192x128 D3Q27 reg+Vreman workload "
-                      f"(reference's own cylinder r=7 at i=50), amdflang -O2 -fopenmp, OMP_NUM_THREADS={threads}, "
-                      f"'Total TS routines' timer {secs:.2f} s"}
+    res = {"value": round(nx * ny * nz * sample_steps / secs / 1e6, 3), "unit": "MLUPS", "cores": threads,
+           "kind": "reference",
+           "sample": f"{sample_steps} steps of the same 384x192x128 D3Q27 reg+Vreman workload (same inputs, "
+                     f"cylinder r={r} at ({ic},{jc})), reference built with amdflang -O2 -fopenmp, "
+                     f"OMP_NUM_THREADS={threads}, 'Total TS routines' timer {secs:.2f} s",
+           "affinity_cpus": len(os.sched_getaffinity(0))}
+    if sample_steps_1t > 0:
+        s1 = run(sample_steps_1t, 1)
+        if s1:
+            res["one_thread"] = {"value": round(nx * ny * nz * sample_steps_1t / s1 / 1e6, 3), "unit": "MLUPS",
+                                 "cores": 1, "sample": f"{sample_steps_1t} steps, OMP_NUM_THREADS=1, {s1:.2f} s"}
+    return res
 
 
 def bench_ensemble(a):
@@ -432,8 +452,17 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--storage", default="f32", choices=["f32", "f64", "f16s", "bf16s"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=8)
-    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
+    ap.add_argument("--cpu-steps", type=int, default=8, help="CPU baseline sample, all threads")
+    ap.add_argument("--cpu-steps-1t", type=int, default=2, help="CPU baseline sample on one thread (0: skip)")
+    ap.add_argument("--cpu-threads", type=int, default=default_cpu_threads(),
+                    help="threads of the CPU baseline (default: the CPUs this process may run on, at most "
+                         "OMP_NUM_THREADS when that is set)")
+    ap.add_argument("--ramp-ms", type=float, default=300.0,
+                    help="untimed steps for this many ms before the warm-up (GPU clock ramp)")
+    ap.add_argument("--no-low-precision", action="store_true",
+                    help="headline: skip the FP16/BF16-storage timing and the error runs")
+    ap.add_argument("--error-steps", type=int, default=1000,
+                    help="headline: step count of the storage-error comparison against FP64 (0: skip)")
     ap.add_argument("--block", type=int, nargs=2, default=(0, 0))
     ap.add_argument("--workload", default="headline", choices=["headline", "ensemble", "adjoint", "tile", "q19"])
     ap.add_argument("--tiles", type=int, default=8)
@@ -460,66 +489,146 @@ def main():
         bench_q19(a)
         return
 
-    from lbm_amd.domain import Domain
-    w = workload(a.storage)
-    nx, ny, nz = w["dims"]
-    nrturb = max(1001, a.steps + a.warmup + 1)
-    print(f"[bench] building inputs {w['dims']} q={w['q']} storage={a.storage}", file=sys.stderr, flush=True)
-    f0, turb, mask = make_inputs(w, nrturb)
+    headline(a)
 
-    dom = Domain(w["q"], w["dims"], a.storage, ibgk=w["ibgk"], ihrr=w["ihrr"], ivreman=w["ivreman"],
+
+STORAGE_DTYPE = {"f32": "f32", "f64": "f64", "f16s": "f16-store/f32", "bf16s": "bf16-store/f32"}
+
+
+def run_timed(w, storage, f0, turb, mask, nrturb, a):
+    """One storage form of the headline workload: untimed clock ramp and warm-up, then exactly
+    a.steps timed steps.  HIP events bracket the step kernel of every max(1, steps // 10)-th timed
+    step (at least 10 samples from 10 steps on); the wall clock brackets the whole timed region."""
+    from lbm_amd.domain import Domain
+    dom = Domain(w["q"], w["dims"], storage, ibgk=w["ibgk"], ihrr=w["ihrr"], ivreman=w["ivreman"],
                  smagorinsky=w["smagorinsky"], kinevisc=w["kinevisc"], ibnd=w["ibnd"], jbnd=w["jbnd"],
                  kbnd=w["kbnd"], iturb_pos=w["ip"], block=tuple(a.block))
     dom.set_state(f0)
     dom.set_inflow(w["uini"], w["udir"])
     dom.set_blanking(mask)
     dom.set_turbulence(turb[0], turb[1], turb[2], nrturb, w["ampl"])
-    print(f"[bench] warmup {a.warmup}", file=sys.stderr, flush=True)
-    dom.step(1, a.warmup)
-    dom.synchronize()
-    # HIP events around the step kernel of every 10th step of the timed region (a record costs the
-    # stream a few microseconds; bracketing every step would add that to every step)
-    dom.set_timing(True, every=10)
-    print(f"[bench] timed {a.steps} steps", file=sys.stderr, flush=True)
+    it = 1
+    # clock ramp (untimed): step until a.ramp_ms of wall time has passed
     t0 = time.perf_counter()
-    dom.step(1 + a.warmup, a.steps)
+    while (time.perf_counter() - t0) * 1e3 < a.ramp_ms:
+        dom.step(it, 10)
+        it += 10
+        dom.synchronize()
+    print(f"[bench] {storage}: ramp {it - 1} steps, warmup {a.warmup}", file=sys.stderr, flush=True)
+    dom.step(it, a.warmup)
+    it += a.warmup
     dom.synchronize()
-    t1 = time.perf_counter()
+    every = max(1, a.steps // 10)
+    dom.set_timing(True, every=every)
+    print(f"[bench] {storage}: timed {a.steps} steps (kernel events every {every})", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    dom.step(it, a.steps)
+    dom.synchronize()
+    secs = time.perf_counter() - t0
     tm = dom.timing_read()
     rho, u, v, ww = dom.macrovars()
     finite = bool(np.isfinite(rho).all() and np.isfinite(u).all())
     dom.close()
-
-    cells = nx * ny * nz
-    secs = t1 - t0
-    mlups = cells * a.steps / secs / 1e6
+    cells = w["dims"][0] * w["dims"][1] * w["dims"][2]
+    bpc = bytes_per_cell(w["q"], storage)
     kern_ms = tm["step_ms"] / max(tm["n_step"], 1)
-    bpc = bytes_per_cell(w["q"], a.storage)
-    achieved = bpc * cells / (kern_ms * 1e-3) / 1e9
+    ms = secs / a.steps * 1e3
+    return {"mlups": cells * a.steps / secs / 1e6, "ms_per_step": ms, "kernel_ms": kern_ms,
+            "kernel_samples": tm["n_step"], "bc_ms": tm["bc_ms"] / max(tm["n_bc"], 1),
+            "achieved": bpc * cells / (kern_ms * 1e-3) / 1e9,
+            "achieved_wall": bpc * cells / (ms * 1e-3) / 1e9, "bpc": bpc, "finite": finite}
+
+
+def storage_errors(w, storages, checkpoints, seed):
+    """Rel-L2 errors of each storage form against FP64 storage (FP64 arithmetic) on the headline
+    workload, every run from the equilibrium of the same FP64 moments."""
+    from lbm_amd import inputs, precision_study
+    dims = w["dims"]
+    q, opts, cyl = precision_study.physics("regvre", dims)
+    assert q == w["q"] and cyl == w["cyl"] and opts["kinevisc"] == w["kinevisc"]
+    nrturb = 250
+    moments = inputs.perturbed_moments(dims, seed, dtype=np.float64)
+    turb = inputs.turbulence_fields(seed, dims[1], dims[2], nrturb, corr=w["corr"], dtype=np.float64)
+    ref = precision_study.run("f64", q, opts, dims, cyl, moments, turb, checkpoints, nrturb)
+    out = {}
+    for st in storages:
+        x = precision_study.run(st, q, opts, dims, cyl, moments, turb, checkpoints, nrturb)
+        out[st] = {str(n): precision_study.compare(x[n], ref[n]) for n in checkpoints}
+        out[st]["finite"] = bool(all(np.isfinite(x[n][1]).all() for n in checkpoints))
+        del x
+    return out
+
+
+def roof_of(r, traffic, traffic_src):
+    return {"bound": "hbm", "achieved": round(r["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(r["achieved"] / HBM_PEAK_GBS, 4),
+            "frac_wall": round(r["achieved_wall"] / HBM_PEAK_GBS, 4),
+            "traffic": traffic, "traffic_source": traffic_src,
+            "kernel": "step_kernel", "bytes_per_cell": r["bpc"], "kernel_ms": round(r["kernel_ms"], 5),
+            "kernel_samples": r["kernel_samples"], "bc_ms_per_step": round(r["bc_ms"], 5)}
+
+
+def headline(a):
+    w = workload(a.storage)
+    nx, ny, nz = w["dims"]
+    # turbulence slices for every step index the run reaches (ramp included)
+    # one turbulence slice per step of the warm-up and timed region (the ramp's steps wrap,
+    # lit = mod(it, nrturb) as in src/m_inflow_turbulence_forcing.F90:45-46)
+    nrturb = max(1001, a.steps + a.warmup + 1)
+    print(f"[bench] building inputs {w['dims']} q={w['q']} storage={a.storage}", file=sys.stderr, flush=True)
+    f0, turb, mask = make_inputs(w, nrturb)
+    cells = nx * ny * nz
+    res = run_timed(w, a.storage, f0, turb, mask, nrturb, a)
     kname = "step_kernel_q27_reg_vreman_" + a.storage
-    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(kname),
-            "kernel": "step_kernel", "bytes_per_cell": bpc, "kernel_ms": round(kern_ms, 5),
-            "bc_ms_per_step": round(tm["bc_ms"] / max(tm["n_bc"], 1), 5)}
-    cpu = None
-    if not a.no_cpu_baseline and a.storage == "f32":
-        print(f"[bench] cpu baseline: reference OpenMP build, {a.cpu_steps} steps", file=sys.stderr, flush=True)
-        try:
-            cpu = cpu_baseline(w, f0, turb, a.cpu_steps, a.cpu_threads)
-        except Exception as e:  # the baseline is reported, never required
-            print(f"[bench] cpu baseline failed: {e}", file=sys.stderr)
     line = {
         "metric": "MLUPS + %HBM roofline (D3Q27 reg+Vreman), FP16-store vs FP32 at stated error",
-        "value": round(mlups, 1), "unit": "MLUPS", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
-        "ms_per_step": round(secs / a.steps * 1e3, 5), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": {"f32": "f32", "f64": "f64", "f16s": "f16-store/f32", "bf16s": "bf16-store/f32"}[a.storage],
+        "value": round(res["mlups"], 1), "unit": "MLUPS", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(res["ms_per_step"], 5), "higher_is_better": True, "scaling": "weak",
+        # the reference publishes no number for this metric (its README times a laptop GPU on
+        # another grid with unstated collision flags)
+        "vs_baseline": None, "dtype": STORAGE_DTYPE[a.storage],
         "data": "synthetic (seeded perturbed uniform flow, seeded smooth inflow forcing)",
         "config": {"workload": "D3Q27 3rd-order regularized + Vreman (c=0.07), 384x192x128, inflow/outflow i, "
                                "periodic j, no-slip k=1 / free-slip k=nz, cylinder r=12 at (96,96), "
                                "inflow forcing at i=10, nu=1e-4",
                    "cells": cells, "storage": a.storage, "parallelism": "single GPU"},
-        "roofline": roof, "cpu_baseline": cpu, "finite": finite,
+        "roofline": roof_of(res, traffic_from_profiles(kname),
+                            f"profiles/{kname}.json (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; copy-kernel scaled)"),
+        "finite": res["finite"],
     }
+    if a.storage == "f32" and not a.no_low_precision:
+        # the FP16 half of the metric: the same workload with weight-shifted FP16 (and BF16) storage
+        lp = {}
+        for st in ("f16s", "bf16s"):
+            r = run_timed(w, st, f0, turb, mask, nrturb, a)
+            kn = "step_kernel_q27_reg_vreman_" + st
+            lp[st] = {"value": round(r["mlups"], 1), "unit": "MLUPS", "ms_per_step": round(r["ms_per_step"], 5),
+                      "ratio_vs_f32": round(r["mlups"] / res["mlups"], 3),
+                      "roofline": roof_of(r, traffic_from_profiles(kn),
+                                          f"profiles/{kn}.json (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)"),
+                      "finite": r["finite"]}
+        if a.error_steps:
+            cps = sorted(set([100] + [a.error_steps]))
+            print(f"[bench] storage errors vs FP64 after {cps} steps", file=sys.stderr, flush=True)
+            errs = storage_errors(w, ("f32", "f16s", "bf16s"), cps, seed=w["seed"])
+            for st in ("f16s", "bf16s"):
+                lp[st]["error_vs_f64"] = errs[st]
+            line["error_vs_f64"] = errs["f32"]
+            line["error_method"] = (
+                "relative L2 over interior cells of the FP64-storage (FP64 arithmetic) run: 'velocity' is "
+                "||(u,v,w) - (u,v,w)_64|| / ||(u,v,w)_64||; per field rho-1, u, v, w; mean_profile and "
+                "tke_profile are the max deviations of the i,j-mean u(k) and TKE(k) relative to their maxima; "
+                "every run starts from the equilibrium of the same seeded FP64 moments")
+        line["f16_storage"] = lp["f16s"]
+        line["bf16_storage"] = lp["bf16s"]
+    cpu = None
+    if not a.no_cpu_baseline and a.storage == "f32":
+        print(f"[bench] cpu baseline: reference OpenMP build, {a.cpu_steps} steps", file=sys.stderr, flush=True)
+        try:
+            cpu = cpu_baseline(w, f0, turb, a.cpu_steps, a.cpu_threads, a.cpu_steps_1t)
+        except Exception as e:  # the baseline is reported, never required
+            print(f"[bench] cpu baseline failed: {e}", file=sys.stderr)
+    line["cpu_baseline"] = cpu
     print(json.dumps(line))
 
 

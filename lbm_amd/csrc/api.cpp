@@ -432,11 +432,10 @@ int lbm_create(const lbm_desc* desc, lbm_ctx** out) {
     c->d.block_y = by;
     if (c->d.iturb_pos == 0) c->d.iturb_pos = 10;
     c->ops = ops;
-    // LBM_STEP_VARIANT overrides the step-kernel variant (bit 0 non-temporal stores, bit 2 two
-    // cells per lane).  One cell per lane with non-temporal stores measured fastest for every
-    // storage format on MI355X (FP32 23.4k, FP16 38.3k MLUPS vs 22.1k / 35.7k with pairs).
-    if (const char* v = getenv("LBM_STEP_VARIANT")) c->variant = atoi(v) & 5;
-    if ((c->variant & 4) && (lay.n(lay.ax0) % 2 != 0 || ops->raw_bytes > 4)) c->variant &= ~4;
+    // LBM_STEP_VARIANT overrides the step-kernel variant: bit 0 non-temporal stores (the default),
+    // bit 1 the form for population buffers of 4 GiB and more (64-bit plane offsets) whatever the
+    // buffer's size, which lets small grids exercise that path.
+    if (const char* v = getenv("LBM_STEP_VARIANT")) c->variant = atoi(v) & 3;
     {
         // With one closed axis (no edges, corners or snapshots then) the step kernel can perform
         // that axis's face pass itself, and with at most that one closed axis the inflow/outflow
@@ -452,8 +451,7 @@ int lbm_create(const lbm_desc* desc, lbm_ctx** out) {
         // halo domains always fold: the separate boundary kernels do not know halo planes
         const bool fold = d.jbnd == LBM_BND_HALO || (fv ? atoi(fv) != 0 : kFoldDefault);
         c->fold_on = c->fold_dir >= 0 && fold;
-        if (d.jbnd == LBM_BND_HALO) c->variant &= ~4;
-        c->inflow_fold_on = d.ibnd == 1 && fold && (nclosed == 0 || c->fold_on) && !(c->variant & 4);
+        c->inflow_fold_on = d.ibnd == 1 && fold && (nclosed == 0 || c->fold_on);
         // LBM_INFLOW_FOLD=0 keeps the face folded but writes the inflow/outflow planes with their own
         // kernel (measured slower: tile 384x24x128 14.0 k vs 16.2 k MLUPS, whole domain -1 %)
         if (const char* iv = getenv("LBM_INFLOW_FOLD")) c->inflow_fold_on = c->inflow_fold_on && atoi(iv) != 0;

@@ -55,7 +55,7 @@ struct StepHost {
     int halo_dir;                             // halo axis (-1: none)
     int part, part_axis, part_edge;           // split step along the halo axis (StepArgs::part)
     int bgk3, hrr, vre;
-    int variant;      // step-kernel variant: bit 0 non-temporal stores, bit 2 two cells per lane
+    int variant;      // step-kernel variant: bit 0 non-temporal stores, bit 1 the >= 4 GiB buffer form
     int block_x, block_y;
 };
 

@@ -637,109 +637,6 @@ __global__ void __launch_bounds__(sizeof(typename C::raw) >= 8 ? 256 : 1024, siz
     step_cell<Q, C, BGK3, HRR, VRE, VAR>(a, bx * blockDim.x + threadIdx.x + 1, cb, cc);
 }
 
-// Two raw values packed in one aligned word: 32 bits for 16-bit storage, 64 bits for FP32.
-template <typename raw>
-struct PairWord;
-template <>
-struct PairWord<unsigned short> {
-    using W = unsigned int;
-    __device__ static inline unsigned short half(W w, int hi) { return (unsigned short)(hi ? (w >> 16) : (w & 0xffffu)); }
-    __device__ static inline W pack(unsigned short lo, unsigned short hi) { return (W)lo | ((W)hi << 16); }
-};
-template <>
-struct PairWord<float> {
-    using W = unsigned long long;
-    __device__ static inline float half(W w, int hi) { return __uint_as_float((unsigned)(hi ? (w >> 32) : (w & 0xffffffffull))); }
-    __device__ static inline W pack(float lo, float hi) {
-        return (W)__float_as_uint(lo) | ((W)__float_as_uint(hi) << 32);
-    }
-};
-
-// Two adjacent cells along the contiguous axis per lane.  Every access is an aligned word of two
-// raw values, so a 64-lane wave moves 2*64*sizeof(raw) bytes per population (256 B for 16-bit
-// storage, where one value per lane would move only one 128-B line per instruction).  A
-// population shifted along the contiguous axis takes its two values from two aligned words.
-// Needs an even number of cells along the contiguous axis.
-// Threads are numbered over (pair, row) of one ax2 slab so that no lane idles when a row holds
-// a non-multiple of 64 pairs; blocks of at most 256 threads leave the compiler room to keep both
-// cells in registers.
-template <int Q, class C, bool BGK3, bool HRR, bool VRE, int VAR, int AX0>
-__global__ void __launch_bounds__(256) step_kernel_pair(const StepArgs<typename C::real> a) {
-    using R = typename C::real;
-    using raw = typename C::raw;
-    using PW = PairWord<raw>;
-    using W = typename PW::W;
-    using Lt = Lattice<Q>;
-    const Layout& L = a.lay;
-    const int n0 = L.n(L.ax0);
-    const int npair = n0 >> 1;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= npair * L.n(L.ax1)) return;
-    const int ca = 2 * (t % npair) + 1;
-    const int cb = t / npair + 1;
-    const int cc = blockIdx.y + 1;
-    int i, j, k, i2, j2, k2;
-    L.ijk(ca, cb, cc, i, j, k);
-    L.ijk(ca + 1, cb, cc, i2, j2, k2);
-    const int s = a.stream;
-    // Offsets of the first cell along i, j, k (index d = c + 1), and of the second cell's
-    // position along the contiguous axis.
-    int o1[3][3], o2f[3];
-    const int xc[3] = {i, j, k}, n[3] = {L.nx, L.ny, L.nz}, per[3] = {L.per_i, L.per_j, L.per_k};
-    const int st[3] = {L.si, L.sj, L.sk};
-#pragma unroll
-    for (int ax = 0; ax < 3; ++ax)
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-            o1[ax][d] = Layout::wrap(xc[ax] - s * (d - 1), n[ax], per[ax]) * st[ax];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) o2f[d] = Layout::wrap(ca + 1 - s * (d - 1), n0, per[AX0]);
-    const raw* __restrict__ src = static_cast<const raw*>(a.src);
-    raw rA[Q], rB[Q];
-#pragma unroll
-    for (int l = 0; l < Q; ++l) {
-        const int c[3] = {Lt::cx(l), Lt::cy(l), Lt::cz(l)};
-        const int cf = c[AX0];   // velocity component along the contiguous axis (compile time)
-        // slow-axis part of the offset (the contiguous axis has stride 1)
-        const int offA = o1[0][c[0] + 1] + o1[1][c[1] + 1] + o1[2][c[2] + 1];
-        const size_t pl = (size_t)l * (size_t)L.plane + (size_t)L.base;
-        const long long eA = (long long)pl + offA;
-        // No run-time branch here: a branch per population would split the loads into basic
-        // blocks and serialise their latencies.  The two-word form is also right for s == 0
-        // (both words are then the cell pair's own word).
-        if (cf == 0) {
-            const W w = *reinterpret_cast<const W*>(src + eA);
-            rA[l] = PW::half(w, 0);
-            rB[l] = PW::half(w, 1);
-        } else {
-            const int fA = o1[AX0][cf + 1];
-            const long long eB = eA - fA + o2f[cf + 1];
-            const W wa = *reinterpret_cast<const W*>(src + (eA & ~1LL));
-            const W wb = *reinterpret_cast<const W*>(src + (eB & ~1LL));
-            rA[l] = PW::half(wa, (int)(eA & 1));
-            rB[l] = PW::half(wb, (int)(eB & 1));
-        }
-    }
-    const int oc = L.cell(i, j, k);
-    const int ox = L.cidx(i, j, k);   // per-cell arrays (the pair is adjacent along ax0 there too)
-    R outA[Q], outB[Q];
-    R tauA = 0, tauB = 0;
-    collide_cell<Q, C, BGK3, HRR, VRE>(a, rA, i, j, k, ca, cb, cc, ox, outA, tauA);
-    collide_cell<Q, C, BGK3, HRR, VRE>(a, rB, i2, j2, k2, ca + 1, cb, cc, ox + 1, outB, tauB);
-    raw* __restrict__ dst = static_cast<raw*>(a.dst);
-#pragma unroll
-    for (int l = 0; l < Q; ++l) {
-        W* p = reinterpret_cast<W*>(dst + (size_t)l * (size_t)L.plane + (size_t)oc);
-        const W w = PW::pack(C::template enc_dep<Q>(outA[l], l), C::template enc_dep<Q>(outB[l], l));
-        if (VAR & 1) __builtin_nontemporal_store(w, p);
-        else *p = w;
-    }
-    if (VRE && a.tau_out != nullptr) {
-        a.tau_out[ox] = tauA;
-        a.tau_out[ox + 1] = tauB;
-    }
-}
-
 // ------------------------------------------------------------------------------------------
 // Boundary kernels (non-periodic ghost layers).  f1 is the buffer the step just wrote, f2 the
 // other one; reads of ghost cells in periodic directions wrap onto the interior.

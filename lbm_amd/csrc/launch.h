@@ -42,24 +42,8 @@ struct Launch {
     template <bool B3, bool H, bool V>
     static hipError_t step_t(const Layout& L, const StepArgs<R>& a, dim3 grid, dim3 block, int var, hipStream_t s) {
         const bool nt = var & 1;
-        if constexpr (sizeof(typename C::raw) <= 4) {
-            if ((var & 4) && a.fold_dir < 0 && !a.inflow_fold && !a.part) {
-                // two cells per lane along the contiguous axis
-                const long long nt_slab = (long long)(L.n(L.ax0) / 2) * L.n(L.ax1);
-                const dim3 g((unsigned)((nt_slab + 255) / 256), L.n(L.ax2), 1);
-                // the contiguous axis is a template parameter: velocity components along it
-                // must be compile-time constants to keep the loads free of branches
-                switch (L.ax0) {
-                    case 0: hipLaunchKernelGGL((step_kernel_pair<Q, C, B3, H, V, 1, 0>), g, dim3(256, 1, 1), 0, s, a); break;
-                    case 1: hipLaunchKernelGGL((step_kernel_pair<Q, C, B3, H, V, 1, 1>), g, dim3(256, 1, 1), 0, s, a); break;
-                    default: hipLaunchKernelGGL((step_kernel_pair<Q, C, B3, H, V, 1, 2>), g, dim3(256, 1, 1), 0, s, a); break;
-                }
-                (void)nt;   // the pair kernel always stores non-temporally
-                return hipGetLastError();
-            }
-        }
-        // one 32-bit offset range covers every plane of the buffer
-        const bool small = (double)L.total * sizeof(typename C::raw) < 4294967296.0;
+        // one 32-bit offset range covers every plane of the buffer (var bit 1 forces the large form)
+        const bool small = (double)L.total * sizeof(typename C::raw) < 4294967296.0 && !(var & 2);
         if constexpr (kNT) {
             if (nt && small) hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 9>), grid, block, 0, s, a);
             else if (nt) hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 1>), grid, block, 0, s, a);

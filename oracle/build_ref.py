@@ -18,6 +18,11 @@ Nothing from the reference is copied into the repository.
   and compiles it in place of the reference's copy.
 * Flags follow src/makefile:82-113 (``-DDOUBLE_PRECISION -fdefault-real-8`` for DP,
   ``-fopenmp -DOPEN_MP`` for MP, ``-DD3Q19``) with amdflang's spelling.
+* The cylinder of ``experiment cylinder`` is hard-coded (radius 7 at i=50, j=nyg/2,
+  src/m_cylinder.F90:8).  With ``cylinder=(ipos, jpos, radius)`` the recipe writes its own
+  ``m_cylinder`` module (same interface, same inside test ``(i-ipos)**2+(j-jpos)**2 <
+  radius**2`` over interior i, j and every k) with those parameters and compiles it in place
+  of the reference's copy, as it does for ``mod_dimensions``.
 * The MPI modules (src/m_mpi_*.F90) and NetCDF output are left out (no -DMPI/-DNETCDF).
 * FFTW3 is absent from the image.  Only the inflow-turbulence generator calls it
   (src/m_pseudo2D.F90:78,150), so the link leaves those three symbols unresolved
@@ -38,18 +43,19 @@ OUT = os.path.join(HERE, "_ref")
 FC = "/opt/rocm/bin/amdflang"
 
 
-def variant_name(q, prec, dims, omp):
+def variant_name(q, prec, dims, omp, cylinder=None):
     nx, ny, nz = dims
-    return f"q{q}_{prec}_{nx}x{ny}x{nz}{'_omp' if omp else ''}"
+    cyl = f"_cyl{cylinder[0]}x{cylinder[1]}r{cylinder[2]}" if cylinder else ""
+    return f"q{q}_{prec}_{nx}x{ny}x{nz}{cyl}{'_omp' if omp else ''}"
 
 
-def exe_path(q, prec, dims, omp=False):
-    return os.path.join(OUT, "bin", variant_name(q, prec, dims, omp))
+def exe_path(q, prec, dims, omp=False, cylinder=None):
+    return os.path.join(OUT, "bin", variant_name(q, prec, dims, omp, cylinder))
 
 
-def _sources():
+def _sources(cylinder=None):
     files = sorted(f for f in os.listdir(REF_SRC) if f.endswith(".F90"))
-    skip = {"mod_dimensions.F90"}
+    skip = {"mod_dimensions.F90"} | ({"m_cylinder.F90"} if cylinder else set())
     return [f for f in files if f not in skip and not f.startswith("m_mpi_")]
 
 
@@ -91,9 +97,9 @@ def _topo_order(files, extra_modules):
     return order
 
 
-def build(q, prec, dims, omp=False, verbose=False):
-    name = variant_name(q, prec, dims, omp)
-    exe = exe_path(q, prec, dims, omp)
+def build(q, prec, dims, omp=False, verbose=False, cylinder=None):
+    name = variant_name(q, prec, dims, omp, cylinder)
+    exe = exe_path(q, prec, dims, omp, cylinder)
     if os.path.exists(exe):
         return exe
     if not os.path.isdir(REF_SRC):
@@ -113,6 +119,29 @@ def build(q, prec, dims, omp=False, verbose=False):
             f"   integer, parameter :: nz = {nz}\n"
             "   integer, parameter :: ntracer = 0\n"
             "end module\n")
+    own = [dimsrc]
+    if cylinder:
+        ipos, jpos, radius = (int(x) for x in cylinder)
+        cylsrc = os.path.join(bdir, "m_cylinder.F90")
+        with open(cylsrc, "w") as fh:
+            fh.write(
+                "module m_cylinder\n"
+                "contains\n"
+                "subroutine cylinder(blanking)\n"
+                "   use mod_dimensions, only : nx, nyg, nz\n"
+                "   implicit none\n"
+                "   logical, intent(inout) :: blanking(0:nx+1,0:nyg+1,0:nz+1)\n"
+                "   integer :: ii, jj\n"
+                f"   integer, parameter :: r = {radius}, ic = {ipos}, jc = {jpos}\n"
+                "   blanking = .false.\n"
+                "   do jj = 1, nyg\n"
+                "      do ii = 1, nx\n"
+                "         blanking(ii,jj,:) = (ii-ic)**2 + (jj-jc)**2 < r**2\n"
+                "      end do\n"
+                "   end do\n"
+                "end subroutine\n"
+                "end module\n")
+        own.append(cylsrc)
     flags = ["-cpp", "-O2"]
     if q == 19:
         flags.append("-DD3Q19")
@@ -121,7 +150,8 @@ def build(q, prec, dims, omp=False, verbose=False):
     if omp:
         flags += ["-fopenmp", "-DOPEN_MP"]
     objs = []
-    srcs = [dimsrc] + [os.path.join(REF_SRC, f) for f in _topo_order(_sources(), {"mod_dimensions"})]
+    # the project-written modules first: every reference module that uses them comes later
+    srcs = own + [os.path.join(REF_SRC, f) for f in _topo_order(_sources(cylinder), {"mod_dimensions"})]
     for src in srcs:
         obj = os.path.join(bdir, os.path.basename(src)[:-4] + ".o")
         cmd = [FC] + flags + ["-module-dir", bdir, "-I", bdir, "-c", src, "-o", obj]
@@ -140,9 +170,10 @@ def main():
     ap.add_argument("--prec", choices=("sp", "dp"), default="sp")
     ap.add_argument("--dims", type=int, nargs=3, required=True)
     ap.add_argument("--omp", action="store_true")
+    ap.add_argument("--cylinder", type=int, nargs=3, default=None, metavar=("IPOS", "JPOS", "RADIUS"))
     ap.add_argument("-v", "--verbose", action="store_true")
     a = ap.parse_args()
-    print(build(a.q, a.prec, tuple(a.dims), a.omp, a.verbose))
+    print(build(a.q, a.prec, tuple(a.dims), a.omp, a.verbose, tuple(a.cylinder) if a.cylinder else None))
 
 
 if __name__ == "__main__":

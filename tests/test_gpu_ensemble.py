@@ -96,6 +96,21 @@ def test_ensemble_matches_single_domain_steps(reference_runs, launch):
         d.close()
 
 
+def test_ensemble_plane_arrangement(reference_runs, monkeypatch):
+    """Members in the arrangement of population buffers of 4 GiB and more (one plane per
+    population), stepped by the batched kernels: the default-arrangement single-domain states."""
+    monkeypatch.setenv("LBM_LAYOUT", "planes")
+    monkeypatch.setenv("LBM_STEP_VARIANT", "3")
+    doms = make()
+    ens = Ensemble(doms)
+    ens.step(1, 1)
+    ens.step(2, 8)
+    assert_same(states(doms), reference_runs[9], "after 9 steps (planes)")
+    ens.close()
+    for d in doms:
+        d.close()
+
+
 def test_tuner_restores_state_and_keeps_results(reference_runs):
     doms = make()
     ens = Ensemble(doms)

@@ -194,6 +194,15 @@ def test_split_steps_match_untiled(name, splits):
     run_case(name, splits, split=True)
 
 
+@pytest.mark.parametrize("name", ["inflow_closedk_solids_turb", "closed_i_periodic_k"])
+def test_two_tiles_plane_arrangement(name, monkeypatch):
+    """Halo pack/unpack and the tiles' steps in the arrangement of population buffers of 4 GiB and
+    more (one plane per population, 64-bit plane offsets), forced on a small grid."""
+    monkeypatch.setenv("LBM_LAYOUT", "planes")
+    monkeypatch.setenv("LBM_STEP_VARIANT", "3")
+    run_case(name, [(1, 5), (6, 13)])
+
+
 def test_three_tiles_fp16():
     run_case("inflow_closedk_f16s", [(1, 4), (5, 9), (10, 13)])
 

@@ -1,5 +1,5 @@
-"""Low-precision population storage (weight-shifted FP16 and BF16, FP32 arithmetic) and the
-two-cells-per-lane kernel.
+"""Low-precision population storage (weight-shifted FP16 and BF16, FP32 arithmetic) and the step
+kernel's form for population buffers of 4 GiB and more.
 
 Error measure (as for the FP16/BF16 targets): relative L2 of the velocity field against the
 library's FP64 storage run on the same input, ||u - u64||_2 / ||u64||_2 over interior cells (u the
@@ -23,17 +23,18 @@ CFG2 = dict(ibgk=3, ihrr=1, ivreman=1, smagorinsky=0.16733, kinevisc=1e-4, ibnd=
 
 
 def run(storage, dims, steps, f0, tu, mask, env=None):
-    old = os.environ.get("LBM_STEP_VARIANT")
-    if env is not None:
-        os.environ["LBM_STEP_VARIANT"] = env
+    """env: variables set while the context is created (the library reads them in lbm_create)."""
+    env = env or {}
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
     try:
         dom = Domain(27, dims, storage, **CFG2)
     finally:
-        if env is not None:
-            if old is None:
-                del os.environ["LBM_STEP_VARIANT"]
+        for k, v in old.items():
+            if v is None:
+                del os.environ[k]
             else:
-                os.environ["LBM_STEP_VARIANT"] = old
+                os.environ[k] = v
     dom.set_state(f0)
     dom.set_inflow(0.05)
     dom.set_blanking(mask)
@@ -74,30 +75,44 @@ def test_lowprec_error_vs_fp64(case, storage, tol):
     assert e_32 <= 1e-5
 
 
-def test_pair_kernel_fp32_matches_oracle(case):
-    """The two-cells-per-lane kernel (forced for FP32) against the oracle."""
+# The form a population buffer of 4 GiB or more takes (D3Q27 FP32 at 768x256x192 is 4.16 GB): the
+# plane arrangement and 64-bit plane offsets.  LBM_LAYOUT=planes and variant bit 1 force it on a
+# small grid.
+LARGE = {"LBM_LAYOUT": "planes", "LBM_STEP_VARIANT": "3"}
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16s", "bf16s", "f64"])
+def test_large_buffer_form_matches_default(case, storage):
+    """The >= 4 GiB form gives the default form's states bit for bit (the same arithmetic, other
+    addressing), after steps through the folded face, the inflow/outflow planes, the cylinder and
+    the forcing slice."""
+    dims, f0, tu, mask = case
+    steps = 30
+    if storage != "f64":
+        f0 = f0.astype(np.float32)
+        tu = tuple(a.astype(np.float32) for a in tu)
+    print(f"large-buffer form {storage} {dims} {steps} steps", flush=True)
+    fa, _, _ = run(storage, dims, steps, f0, tu, mask)
+    fb, _, _ = run(storage, dims, steps, f0, tu, mask, env=LARGE)
+    assert np.isfinite(fb).all()
+    assert np.array_equal(fa, fb), float(np.abs(fa - fb).max())
+
+
+def test_large_buffer_form_fp32_matches_oracle(case):
+    """The >= 4 GiB form against the oracle (FP32)."""
     dims, f0, tu, mask = case
     steps = 60
     f0 = f0.astype(np.float32)
     tu32 = tuple(a.astype(np.float32) for a in tu)
-    f, _, _ = run("f32", dims, steps, f0, tu32, mask, env="5")
+    f, _, _ = run("f32", dims, steps, f0, tu32, mask, env=LARGE)
     so = {k: (float(np.float32(v)) if isinstance(v, float) else v) for k, v in CFG2.items()}
     sim = oracle.OracleSim(27, dims, "f32", **so)
     sim.restart(f0)
     sim.run(1, 1 + steps, uini=float(np.float32(0.05)), blank=mask,
             turb=(tu32[0], tu32[1], tu32[2], len(tu32[0]) - 1, float(np.float32(1e-3))))
     d = np.abs(f - sim.f1.astype(np.float64))
-    print(f"pair fp32 vs oracle: max|df| {d.max():.3e}", flush=True)
+    print(f"large-buffer form fp32 vs oracle: max|df| {d.max():.3e}", flush=True)
     assert d.max() <= 1e-5
-
-
-def test_pair_and_scalar_kernels_agree_fp16(case):
-    dims, f0, tu, mask = case
-    fa, _, ua = run("f16s", dims, 20, f0, tu, mask, env="5")   # pairs
-    fb, _, ub = run("f16s", dims, 20, f0, tu, mask, env="1")   # one cell per lane
-    d = np.abs(fa - fb).max()
-    print(f"f16 pair vs scalar max|df| {d:.3e}", flush=True)
-    assert d <= 1e-6
 
 
 @pytest.mark.parametrize("storage", ["f16s", "bf16s"])
