@@ -239,6 +239,15 @@ int lbm_get_averages(lbm_ctx* ctx, void* uave, void* vave, void* wave, void* ti,
  * density is <= 0 (src/m_diag.F90:169-173); the message gives the count and the smallest density. */
 int lbm_get_totals(lbm_ctx* ctx, double* mass, double* momentum3);
 
+/* lbm_get_totals that allocates nothing.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (tight serving loops, harnesses that forbid hipMalloc).  The
+ * totals kernel's per-block partial results go to a fixed array of the library on each GPU
+ * (at most 1024 blocks x 6 doubles = 48 KiB, held until the process ends; the kernel
+ * grid-strides, so every grid fits) instead of a context buffer allocated on first use.
+ * Outputs and status as lbm_get_totals.  Every call on one GPU shares that GPU's array: calls
+ * that overlap in time on one GPU (from several host threads, on any contexts) race. */
+int lbm_get_totals_noalloc(lbm_ctx* ctx, double* mass, double* momentum3);
+
 /* Bytes of HBM the two population buffers hold, the device layout pitch (elements per row) and
  * the elements of one per-cell array (the padded grid in rows of that pitch). */
 int lbm_get_info(lbm_ctx* ctx, int64_t* device_bytes, int32_t* row_pitch, int64_t* plane_elems);

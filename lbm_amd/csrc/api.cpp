@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -21,6 +22,36 @@ using namespace lbm;
 using namespace lbm_host;
 
 static thread_local std::string g_create_error;
+
+// Scratch of lbm_get_totals_noalloc: the totals kernel's per-block partial results, at most
+// kTotalsMaxBlocks blocks of kTotalsSlots doubles (48 KiB on each GPU, held until the process
+// ends).  In host code this name is only a placeholder: kernels and copies get each device's
+// own address, resolved by scratch_resolve() when a context is created on that device.
+static __device__ __align__(256) double scratch[kTotalsMaxBlocks * kTotalsSlots];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};         // device address of scratch, per device
+hipError_t g_scratch_err[kMaxDevices] = {};  // result of the lookup when it failed
+
+// With device dev current: look up that device's copy of scratch once.
+void scratch_resolve(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) return;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch[dev]) return;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e == hipSuccess && p) {
+        g_scratch[dev] = static_cast<double*>(p);
+        g_scratch_err[dev] = hipSuccess;
+    } else {
+        g_scratch_err[dev] = e == hipSuccess ? hipErrorInvalidSymbol : e;
+    }
+}
+
+}  // namespace
 
 namespace lbm_host {
 
@@ -457,6 +488,8 @@ int lbm_create(const lbm_desc* desc, lbm_ctx** out) {
         if (const char* iv = getenv("LBM_INFLOW_FOLD")) c->inflow_fold_on = c->inflow_fold_on && atoi(iv) != 0;
     }
     hipGetDevice(&c->device);
+    // lbm_get_totals_noalloc's scratch on this device (a failed lookup is reported by that call)
+    scratch_resolve(c->device);
     c->lay = lay;
     c->tau0 = round_real(c, round_real(c, 3.0 * round_real(c, d.kinevisc)) + 0.5);
     c->fac0 = round_real(c, 1.0 - round_real(c, 1.0 / c->tau0));
@@ -1113,6 +1146,31 @@ int lbm_get_totals(lbm_ctx* c, double* mass, double* mom) {
     HIPCHK(c, hipMemcpyAsync(h.data(), c->partial.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return finish_totals(c, h.data(), nb, mass, mom);
+}
+
+int lbm_get_totals_noalloc(lbm_ctx* c, double* mass, double* mom) {
+    if (int rc_ = check_ctx(c)) return rc_;
+    if (int rc = check_no_part(c)) return rc;
+    if (!c->have_state) return fail(c, LBM_ESTATE, "no state");
+    // the totals kernel grid-strides, so every grid needs at most kTotalsMaxBlocks partial results
+    const int nb = totals_blocks((long long)c->d.nx * c->d.ny * c->d.nz);
+    if (nb < 1 || nb > kTotalsMaxBlocks)
+        return fail(c, LBM_EINVAL, "totals: " + std::to_string(nb) + " partial results exceed the scratch of " +
+                                       std::to_string(kTotalsMaxBlocks));
+    double* p = nullptr;
+    hipError_t pe = hipErrorInvalidDevice;
+    if (c->device >= 0 && c->device < kMaxDevices) {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        p = g_scratch[c->device];
+        pe = g_scratch_err[c->device];
+    }
+    if (!p) return hipfail(c, pe, "hipGetSymbolAddress(scratch) for this context's device");
+    // every block writes all of its slots, so nothing left by an earlier call is read
+    HIPCHK(c, c->ops->totals(c->lay, c->buf[c->cur], c->pending ? 0 : 1, p, nb, c->stream));
+    double h[kTotalsMaxBlocks * kTotalsSlots];
+    HIPCHK(c, hipMemcpyAsync(h, p, (size_t)nb * kTotalsSlots * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return finish_totals(c, h, nb, mass, mom);
 }
 
 int lbm_get_info(lbm_ctx* c, int64_t* bytes, int32_t* pitch, int64_t* plane) {

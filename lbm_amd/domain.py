@@ -29,7 +29,7 @@ EXPORTS = ["lbm_create", "lbm_destroy", "lbm_last_error", "lbm_set_state", "lbm_
            "lbm_get_state", "lbm_get_macros", "lbm_get_tau", "lbm_set_blanking", "lbm_set_ablvisc",
            "lbm_set_inflow", "lbm_set_inflow_series", "lbm_set_turbulence", "lbm_set_external_forcing",
            "lbm_step", "lbm_synchronize", "lbm_stream", "lbm_set_timing", "lbm_timing_read",
-           "lbm_get_totals", "lbm_get_info",
+           "lbm_get_totals", "lbm_get_totals_noalloc", "lbm_get_info",
            "lbm_ensemble_create", "lbm_ensemble_destroy", "lbm_ensemble_last_error", "lbm_ensemble_step",
            "lbm_ensemble_synchronize", "lbm_ensemble_set_launch", "lbm_ensemble_get_launch",
            "lbm_ensemble_tune", "lbm_ensemble_set_timing", "lbm_ensemble_timing_read",
@@ -96,6 +96,7 @@ def load_library(path: str = LIB_PATH):
         "lbm_set_timing": ([vp, i32], i32),
         "lbm_timing_read": ([vp, C.POINTER(dbl), C.POINTER(i32), C.POINTER(dbl), C.POINTER(i32)], i32),
         "lbm_get_totals": ([vp, C.POINTER(dbl), C.POINTER(dbl)], i32),
+        "lbm_get_totals_noalloc": ([vp, C.POINTER(dbl), C.POINTER(dbl)], i32),
         "lbm_get_info": ([vp, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_int64)], i32),
         "lbm_ensemble_create": ([C.POINTER(vp), i32, C.POINTER(vp)], i32),
         "lbm_ensemble_destroy": ([vp], i32),
@@ -224,10 +225,12 @@ class Domain:
                   "lbm_get_averages")
         return (*out, n.value)
 
-    def totals(self):
+    def totals(self, noalloc=False):
+        """Mass and momentum (FP64); noalloc: through lbm_get_totals_noalloc (no device allocation)."""
         m = C.c_double()
         mom = (C.c_double * 3)()
-        self._chk(self.lib.lbm_get_totals(self.h, C.byref(m), mom), "lbm_get_totals")
+        name = "lbm_get_totals_noalloc" if noalloc else "lbm_get_totals"
+        self._chk(getattr(self.lib, name)(self.h, C.byref(m), mom), name)
         return m.value, np.array(mom[:])
 
     def info(self):
