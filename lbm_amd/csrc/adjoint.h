@@ -28,37 +28,39 @@
 
 namespace lbm {
 
-// Adjoint of one cell.  Dg(l): departure g_l - w_l of the streamed (pre-collision) population l;
-// Lg(l): adjoint of the post-collision population l; kappa: weight of the kinetic energy
-// 0.5*rho*|u|^2 of g in the objective (0: not in the objective box); coll = false: objective term
-// only.  The adjoint of g_l goes to sink(l, value); lF receives the adjoint of the cell's body
-// force.  The per-direction loops are rolled or partly unrolled (the lattice constants of a
-// run-time direction are scalar shifts of packed constants) so that the cell's state stays small;
-// Dg and Lg may be called more than once per direction.
-template <int Q, bool BGK3, bool HRR, bool VRE, typename R, class DG, class LG, class Sink>
-__host__ __device__ inline void collide_adjoint_f(const StepArgs<R>& a, DG&& Dg, int k, int oc, bool solid, LG&& Lg,
-                                                  R kappa, bool coll, Sink&& sink, R (&lF)[3]) {
+// Adjoint of one cell, the reverse of collide_cell (kernels.h) in the same pair-symmetric form.
+// D: departures g_l - w_l of the streamed (pre-collision) populations; Ly: adjoint of the
+// post-collision populations; kappa: weight of the kinetic energy 0.5*rho*|u|^2 of g in the
+// objective (0: not in the objective box); coll = false: objective term only (Ly unused).  lg
+// receives the adjoint of g, lF the adjoint of the cell's body force.
+//
+// Directions come in opposite pairs (l, l+1), c_{l+1} = -c_l.  Every per-direction factor of the
+// forward is even or odd in c, so each pair contributes  ls*(even) + ld*(odd)  with
+// ls = Ly_l + Ly_{l+1}, ld = Ly_l - Ly_{l+1} (the solid swap exchanges the pair: ld changes sign),
+// and the outputs are lg_l = even + odd, lg_{l+1} = even - odd.  The pair loops are unrolled, so
+// every lattice constant is an immediate.
+template <int Q, bool BGK3, bool HRR, bool VRE, typename R>
+__host__ __device__ __forceinline__ void collide_adjoint(const StepArgs<R>& a, const R (&D)[Q], int k, int oc, bool solid,
+                                                         const R (&Ly)[Q], R kappa, bool coll, R (&lg)[Q], R (&lF)[3]) {
     using Lt = Lattice<Q>;
-    // The loops that read the adjoint of the outputs run 4 directions per iteration in FP32, so
-    // that four loads are in flight together (reverse kernel 344 -> 325 us on 192x96x96 D3Q19);
-    // in FP64 that costs occupancy (134 -> 174 VGPRs) and is slower, so they stay rolled.
-    constexpr int kLoadUnroll = sizeof(R) == 4 ? 4 : 1;
-    // ---- forward recompute (collide_cell's quantities, in plain rather than paired form)
-    R dr = 0, jx = 0, jy = 0, jz = 0, pxx = 0, pyy = 0, pzz = 0, pxy = 0, pxz = 0, pyz = 0;
+    constexpr int NP = (Q - 1) / 2;
+    // ---- forward recompute (collide_cell's moments)
+    R dr = D[0], jx = 0, jy = 0, jz = 0, pxx = 0, pyy = 0, pzz = 0, pxy = 0, pxz = 0, pyz = 0;
 #pragma unroll
-    for (int l = 0; l < Q; ++l) {
-        const R cx = R(Lt::cx(l)), cy = R(Lt::cy(l)), cz = R(Lt::cz(l));
-        const R d = Dg(l);
-        dr += d;
-        jx += cx * d;
-        jy += cy * d;
-        jz += cz * d;
-        pxx += cx * cx * d;
-        pyy += cy * cy * d;
-        pzz += cz * cz * d;
-        pxy += cx * cy * d;
-        pxz += cx * cz * d;
-        pyz += cy * cz * d;
+    for (int p = 0; p < NP; ++p) {
+        const int l = 2 * p + 1;
+        const int cx = Lt::cx(l), cy = Lt::cy(l), cz = Lt::cz(l);
+        const R sp = D[l] + D[l + 1], dp = D[l] - D[l + 1];
+        dr += sp;
+        if (cx > 0) jx += dp; else if (cx < 0) jx -= dp;
+        if (cy > 0) jy += dp; else if (cy < 0) jy -= dp;
+        if (cz > 0) jz += dp; else if (cz < 0) jz -= dp;
+        if (cx != 0) pxx += sp;
+        if (cy != 0) pyy += sp;
+        if (cz != 0) pzz += sp;
+        if (cx * cy > 0) pxy += sp; else if (cx * cy < 0) pxy -= sp;
+        if (cx * cz > 0) pxz += sp; else if (cx * cz < 0) pxz -= sp;
+        if (cy * cz > 0) pyz += sp; else if (cy * cz < 0) pyz -= sp;
     }
     const R rho = R(1) + dr;
     const R inv = R(1) / rho;
@@ -67,8 +69,15 @@ __host__ __device__ inline void collide_adjoint_f(const StepArgs<R>& a, DG&& Dg,
     lF[0] = lF[1] = lF[2] = R(0);
     if (!coll) {
         // objective only: d(0.5 |j|^2 / rho)/dg_l = c.u - |u|^2/2
-#pragma unroll 1
-        for (int l = 0; l < Q; ++l) sink(l, kappa * (sdot<R>(Lt::cx(l), Lt::cy(l), Lt::cz(l), ux, uy, uz) - R(0.5) * usq));
+        const R e = R(-0.5) * kappa * usq;
+        lg[0] = e;
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            const int l = 2 * p + 1;
+            const R o = kappa * sdot<R>(Lt::cx(l), Lt::cy(l), Lt::cz(l), ux, uy, uz);
+            lg[l] = e + o;
+            lg[l + 1] = e - o;
+        }
         return;
     }
     const R third = R(1) / R(3);
@@ -87,7 +96,8 @@ __host__ __device__ inline void collide_adjoint_f(const StepArgs<R>& a, DG&& Dg,
         b23 = a.eps + (axy * axz + ayy * ayz + ayz * azz);
         const R bb = b11 * b22 - b12 * b12 + b11 * b33 - b13 * b13 + b22 * b33 - b23 * b23;
         t = bb / amag;
-        const R eddy = t > a.eps ? a.vconst * std::sqrt(t) : R(0);
+        using std::sqrt;
+        const R eddy = t > a.eps ? a.vconst * sqrt(t) : R(0);
         tau = R(3) * (a.kinevisc + eddy + a.ablvisc[k - 1]) + R(0.5);
         fac = R(1) - R(1) / tau;
     }
@@ -95,110 +105,150 @@ __host__ __device__ inline void collide_adjoint_f(const StepArgs<R>& a, DG&& Dg,
     const R gx = axx * ux + axy * uy + axz * uz, gy = axy * ux + ayy * uy + ayz * uz, gz = axz * ux + ayz * uy + azz * uz;
 
     // ---- reverse
-    // adjoint of the population before the solid swap of pairs (l, l+1), l odd
-    auto ly = [&](int l) { return Lg(solid && l > 0 ? ((l & 1) ? l + 1 : l - 1) : l); };
+    // pair sums / differences of the adjoint before the solid swap
+    const R sgn = solid ? R(-1) : R(1);
     R lrho = 0, lux = 0, luy = 0, luz = 0;
     R lxx = 0, lyy = 0, lzz = 0, lxy = 0, lxz = 0, lyz = 0;   // adjoint of a2
     R lfac = 0;
+    R su = 0;   // sum of the coefficients of u in the adjoint of u (added as su * u at the end)
+    const R L0 = Ly[0];
+    const R w0 = weight<Q, R>(0);
 
-    // body force: y += feq2(rho, v) - feq2(rho, u), v = u - F/rho (skipped when rho <= 0)
+    // body force: y += feq2(rho, v) - feq2(rho, u), v = u - F/rho (skipped when rho <= 0).
+    // Per pair: P2(v) - P2(u) = [-1.5 (v.v - u.u) + 4.5 (cv^2 - cu^2)] + [3 (cv - cu)];
+    // grad_u: -3 (v - u) + 9 (cv - cu) c (even); grad_v P2(v) = [-3 v + 9 cv c] + [3 c].
     if (a.Fx != nullptr && rho > R(0)) {
         const R Fx = a.Fx[oc], Fy = a.Fy[oc], Fz = a.Fz[oc];
         const R vx = ux - Fx * inv, vy = uy - Fy * inv, vz = uz - Fz * inv;
         const R vsq = vx * vx + vy * vy + vz * vz;
-        R lvx = 0, lvy = 0, lvz = 0;
-#pragma unroll kLoadUnroll
-        for (int l = 0; l < Q; ++l) {
+        const R dsq = R(-1.5) * (vsq - usq);
+        // coefficients of (v - u) and v in lu and lv; the c-terms go straight to lu / lv
+        R sdv = w0 * L0 * R(-3), sv = w0 * L0 * R(-3);
+        R lvx = 0, lvy = 0, lvz = 0, lgx = 0, lgy = 0, lgz = 0;
+        lrho += w0 * L0 * dsq;
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            const int l = 2 * p + 1;
             const int cx = Lt::cx(l), cy = Lt::cy(l), cz = Lt::cz(l);
             const R w = weight<Q, R>(l);
+            const R ls = w * (Ly[l] + Ly[l + 1]);
+            const R ld = w * sgn * (Ly[l] - Ly[l + 1]);
             const R cv = sdot<R>(cx, cy, cz, vx, vy, vz), cu = sdot<R>(cx, cy, cz, ux, uy, uz);
-            const R Pv = R(1) - R(1.5) * vsq + R(3) * cv + R(4.5) * cv * cv;
-            const R Pu = R(1) - R(1.5) * usq + R(3) * cu + R(4.5) * cu * cu;
-            const R wl = w * ly(l);
-            const R wr = wl * rho;
-            lrho += wl * (Pv - Pu);
-            // grad P2(v) - grad P2(u) = -3 (v - u) + 9 (c.v - c.u) c, and grad P2(v)
-            const R dc = R(9) * (cv - cu);
-            lux += wr * (R(-3) * (vx - ux) + dc * R(cx));
-            luy += wr * (R(-3) * (vy - uy) + dc * R(cy));
-            luz += wr * (R(-3) * (vz - uz) + dc * R(cz));
-            const R c9 = R(3) + R(9) * cv;
-            lvx += wr * (R(-3) * vx + c9 * R(cx));
-            lvy += wr * (R(-3) * vy + c9 * R(cy));
-            lvz += wr * (R(-3) * vz + c9 * R(cz));
+            lrho += ls * (dsq + R(4.5) * (cv * cv - cu * cu)) + ld * (R(3) * (cv - cu));
+            sdv += R(-3) * ls;
+            sv += R(-3) * ls;
+            const R gu = R(9) * (cv - cu) * ls;            // coefficient of c in lu
+            const R gv = R(9) * cv * ls + R(3) * ld;        // coefficient of c in lv
+            if (cx > 0) { lgx += gu; lvx += gv; } else if (cx < 0) { lgx -= gu; lvx -= gv; }
+            if (cy > 0) { lgy += gu; lvy += gv; } else if (cy < 0) { lgy -= gu; lvy -= gv; }
+            if (cz > 0) { lgz += gu; lvz += gv; } else if (cz < 0) { lgz -= gu; lvz -= gv; }
         }
-        // dv/drho = F / rho^2, dv/dF = -1/rho (du/du contributions are in lu above)
+        lvx = rho * (lvx + sv * vx);
+        lvy = rho * (lvy + sv * vy);
+        lvz = rho * (lvz + sv * vz);
+        lux += rho * (lgx + sdv * (vx - ux));
+        luy += rho * (lgy + sdv * (vy - uy));
+        luz += rho * (lgz + sdv * (vz - uz));
+        // dv/drho = F / rho^2, dv/dF = -1/rho (the du/du parts are in lu above)
         lrho += (lvx * Fx + lvy * Fy + lvz * Fz) * inv * inv;
         lF[0] = -lvx * inv;
         lF[1] = -lvy * inv;
         lF[2] = -lvz * inv;
     }
 
-    // relaxation y = feq + fac * fneq
-#pragma unroll kLoadUnroll
-    for (int l = 0; l < Q; ++l) {
+    // relaxation y = feq + fac * fneq.  Per pair (cu = c.u, cu2 = cu^2):
+    //   P = [1 - 1.5 u.u + 4.5 cu2] + [3 cu (+ 4.5 cu (cu2 - u.u))]
+    //   grad_u P = [-3 u + 9 cu c] + [(3 (+ 4.5 (3 cu2 - u.u))) c (- 9 cu u)]
+    //   fneq = 4.5 w ([cAc - T/3] + [cu (3 cAc - T) - 2 c.g])
+    // and d fneq / d a2, d fneq / d u (odd: c (3 cAc - T) - 2 a2 c).
+    const R P0 = R(1) - R(1.5) * usq;
+    const R lw0 = HRR ? L0 : (R(1) - fac) * L0;
+    lrho += lw0 * w0 * P0;
+    su += lw0 * w0 * rho * R(-3);
+    if (HRR) {
+        lfac += L0 * (R(-1.5) * w0 * T);
+        const R phi0 = fac * L0 * (R(4.5) * w0) * (-third);
+        lxx += phi0;
+        lyy += phi0;
+        lzz += phi0;
+    } else {
+        lfac += L0 * ((w0 + D[0]) - w0 * rho * P0);
+    }
+    R lcx = 0, lcy = 0, lcz = 0;   // coefficients of c in the adjoint of u
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+        const int l = 2 * p + 1;
         const int cx = Lt::cx(l), cy = Lt::cy(l), cz = Lt::cz(l);
-        const R fx = R(cx), fy = R(cy), fz = R(cz);
         const R w = weight<Q, R>(l);
+        const R ly0 = solid ? Ly[l + 1] : Ly[l], ly1 = solid ? Ly[l] : Ly[l + 1];
         const R cu = sdot<R>(cx, cy, cz, ux, uy, uz);
-        R P = R(1) - R(1.5) * usq + R(3) * cu + R(4.5) * cu * cu;
-        // grad P = -3u + 3c + 9 cu c [+ 4.5 ((3 cu^2 - u.u) c - 2 cu u)]
-        R gq = R(3) + R(9) * cu, gu = R(-3);
-        if (BGK3 && l > 0) {
-            P += R(4.5) * cu * (cu * cu - usq);
-            gq += R(4.5) * (R(3) * cu * cu - usq);
-            gu -= R(9) * cu;
+        const R cu2 = cu * cu;
+        const R Pe = P0 + R(4.5) * cu2;
+        R Po = R(3) * cu, gq = R(3), gr = R(0);
+        if (BGK3) {
+            Po += R(4.5) * cu * (cu2 - usq);
+            gq += R(4.5) * (R(3) * cu2 - usq);
+            gr = R(-9) * cu;
         }
-        const R le = ly(l);
+        const R ls0 = ly0 + ly1, ld0 = ly0 - ly1;
+        // feq part (weight w, times (1 - fac) for BGK relaxation)
+        const R ls = HRR ? ls0 : (R(1) - fac) * ls0;
+        const R ld = HRR ? ld0 : (R(1) - fac) * ld0;
+        lrho += w * (ls * Pe + ld * Po);
+        const R wr = w * rho;
+        su += wr * (R(-3) * ls + gr * ld);
+        const R cc = wr * (R(9) * cu * ls + gq * ld);
+        if (cx > 0) lcx += cc; else if (cx < 0) lcx -= cc;
+        if (cy > 0) lcy += cc; else if (cy < 0) lcy -= cc;
+        if (cz > 0) lcz += cc; else if (cz < 0) lcz -= cc;
         if (HRR) {
-            R cac = R(0);
+            R cac = sdot<R>(cx * cy, cx * cz, cy * cz, R(2) * axy, R(2) * axz, R(2) * ayz);
             if (cx != 0) cac += axx;
             if (cy != 0) cac += ayy;
             if (cz != 0) cac += azz;
-            cac += R(2) * sdot<R>(cx * cy, cx * cz, cy * cz, axy, axz, ayz);
             const R cg = sdot<R>(cx, cy, cz, gx, gy, gz);
             const R w45 = R(4.5) * w;
-            const R fneq = w45 * ((cac - T * third) + (cu * (R(3) * cac - T) - R(2) * cg));
-            const R lw = le * w;
-            const R lwr = lw * rho;
-            lrho += lw * P;
-            lux += lwr * (gu * ux + gq * fx);
-            luy += lwr * (gu * uy + gq * fy);
-            luz += lwr * (gu * uz + gq * fz);
-            lfac += le * fneq;
-            const R phi = fac * le * w45;
-            // d fneq / d a2 (six symmetric entries)
-            const R c2 = R(2) * cu;
-            lxx += phi * ((fx * fx - third) + cu * (R(3) * fx * fx - R(1)) - R(2) * fx * ux);
-            lyy += phi * ((fy * fy - third) + cu * (R(3) * fy * fy - R(1)) - R(2) * fy * uy);
-            lzz += phi * ((fz * fz - third) + cu * (R(3) * fz * fz - R(1)) - R(2) * fz * uz);
-            lxy += phi * (R(2) * fx * fy + R(3) * c2 * fx * fy - R(2) * (fx * uy + fy * ux));
-            lxz += phi * (R(2) * fx * fz + R(3) * c2 * fx * fz - R(2) * (fx * uz + fz * ux));
-            lyz += phi * (R(2) * fy * fz + R(3) * c2 * fy * fz - R(2) * (fy * uz + fz * uy));
-            // d fneq / d u = c (3 cAc - T) - 2 a2 c
             const R s3 = R(3) * cac - T;
-            lux += phi * (fx * s3 - R(2) * (axx * fx + axy * fy + axz * fz));
-            luy += phi * (fy * s3 - R(2) * (axy * fx + ayy * fy + ayz * fz));
-            luz += phi * (fz * s3 - R(2) * (axz * fx + ayz * fy + azz * fz));
+            lfac += w45 * (ls0 * (cac - T * third) + ld0 * (cu * s3 - R(2) * cg));
+            const R ps = fac * w45 * ls0, pd = fac * w45 * ld0;
+            // d fneq / d a_pp: [c_p^2 - 1/3] + [cu (3 c_p^2 - 1) - 2 c_p u_p]
+            const R pdc = pd * cu;
+            lxx += ps * R(cx * cx - third) + pdc * R(3 * cx * cx - 1) - R(2 * cx) * pd * ux;
+            lyy += ps * R(cy * cy - third) + pdc * R(3 * cy * cy - 1) - R(2 * cy) * pd * uy;
+            lzz += ps * R(cz * cz - third) + pdc * R(3 * cz * cz - 1) - R(2 * cz) * pd * uz;
+            // d fneq / d a_pq (p != q): [2 c_p c_q] + [6 cu c_p c_q - 2 (c_p u_q + c_q u_p)]
+            if (cx * cy != 0) lxy += R(cx * cy) * (R(2) * ps + R(6) * pdc) - R(2) * pd * (R(cx) * uy + R(cy) * ux);
+            else lxy -= R(2) * pd * (R(cx) * uy + R(cy) * ux);
+            if (cx * cz != 0) lxz += R(cx * cz) * (R(2) * ps + R(6) * pdc) - R(2) * pd * (R(cx) * uz + R(cz) * ux);
+            else lxz -= R(2) * pd * (R(cx) * uz + R(cz) * ux);
+            if (cy * cz != 0) lyz += R(cy * cz) * (R(2) * ps + R(6) * pdc) - R(2) * pd * (R(cy) * uz + R(cz) * uy);
+            else lyz -= R(2) * pd * (R(cy) * uz + R(cz) * uy);
+            // d fneq / d u = c (3 cAc - T) - 2 a2 c  (odd)
+            const R pds = pd * s3;
+            const R ax = sdot<R>(cx, cy, cz, axx, axy, axz), ay = sdot<R>(cx, cy, cz, axy, ayy, ayz),
+                    az = sdot<R>(cx, cy, cz, axz, ayz, azz);
+            if (cx > 0) lcx += pds; else if (cx < 0) lcx -= pds;
+            if (cy > 0) lcy += pds; else if (cy < 0) lcy -= pds;
+            if (cz > 0) lcz += pds; else if (cz < 0) lcz -= pds;
+            lux -= R(2) * pd * ax;
+            luy -= R(2) * pd * ay;
+            luz -= R(2) * pd * az;
         } else {
-            // y = feq + fac (g - feq)
-            const R feq = w * rho * P;
-            const R lfe = (R(1) - fac) * le;
-            const R lw = lfe * w;
-            const R lwr = lw * rho;
-            lrho += lw * P;
-            lux += lwr * (gu * ux + gq * fx);
-            luy += lwr * (gu * uy + gq * fy);
-            luz += lwr * (gu * uz + gq * fz);
-            lfac += le * ((w + Dg(l)) - feq);
+            // y = feq + fac (g - feq): d/dfac = g - feq
+            const R wrP = wr;
+            lfac += ls0 * (w - wrP * Pe) - ld0 * (wrP * Po) + (ly0 * D[l] + ly1 * D[l + 1]);
         }
     }
+    lux += su * ux + lcx;
+    luy += su * uy + lcy;
+    luz += su * uz + lcz;
 
     if (VRE) {
         // fac = 1 - 1/tau, tau = 3 (nu + nu_t + abl) + 1/2, nu_t = c sqrt(t) (t > eps)
+        using std::sqrt;
         const R ltau = lfac / (tau * tau);
         const R lnut = R(3) * ltau;
-        const R lt = t > a.eps ? lnut * a.vconst / (R(2) * std::sqrt(t)) : R(0);
+        const R lt = t > a.eps ? lnut * a.vconst / (R(2) * sqrt(t)) : R(0);
         const R lbb = lt / amag;
         const R lam = amag_raw > a.tiny ? -lt * t / amag : R(0);
         const R l11 = lbb * (b22 + b33), l22 = lbb * (b11 + b33), l33 = lbb * (b11 + b22);
@@ -218,39 +268,40 @@ __host__ __device__ inline void collide_adjoint_f(const StepArgs<R>& a, DG&& Dg,
     luy -= rho * (R(2) * uy * lyy + ux * lxy + uz * lyz);
     luz -= rho * (R(2) * uz * lzz + ux * lxz + uy * lyz);
     // u = j / rho
-    const R ljx = lux * inv, ljy = luy * inv, ljz = luz * inv;
+    const R ljx = lux * inv + kappa * ux, ljy = luy * inv + kappa * uy, ljz = luz * inv + kappa * uz;
     lrho -= (lux * ux + luy * uy + luz * uz) * inv;
-    const R kx = kappa * ux, ky = kappa * uy, kz = kappa * uz, k0 = lrho - R(0.5) * kappa * usq;
-#pragma unroll 1
-    for (int l = 0; l < Q; ++l) {
+    const R k0 = lrho - R(0.5) * kappa * usq;
+    lg[0] = HRR ? k0 : k0 + fac * L0;
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+        const int l = 2 * p + 1;
         const int cx = Lt::cx(l), cy = Lt::cy(l), cz = Lt::cz(l);
-        R v = k0 + sdot<R>(cx, cy, cz, ljx + kx, ljy + ky, ljz + kz);
-        if (cx != 0) v += lxx;
-        if (cy != 0) v += lyy;
-        if (cz != 0) v += lzz;
-        v += sdot<R>(cx * cy, cx * cz, cy * cz, lxy, lxz, lyz);
-        if (!HRR) v += fac * ly(l);
-        sink(l, v);
+        R e = k0;
+        if (cx != 0) e += lxx;
+        if (cy != 0) e += lyy;
+        if (cz != 0) e += lzz;
+        e += sdot<R>(cx * cy, cx * cz, cy * cz, lxy, lxz, lyz);
+        const R o = sdot<R>(cx, cy, cz, ljx, ljy, ljz);
+        R v0 = e + o, v1 = e - o;
+        if (!HRR) {
+            v0 += fac * (solid ? Ly[l + 1] : Ly[l]);
+            v1 += fac * (solid ? Ly[l] : Ly[l + 1]);
+        }
+        lg[l] = v0;
+        lg[l + 1] = v1;
     }
-}
-
-// Array form (host checks).
-template <int Q, bool BGK3, bool HRR, bool VRE, typename R>
-__host__ __device__ inline void collide_adjoint(const StepArgs<R>& a, const R (&D)[Q], int k, int oc, bool solid,
-                                                const R (&ly_in)[Q], R kappa, bool coll, R (&lg)[Q], R (&lF)[3]) {
-    collide_adjoint_f<Q, BGK3, HRR, VRE, R>(
-        a, [&](int l) { return D[l]; }, k, oc, solid, [&](int l) { return ly_in[l]; }, kappa, coll,
-        [&](int l, R v) { lg[l] = v; }, lF);
 }
 
 // ------------------------------------------------------------------------------------------
 // Reverse of the stream + collide kernel for step n.  src: forward buffer X(n-1) (the step's
-// input); lam_y: adjoint buffer of X(n) (read, interior); lam_x: adjoint buffer of X(n-1)
-// (interior slots assigned, ghost slots accumulated).  box: objective cells (inclusive, 1-based);
-// kappa: objective weight of the moments of X(n-1) streamed, i.e. this step's gathered input.
-// gF: FP64 accumulators of the body-force adjoint (3 cell planes) or nullptr.
+// input); lam_y: adjoint buffer of X(n) (read, interior; unused when coll == 0); lam_x: adjoint
+// buffer of X(n-1) (interior slots assigned, ghost slots accumulated).  box: objective cells
+// (inclusive, 1-based); kappa: objective weight of the moments of X(n-1) streamed, i.e. this
+// step's gathered input.  gF: FP64 accumulators of the body-force adjoint (3 cell planes) or
+// nullptr.  The gather offsets are formed as in step_cell (uniform strides, wrap distances of the
+// periodic axes); SMALL: the buffer is below 4 GiB (32-bit byte offsets from one base).
 // ------------------------------------------------------------------------------------------
-template <int Q, class C, bool BGK3, bool HRR, bool VRE>
+template <int Q, class C, bool BGK3, bool HRR, bool VRE, bool SMALL>
 __global__ void __launch_bounds__(256) adjoint_step_kernel(const StepArgs<typename C::real> a, const typename C::raw* lam_y,
                                                            typename C::raw* lam_x, AdjBox box, typename C::real kappa,
                                                            int coll, double* gF) {
@@ -267,55 +318,85 @@ __global__ void __launch_bounds__(256) adjoint_step_kernel(const StepArgs<typena
     const int x[3] = {i, j, k};
     const int n[3] = {L.nx, L.ny, L.nz};
     const int per[3] = {L.per_i, L.per_j, L.per_k};
-    const raw* __restrict__ src = static_cast<const raw*>(a.src);
-    // source cell of direction l (periodic axes folded)
-    auto source = [&](int l) {
-        return L.cell(Layout::wrap(i - Lt::cx(l), L.nx, L.per_i), Layout::wrap(j - Lt::cy(l), L.ny, L.per_j),
-                      Layout::wrap(k - Lt::cz(l), L.nz, L.per_k));
-    };
-    auto Dg = [&](int l) { return C::template dep<Q>(src[(size_t)l * L.plane + source(l)], l); };
+    // element offsets of the sources i - cx, j - cy, k - cz (index c + 1)
     const int oc = L.cell(i, j, k);
-    auto Lg = [&](int l) { return C::template decode<Q>(lam_y[(size_t)l * L.plane + oc], l); };
+    int ox[3], oy[3], oz[3];
+    auto offsets = [&](int xx, int nn, int pp, int st, int (&o)[3]) {
+        const int wrapd = pp ? nn * st : 0;
+        const int lo = (pp && xx == nn) ? -wrapd : 0;
+        const int hi = (pp && xx == 1) ? wrapd : 0;
+        o[0] = st + lo;
+        o[1] = 0;
+        o[2] = -(st - hi);
+    };
+    offsets(i, L.nx, L.per_i, L.si, ox);
+    offsets(j, L.ny, L.per_j, L.sj, oy);
+    offsets(k, L.nz, L.per_k, L.sk, oz);
+    auto src_off = [&](int l) { return oc + oz[Lt::cz(l) + 1] + oy[Lt::cy(l) + 1] + ox[Lt::cx(l) + 1]; };
+    // population l of element offset o
+    const size_t plane = (size_t)L.plane;
+    auto at = [&](auto* base, int l, int o) {
+        if constexpr (SMALL) {
+            using B = std::conditional_t<std::is_const<std::remove_pointer_t<decltype(base)>>::value, const char*, char*>;
+            return reinterpret_cast<decltype(base)>(reinterpret_cast<B>(base) +
+                                                    ((uint32_t)l * (uint32_t)(plane * sizeof(raw)) + (uint32_t)o * (uint32_t)sizeof(raw)));
+        } else {
+            return base + ((size_t)l * plane + (size_t)o);
+        }
+    };
+    const raw* __restrict__ src = static_cast<const raw*>(a.src);
+    R D[Q], Ly[Q];
+#pragma unroll
+    for (int l = 0; l < Q; ++l) D[l] = C::template dep<Q>(*at(src, l, src_off(l)), l);
+    if (coll) {
+#pragma unroll
+        for (int l = 0; l < Q; ++l) Ly[l] = C::template decode<Q>(*at(lam_y, l, oc), l);
+    } else {
+#pragma unroll
+        for (int l = 0; l < Q; ++l) Ly[l] = R(0);
+    }
     bool solid = false;
     if (a.solid != nullptr) {
         const uint32_t word = a.solid[((cc - 1) * L.n(L.ax1) + (cb - 1)) * a.solid_wpr + ((ca - 1) >> 5)];
         solid = (word >> ((ca - 1) & 31)) & 1u;
     }
     const bool inbox = i >= box.lo[0] && i <= box.hi[0] && j >= box.lo[1] && j <= box.hi[1] && k >= box.lo[2] && k <= box.hi[2];
+    const int ocx = L.cidx(i, j, k);   // per-cell arrays (force, its gradient)
+    R lg[Q], lF[3];
+    collide_adjoint<Q, BGK3, HRR, VRE, R>(a, D, k, ocx, solid, Ly, inbox ? kappa : R(0), coll != 0, lg, lF);
     // a cell next to a non-periodic face: some sources are ghosts, some own slots are unhit
     bool edge = false;
 #pragma unroll
     for (int d = 0; d < 3; ++d)
         if (!per[d] && (x[d] == 1 || x[d] == n[d])) edge = true;
     raw* __restrict__ lx = lam_x;
-    R lF[3];
-    auto sink = [&](int l, R v) {
-        raw* p = lx + (size_t)l * L.plane + source(l);
-        if (!edge) {
-            *p = C::template encode<Q>(v, l);
-            return;
-        }
-        // the source slot: interior -> assign; ghost -> accumulate
-        bool ghost = false, unhit = false;
+    if (!edge) {
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            if (per[d]) continue;
-            const int cd = Lt::c(l, d);
-            const int s = x[d] - cd, t = x[d] + cd;
-            if (s < 1 || s > n[d]) ghost = true;
-            if (t < 1 || t > n[d]) unhit = true;
+        for (int l = 0; l < Q; ++l) *at(lx, l, src_off(l)) = C::template encode<Q>(lg[l], l);
+    } else {
+#pragma unroll
+        for (int l = 0; l < Q; ++l) {
+            // the source slot: interior -> assign; ghost -> accumulate
+            bool ghost = false, unhit = false;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                if (per[d]) continue;
+                const int cd = Lt::c(l, d);
+                const int s = x[d] - cd, t = x[d] + cd;
+                if (s < 1 || s > n[d]) ghost = true;
+                if (t < 1 || t > n[d]) unhit = true;
+            }
+            raw* p = at(lx, l, src_off(l));
+            if (ghost) *p = C::template encode<Q>(C::template decode<Q>(*p, l) + lg[l], l);
+            else *p = C::template encode<Q>(lg[l], l);
+            // this cell's own slot of direction l is pulled by no interior cell: clear it
+            if (unhit) *at(lx, l, oc) = C::template encode<Q>(R(0), l);
         }
-        if (ghost) *p = C::template encode<Q>(C::template decode<Q>(*p, l) + v, l);
-        else *p = C::template encode<Q>(v, l);
-        // this cell's own slot of direction l is pulled by no interior cell: clear it
-        if (unhit) lx[(size_t)l * L.plane + oc] = C::template encode<Q>(R(0), l);
-    };
-    const int ox = L.cidx(i, j, k);   // per-cell arrays (force, its gradient)
-    collide_adjoint_f<Q, BGK3, HRR, VRE, R>(a, Dg, k, ox, solid, Lg, inbox ? kappa : R(0), coll != 0, sink, lF);
+    }
     if (gF != nullptr && a.Fx != nullptr) {
-        gF[ox] += (double)lF[0];
-        gF[(size_t)L.cplane + ox] += (double)lF[1];
-        gF[2 * (size_t)L.cplane + ox] += (double)lF[2];
+        gF[ocx] += (double)lF[0];
+        gF[(size_t)L.cplane + ocx] += (double)lF[1];
+        gF[2 * (size_t)L.cplane + ocx] += (double)lF[2];
     }
 }
 

@@ -355,9 +355,15 @@ struct Launch {
         if constexpr (kAdj) {
             const dim3 block(64, 4, 1);
             const dim3 grid((L.n(L.ax0) + 63) / 64, (L.n(L.ax1) + 3) / 4, L.n(L.ax2));
-            hipLaunchKernelGGL((adjoint_step_kernel<Q, C, B3, H, V>), grid, block, 0, s, a,
-                               static_cast<const typename C::raw*>(ly), static_cast<typename C::raw*>(lx), box, kappa,
-                               coll, gF);
+            const auto* lyp = static_cast<const typename C::raw*>(ly);
+            auto* lxp = static_cast<typename C::raw*>(lx);
+            // 32-bit byte offsets from one base when a buffer is below 4 GiB
+            if ((double)L.total * sizeof(typename C::raw) < 4294967296.0)
+                hipLaunchKernelGGL((adjoint_step_kernel<Q, C, B3, H, V, true>), grid, block, 0, s, a, lyp, lxp, box, kappa,
+                                   coll, gF);
+            else
+                hipLaunchKernelGGL((adjoint_step_kernel<Q, C, B3, H, V, false>), grid, block, 0, s, a, lyp, lxp, box, kappa,
+                                   coll, gF);
         }
     }
 
