@@ -128,6 +128,18 @@ int lbm_generate_turbulence(lbm_ctx* ctx, int nrturb, double ampl, uint64_t seed
  * (src/m_forcings_apply.F90:3); NULL switches it off. */
 int lbm_set_external_forcing(lbm_ctx* ctx, const void* force, int realkind);
 
+/* Buoyancy: the external-force producer of src/main.F90:293 for iablvisc = 2, replacing
+ * buoyancy_forcing (src/m_buoyancy_forcing.F90:3-46, kernel src/m_buoyancy_forcing_kernel.F90:37):
+ * every following step adds  F_z -= scaling * (pottemp - theta0) / theta0  to the forcing of each
+ * interior cell (on top of the caller's external_forcing or the turbines' deposit) before the
+ * forcings_apply increment.  pottemp(0:nx+1,0:ny+1,0:nz+1) is the potential temperature of the
+ * step; theta0 the reference's pottemp0 (300 K, src/m_abl_initialize.F90:2); scaling
+ * = g * p2l%time^2 / p2l%length with g = 9.81 (src/m_buoyancy_forcing.F90:33-34).  Call it again
+ * with the new field whenever the caller's transport of pottemp (heatflux + advection,
+ * src/main.F90:332-339, not in this library) has advanced it; NULL switches the buoyancy off.
+ * Not in ensembles or adjoint runs. */
+int lbm_set_buoyancy(lbm_ctx* ctx, const void* pottemp, double theta0, double scaling, int realkind);
+
 /* Actuator-line turbines: the external-force producer of src/main.F90:291-292, replacing
  * turbine_forcing (src/m_turbine_forcing.F90:8-140) with turbine_distribute_points
  * (src/m_turbine_distribute_points.F90:6-64), turbine_point_forces (src/m_turbine_point_forces.F90:20-82,
@@ -302,9 +314,11 @@ int lbm_ensemble_timing_read(lbm_ens* ens, double* ms, int* nsteps);
  *   grad_shear  dJ/d uvel_shear(k), k = 1..nz (inflow/outflow i only, else zeros); may be NULL.
  * The context then holds the state after the nsteps steps, as after lbm_step.  Needs FP32 or
  * FP64 storage, a stepped state (not the first step after set_state / init_equilibrium), no
- * inflow turbulence, and at most one closed axis.  Device memory: (nsteps/every + 4) state
- * pairs + every single states.  Adjoint arithmetic in the storage precision, gradient
- * accumulation in FP64.
+ * inflow turbulence, no turbines or buoyancy, at most one closed axis and at most 2^24 cells in a
+ * plane of the two fastest storage axes.  Device memory: (nsteps/every + 4) state pairs + every
+ * single states, kept by the context for the next run (a gradient loop allocates once) and
+ * released by lbm_destroy.  Adjoint arithmetic in the storage precision, gradient accumulation
+ * in FP64.
  */
 int lbm_objective_run(lbm_ctx* ctx, int it0, int nsteps, const int32_t* box, double* J);
 

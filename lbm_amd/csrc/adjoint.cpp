@@ -53,7 +53,7 @@ int lbm_objective_run(lbm_ctx* c, int it0, int nsteps, const int32_t* box, doubl
     if (int rc_ = check_ctx(c)) return rc_;
     if (!J) return fail(c, LBM_EINVAL, "null argument");
     if (nsteps < 1) return fail(c, LBM_EINVAL, "nsteps must be >= 1");
-    if (c->turb_on) return fail(c, LBM_EINVAL, "objective runs take no turbines");
+    if (c->turb_on || c->buoy_on) return fail(c, LBM_EINVAL, "objective runs take no turbines or buoyancy");
     AdjBox b;
     if (int rc = check_box(c, box, &b)) return rc;
     DevBuf acc;
@@ -137,31 +137,40 @@ int lbm_adjoint_run(lbm_ctx* c, int it0, int nsteps, int every, const int32_t* b
     if (!c->ops->adj_step) return fail(c, LBM_EINVAL, "the adjoint needs FP32 or FP64 storage");
     if (c->has_turb) return fail(c, LBM_EINVAL, "the adjoint does not differentiate the inflow-turbulence forcing: switch it off");
     if (c->turb_on) return fail(c, LBM_EINVAL, "the adjoint does not differentiate the turbines: remove them");
+    if (c->buoy_on) return fail(c, LBM_EINVAL, "the adjoint does not differentiate the buoyancy forcing: switch it off");
     if (c->avg_on) return fail(c, LBM_ESTATE, "switch time averaging off before an adjoint run");
     const int closed = (d.ibnd > 10) + (d.jbnd > 10) + (d.kbnd > 10);
     if (c->d.jbnd == LBM_BND_HALO) return fail(c, LBM_EINVAL, "the adjoint takes untiled domains (no halo planes)");
     if (closed > 1 || (closed == 1 && c->fold_dir < 0))
         return fail(c, LBM_EINVAL, "the adjoint supports at most one closed axis (its face pass runs inside the step kernel)");
     if (nsteps < 1 || every < 1) return fail(c, LBM_EINVAL, "nsteps and every must be >= 1");
+    // the reverse kernel splits its flat index over a plane of the two fastest storage axes in FP32
+    if ((long long)c->lay.n(c->lay.ax0) * c->lay.n(c->lay.ax1) >= (1LL << 24))
+        return fail(c, LBM_EINVAL, "the adjoint takes at most 2^24 cells in a plane of the two fastest storage axes");
     AdjBox b;
     if (int rc = check_box(c, box, &b)) return rc;
 
     const size_t bb = c->buf_bytes;
     const int nck = (nsteps + every - 1) / every;
     const size_t plane = (size_t)c->lay.cplane;   // per-cell arrays (force gradient)
-    DevBuf ck, snaps, lam, fin, gF, gS, acc;
-    auto cleanup = [&]() {
+    // the workspace stays with the context (a gradient loop calls this repeatedly); a failed
+    // allocation frees it all
+    DevBuf &ck = c->adj_ck, &snaps = c->adj_snaps, &lam = c->adj_lam, &fin = c->adj_fin, &gF = c->adj_gF,
+           &gS = c->adj_gS, &acc = c->adj_acc;
+    auto release = [&]() {
         for (DevBuf* x : {&ck, &snaps, &lam, &fin, &gF, &gS, &acc}) dev_free(*x);
     };
     int rc = LBM_OK;
+    const bool want_gF = grad_force && c->has_force;
     if ((rc = dev_alloc(c, ck, (size_t)nck * 2 * bb)) || (rc = dev_alloc(c, snaps, (size_t)every * bb)) ||
         (rc = dev_alloc(c, lam, 2 * bb)) || (rc = dev_alloc(c, fin, 2 * bb)) ||
         (rc = dev_alloc(c, acc, 2 * (size_t)objective_blocks(b) * sizeof(double))) ||
         (rc = dev_alloc(c, gS, (size_t)d.ny * d.nz * sizeof(double))) ||
-        (grad_force && c->has_force && (rc = dev_alloc(c, gF, 3 * plane * sizeof(double))))) {
-        cleanup();
+        (want_gF && (rc = dev_alloc(c, gF, 3 * plane * sizeof(double))))) {
+        release();
         return rc;
     }
+    double* gFp = want_gF ? static_cast<double*>(gF.p) : nullptr;
     char* ckp = static_cast<char*>(ck.p);
     char* snp = static_cast<char*>(snaps.p);
     char* lmp = static_cast<char*>(lam.p);
@@ -188,7 +197,7 @@ int lbm_adjoint_run(lbm_ctx* c, int it0, int nsteps, int every, const int32_t* b
         // ---- reverse sweep
         HIPCHK(c, hipMemsetAsync(lam.p, 0, 2 * bb, c->stream));
         HIPCHK(c, hipMemsetAsync(gS.p, 0, (size_t)d.ny * d.nz * sizeof(double), c->stream));
-        if (gF.p) HIPCHK(c, hipMemsetAsync(gF.p, 0, 3 * plane * sizeof(double), c->stream));
+        if (gFp) HIPCHK(c, hipMemsetAsync(gFp, 0, 3 * plane * sizeof(double), c->stream));
         const double kappa = 1.0 / ((double)nsteps * box_cells(b));
         StepHost h = step_host(c, false);
         h.stream = 1;
@@ -230,7 +239,7 @@ int lbm_adjoint_run(lbm_ctx* c, int it0, int nsteps, int every, const int32_t* b
                 }
                 if (c->fold_dir >= 0) HIPCHK(c, c->ops->adj_face(c->lay, ly, lx, c->fold_dir, opt_lo, opt_hi, c->stream));
                 h.src = snp + (size_t)(n - 1 - n0) * bb;
-                HIPCHK(c, c->ops->adj_step(c->lay, h, ly, lx, b, n - 1 >= 1 ? kappa : 0.0, 1, static_cast<double*>(gF.p),
+                HIPCHK(c, c->ops->adj_step(c->lay, h, ly, lx, b, n - 1 >= 1 ? kappa : 0.0, 1, gFp,
                                            c->stream));
             }
         }
@@ -245,9 +254,9 @@ int lbm_adjoint_run(lbm_ctx* c, int it0, int nsteps, int every, const int32_t* b
         std::vector<double> hs((size_t)d.ny * d.nz), hf, ha(2 * (size_t)nblk);
         HIPCHK(c, hipMemcpyAsync(ha.data(), acc.p, ha.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipMemcpyAsync(hs.data(), gS.p, hs.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        if (gF.p) {
+        if (gFp) {
             hf.resize(3 * plane);
-            HIPCHK(c, hipMemcpyAsync(hf.data(), gF.p, hf.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipMemcpyAsync(hf.data(), gFp, hf.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         }
         HIPCHK(c, hipStreamSynchronize(c->stream));
         for (int q = 0; q < nblk; ++q) s += ha[2 * q];
@@ -261,7 +270,7 @@ int lbm_adjoint_run(lbm_ctx* c, int it0, int nsteps, int every, const int32_t* b
         if (grad_force) {
             const size_t nxp = d.nx + 2, nyp = d.ny + 2, nzp = d.nz + 2;
             std::fill(grad_force, grad_force + 3 * nxp * nyp * nzp, 0.0);
-            if (gF.p)
+            if (gFp)
                 for (int k = 1; k <= d.nz; ++k)
                     for (int j = 1; j <= d.ny; ++j)
                         for (int i = 1; i <= d.nx; ++i) {
@@ -274,9 +283,7 @@ int lbm_adjoint_run(lbm_ctx* c, int it0, int nsteps, int every, const int32_t* b
         }
         return LBM_OK;
     };
-    rc = run();
-    cleanup();
-    return rc;
+    return run();
 }
 
 }  // extern "C"

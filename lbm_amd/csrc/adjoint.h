@@ -63,7 +63,7 @@ __host__ __device__ __forceinline__ void collide_adjoint(const StepArgs<R>& a, c
         if (cy * cz > 0) pyz += sp; else if (cy * cz < 0) pyz -= sp;
     }
     const R rho = R(1) + dr;
-    const R inv = R(1) / rho;
+    const R inv = recip(rho);
     const R ux = jx * inv, uy = jy * inv, uz = jz * inv;
     const R usq = ux * ux + uy * uy + uz * uz;
     lF[0] = lF[1] = lF[2] = R(0);
@@ -95,11 +95,10 @@ __host__ __device__ __forceinline__ void collide_adjoint(const StepArgs<R>& a, c
         b13 = a.eps + (axx * axz + axy * ayz + axz * azz);
         b23 = a.eps + (axy * axz + ayy * ayz + ayz * azz);
         const R bb = b11 * b22 - b12 * b12 + b11 * b33 - b13 * b13 + b22 * b33 - b23 * b23;
-        t = bb / amag;
-        using std::sqrt;
-        const R eddy = t > a.eps ? a.vconst * sqrt(t) : R(0);
+        t = bb * recip(amag);
+        const R eddy = t > a.eps ? a.vconst * sqrt_r(t) : R(0);
         tau = R(3) * (a.kinevisc + eddy + a.ablvisc[k - 1]) + R(0.5);
-        fac = R(1) - R(1) / tau;
+        fac = R(1) - recip(tau);
     }
     const R T = axx + ayy + azz;
     const R gx = axx * ux + axy * uy + axz * uz, gy = axy * ux + ayy * uy + ayz * uz, gz = axz * ux + ayz * uy + azz * uz;
@@ -175,6 +174,8 @@ __host__ __device__ __forceinline__ void collide_adjoint(const StepArgs<R>& a, c
         lfac += L0 * ((w0 + D[0]) - w0 * rho * P0);
     }
     R lcx = 0, lcy = 0, lcz = 0;   // coefficients of c in the adjoint of u
+    // pair sums of the regularized part (see the loop)
+    R sr = 0, rx = 0, ry = 0, rz = 0, oxy = 0, oxz = 0, oyz = 0, dx = 0, dy = 0, dz = 0;
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
         const int l = 2 * p + 1;
@@ -210,29 +211,28 @@ __host__ __device__ __forceinline__ void collide_adjoint(const StepArgs<R>& a, c
             const R w45 = R(4.5) * w;
             const R s3 = R(3) * cac - T;
             lfac += w45 * (ls0 * (cac - T * third) + ld0 * (cu * s3 - R(2) * cg));
-            const R ps = fac * w45 * ls0, pd = fac * w45 * ld0;
-            // d fneq / d a_pp: [c_p^2 - 1/3] + [cu (3 c_p^2 - 1) - 2 c_p u_p]
-            const R pdc = pd * cu;
-            lxx += ps * R(cx * cx - third) + pdc * R(3 * cx * cx - 1) - R(2 * cx) * pd * ux;
-            lyy += ps * R(cy * cy - third) + pdc * R(3 * cy * cy - 1) - R(2 * cy) * pd * uy;
-            lzz += ps * R(cz * cz - third) + pdc * R(3 * cz * cz - 1) - R(2 * cz) * pd * uz;
-            // d fneq / d a_pq (p != q): [2 c_p c_q] + [6 cu c_p c_q - 2 (c_p u_q + c_q u_p)]
-            if (cx * cy != 0) lxy += R(cx * cy) * (R(2) * ps + R(6) * pdc) - R(2) * pd * (R(cx) * uy + R(cy) * ux);
-            else lxy -= R(2) * pd * (R(cx) * uy + R(cy) * ux);
-            if (cx * cz != 0) lxz += R(cx * cz) * (R(2) * ps + R(6) * pdc) - R(2) * pd * (R(cx) * uz + R(cz) * ux);
-            else lxz -= R(2) * pd * (R(cx) * uz + R(cz) * ux);
-            if (cy * cz != 0) lyz += R(cy * cz) * (R(2) * ps + R(6) * pdc) - R(2) * pd * (R(cy) * uz + R(cz) * uy);
-            else lyz -= R(2) * pd * (R(cy) * uz + R(cz) * uy);
-            // d fneq / d u = c (3 cAc - T) - 2 a2 c  (odd)
+            const R kf = fac * w45;
+            const R ps = kf * ls0, pd = kf * ld0;
+            // d fneq / d a_pp = [c_p^2 - 1/3] + [cu (3 c_p^2 - 1) - 2 c_p u_p],
+            // d fneq / d a_pq = [2 c_p c_q] + [6 cu c_p c_q - 2 (c_p u_q + c_q u_p)]  (p != q),
+            // d fneq / d u = c (3 cAc - T) - 2 a2 c: with r = ps + 3 pd cu they are sums over the pairs
+            // of r (all pairs, and the pairs with c_p != 0), of the signed c_p c_q r and c_p pd, and of
+            // the signed c_p pd s3, formed into the adjoints of a2 and u after the loop
+            const R r = ps + R(3) * (pd * cu);
+            sr += r;
+            if (cx != 0) rx += r;
+            if (cy != 0) ry += r;
+            if (cz != 0) rz += r;
+            if (cx * cy > 0) oxy += r; else if (cx * cy < 0) oxy -= r;
+            if (cx * cz > 0) oxz += r; else if (cx * cz < 0) oxz -= r;
+            if (cy * cz > 0) oyz += r; else if (cy * cz < 0) oyz -= r;
+            if (cx > 0) dx += pd; else if (cx < 0) dx -= pd;
+            if (cy > 0) dy += pd; else if (cy < 0) dy -= pd;
+            if (cz > 0) dz += pd; else if (cz < 0) dz -= pd;
             const R pds = pd * s3;
-            const R ax = sdot<R>(cx, cy, cz, axx, axy, axz), ay = sdot<R>(cx, cy, cz, axy, ayy, ayz),
-                    az = sdot<R>(cx, cy, cz, axz, ayz, azz);
             if (cx > 0) lcx += pds; else if (cx < 0) lcx -= pds;
             if (cy > 0) lcy += pds; else if (cy < 0) lcy -= pds;
             if (cz > 0) lcz += pds; else if (cz < 0) lcz -= pds;
-            lux -= R(2) * pd * ax;
-            luy -= R(2) * pd * ay;
-            luz -= R(2) * pd * az;
         } else {
             // y = feq + fac (g - feq): d/dfac = g - feq
             const R wrP = wr;
@@ -242,15 +242,27 @@ __host__ __device__ __forceinline__ void collide_adjoint(const StepArgs<R>& a, c
     lux += su * ux + lcx;
     luy += su * uy + lcy;
     luz += su * uz + lcz;
+    if (HRR) {
+        const R s3r = sr * third;
+        lxx += rx - s3r - R(2) * ux * dx;
+        lyy += ry - s3r - R(2) * uy * dy;
+        lzz += rz - s3r - R(2) * uz * dz;
+        lxy += R(2) * (oxy - (uy * dx + ux * dy));
+        lxz += R(2) * (oxz - (uz * dx + ux * dz));
+        lyz += R(2) * (oyz - (uz * dy + uy * dz));
+        lux -= R(2) * (axx * dx + axy * dy + axz * dz);
+        luy -= R(2) * (axy * dx + ayy * dy + ayz * dz);
+        luz -= R(2) * (axz * dx + ayz * dy + azz * dz);
+    }
 
     if (VRE) {
         // fac = 1 - 1/tau, tau = 3 (nu + nu_t + abl) + 1/2, nu_t = c sqrt(t) (t > eps)
-        using std::sqrt;
-        const R ltau = lfac / (tau * tau);
+        const R itau = recip(tau), iam = recip(amag);
+        const R ltau = lfac * (itau * itau);
         const R lnut = R(3) * ltau;
-        const R lt = t > a.eps ? lnut * a.vconst / (R(2) * sqrt(t)) : R(0);
-        const R lbb = lt / amag;
-        const R lam = amag_raw > a.tiny ? -lt * t / amag : R(0);
+        const R lt = t > a.eps ? lnut * a.vconst * R(0.5) * recip(sqrt_r(t)) : R(0);
+        const R lbb = lt * iam;
+        const R lam = amag_raw > a.tiny ? -lt * t * iam : R(0);
         const R l11 = lbb * (b22 + b33), l22 = lbb * (b11 + b33), l33 = lbb * (b11 + b22);
         const R l12 = R(-2) * b12 * lbb, l13 = R(-2) * b13 * lbb, l23 = R(-2) * b23 * lbb;
         lxx += R(2) * axx * l11 + axy * l12 + axz * l13 + R(2) * axx * lam;
@@ -309,10 +321,16 @@ __global__ void __launch_bounds__(256) adjoint_step_kernel(const StepArgs<typena
     using raw = typename C::raw;
     using Lt = Lattice<Q>;
     const Layout& L = a.lay;
-    const int ca = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int cb = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const int cc = blockIdx.z + 1;
-    if (ca > L.n(L.ax0) || cb > L.n(L.ax1) || cc > L.n(L.ax2)) return;
+    // flat over the cells of one plane of ax0 x ax1 (no idle lanes when n0 is not a multiple of 64),
+    // blockIdx.y along ax2; t / n0 through the FP32 reciprocal with one correction step (exact for
+    // t < 2^24)
+    const int n0 = L.n(L.ax0);
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n0 * L.n(L.ax1)) return;
+    int q = (int)((float)t * __builtin_amdgcn_rcpf((float)n0));
+    int rem = t - q * n0;
+    if (rem < 0) { --q; rem += n0; } else if (rem >= n0) { ++q; rem -= n0; }
+    const int ca = rem + 1, cb = q + 1, cc = blockIdx.y + 1;
     int i, j, k;
     L.ijk(ca, cb, cc, i, j, k);
     const int x[3] = {i, j, k};

@@ -291,6 +291,9 @@ StepHost step_host(const lbm_ctx* c, bool fold) {
         h.flo[a] = c->turb_on ? c->th.lo[a] : 1;
         h.fhi[a] = c->turb_on ? c->th.hi[a] : n[a];
     }
+    h.theta = c->buoy_on ? c->theta.p : nullptr;
+    h.theta0 = c->btheta0;
+    h.bscale = c->bscale;
     h.tau_out = (d.store_tau && d.ivreman) ? c->tau.p : nullptr;
     h.bgk3 = d.ibgk == 3;
     h.hrr = d.ihrr == 1;
@@ -538,7 +541,8 @@ int lbm_destroy(lbm_ctx* c) {
     DevBuf* all[] = {&c->ablvisc, &c->shear, &c->solid, &c->turb[0], &c->turb[1], &c->turb[2], &c->force[0],
                      &c->force[1], &c->force[2], &c->tau, &c->ext[0], &c->ext[1], &c->ext[2], &c->ext[3],
                      &c->stage, &c->partial, &c->snap, &c->t_rot, &c->t_theta, &c->t_blade, &c->t_nfoil,
-                     &c->t_nang, &c->t_foil, &c->t_pts, &c->t_box};
+                     &c->t_nang, &c->t_foil, &c->t_pts, &c->t_box, &c->theta, &c->adj_ck, &c->adj_snaps,
+                     &c->adj_lam, &c->adj_fin, &c->adj_gF, &c->adj_gS, &c->adj_acc};
     for (DevBuf* b : all) dev_free(*b);
     for (hipEvent_t ev : c->ev_step) hipEventDestroy(ev);
     for (hipEvent_t ev : c->ev_bc) hipEventDestroy(ev);
@@ -1058,6 +1062,37 @@ int lbm_set_external_forcing(lbm_ctx* c, const void* force, int realkind) {
     }
     (void)nc;
     c->has_force = true;
+    return LBM_OK;
+}
+
+int lbm_set_buoyancy(lbm_ctx* c, const void* pottemp, double theta0, double scaling, int realkind) {
+    if (int rc_ = check_ctx(c)) return rc_;
+    if (int rc = check_no_part(c)) return rc;
+    if (!pottemp) {
+        c->buoy_on = false;
+        return LBM_OK;
+    }
+    if (int rc = check_kind(c, realkind)) return rc;
+    if (!(theta0 != 0.0) || !std::isfinite(theta0) || !std::isfinite(scaling))
+        return fail(c, LBM_EINVAL, "buoyancy: theta0 must be finite and non-zero, scaling finite");
+    // the reference layout (0:nx+1, 0:ny+1, 0:nz+1) -> the cell layout of the compute type
+    const Layout& L = c->lay;
+    const int rb = c->ops->real_bytes;
+    std::vector<unsigned char> img((size_t)L.cplane * rb, 0);
+    for (int k = 0; k <= L.nz + 1; ++k)
+        for (int j = 0; j <= L.ny + 1; ++j)
+            for (int i = 0; i <= L.nx + 1; ++i) {
+                const size_t hc = ((size_t)k * (L.ny + 2) + j) * (L.nx + 2) + i;
+                const double v = realkind == 8 ? static_cast<const double*>(pottemp)[hc]
+                                               : static_cast<const float*>(pottemp)[hc];
+                put_real(img.data(), (size_t)L.cidx(i, j, k), v, rb);
+            }
+    if (int rc = dev_alloc(c, c->theta, img.size())) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->theta.p, img.data(), img.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->btheta0 = round_real(c, theta0);
+    c->bscale = round_real(c, scaling);
+    c->buoy_on = true;
     return LBM_OK;
 }
 

@@ -57,6 +57,13 @@ struct lbm_ctx {
     bool turb_on = false;
     lbm::TurbHost th{};
     lbm_host::DevBuf t_rot, t_theta, t_blade, t_nfoil, t_nang, t_foil, t_pts, t_box;
+    // adjoint workspace (lbm_adjoint_run): checkpoints, recomputed states, adjoint buffers, the final
+    // state, gradient and objective accumulators; kept between runs, grown when a run needs more
+    lbm_host::DevBuf adj_ck, adj_snaps, adj_lam, adj_fin, adj_gF, adj_gS, adj_acc;
+    // buoyancy forcing (lbm_set_buoyancy): potential temperature, reference temperature, scaling
+    lbm_host::DevBuf theta;
+    bool buoy_on = false;
+    double btheta0 = 0.0, bscale = 0.0;
     // time averaging: sums of u, v, w, u^2, v^2, w^2 (6 cell planes of the compute type)
     lbm_host::DevBuf avg;
     bool avg_on = false;

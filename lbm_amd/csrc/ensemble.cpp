@@ -271,8 +271,8 @@ int lbm_ensemble_create(lbm_ctx* const* members, int n, lbm_ens** out) {
             return bad(LBM_EINVAL, "member " + std::to_string(m) + ": ensembles take FP32, FP64, FP16 or BF16 storage");
         if (d.jbnd == LBM_BND_HALO)
             return bad(LBM_EINVAL, "member " + std::to_string(m) + ": halo domains are stepped on their own");
-        if (c->turb_on)
-            return bad(LBM_EINVAL, "member " + std::to_string(m) + ": members with turbines are stepped on their own");
+        if (c->turb_on || c->buoy_on)
+            return bad(LBM_EINVAL, "member " + std::to_string(m) + ": members with turbines or buoyancy are stepped on their own");
         if (closed_axes(d) > 1 || (closed_axes(d) == 1 && c->fold_dir < 0))
             return bad(LBM_EINVAL, "member " + std::to_string(m) +
                                        ": ensembles take members with at most one closed axis (its face pass runs inside the step kernel)");

@@ -45,6 +45,8 @@ struct StepHost {
     double ampl;
     const void* F[3];
     int flo[3], fhi[3];   // cells (i, j, k; inclusive) whose force is read; the others have none
+    const void* theta;    // potential temperature (cell layout) of the buoyancy forcing, or nullptr
+    double theta0, bscale;
     void* avg;            // averaging sums (6 cell planes of the compute type) or nullptr
     void* tau_out;
     int fold_dir, fold_opt_lo, fold_opt_hi;   // closed face performed by the step kernel

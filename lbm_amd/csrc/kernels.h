@@ -53,6 +53,8 @@ struct StepArgs {
     const R* Fy;
     const R* Fz;
     int flo[3], fhi[3];    // cells (i, j, k; inclusive) whose force is read (the others have none)
+    const R* theta;        // potential temperature (cell layout) of the buoyancy forcing; nullptr: off
+    R btheta0, bscale;     // its reference temperature and g * time^2 / length
     R* tau_out;            // cell layout; nullptr: tau not stored
     int fold_dir;          // closed axis whose face pass this kernel performs (-1: none)
     int fold_opt_lo, fold_opt_hi;
@@ -69,10 +71,11 @@ struct StepArgs {
     int part, part_axis, part_edge;
 };
 
-// Signed sum  sx*a + sy*b + sz*c  for compile-time signs in {-1,0,1}: adds only.
+// Signed sum  sx*a + sy*b + sz*c  for compile-time signs in {-1,0,1}: adds only.  With no term the
+// result is -0, the additive identity, so that a sum it starts folds away.
 template <typename R>
 __host__ __device__ __forceinline__ R sdot(int sx, int sy, int sz, R a, R b, R c) {
-    R s = 0;
+    R s = -R(0);
     bool first = true;
     if (sx != 0) { s = sx > 0 ? a : -a; first = false; }
     if (sy != 0) { s = first ? (sy > 0 ? b : -b) : (sy > 0 ? s + b : s - b); first = false; }
@@ -89,6 +92,18 @@ __host__ __device__ __forceinline__ R recip(R x) {
     if constexpr (std::is_same<R, float>::value) return __builtin_amdgcn_rcpf(x);
 #endif
     return R(1) / x;
+}
+
+// Square root: the hardware's in FP32 device code (the Vreman ratio is a normal number wherever it
+// is used), the IEEE root otherwise.  The adjoint uses it; the forward collision keeps the IEEE
+// root (a one-ulp change of the eddy viscosity moves the realization of a turbulent run).
+template <typename R>
+__host__ __device__ __forceinline__ R sqrt_r(R x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (std::is_same<R, float>::value) return __builtin_amdgcn_sqrtf(x);
+#endif
+    using std::sqrt;   // the tangent type brings its own sqrt
+    return sqrt(x);
 }
 
 // Hermite equilibrium of direction l (src/m_postcoll_kernel.F90:136-183 in closed form):
@@ -315,9 +330,23 @@ __host__ __device__ __forceinline__ void collide_cell(const StepArgs<typename C:
     bool in_fbox = true;
     if constexpr (sizeof(R) <= 8)
         in_fbox = i >= a.flo[0] && i <= a.fhi[0] && j >= a.flo[1] && j <= a.fhi[1] && k >= a.flo[2] && k <= a.fhi[2];
-    if (a.Fx != nullptr && rh > R(0) && in_fbox) {
+    if (a.theta == nullptr) {
+        if (a.Fx != nullptr && rh > R(0) && in_fbox) {
+            const R ir = recip(rh);
+            add_feq_diff(ux - a.Fx[oc] * ir, uy - a.Fy[oc] * ir, uz - a.Fz[oc] * ir, std::integral_constant<bool, false>());
+        }
+    } else if (rh > R(0)) {
+        // buoyancy added to the forcing of every interior cell (src/m_buoyancy_forcing_kernel.F90:37,
+        // the reference's operation order): F_z - scaling * (theta - theta0) / theta0
+        R Fx = 0, Fy = 0, Fz = 0;
+        if (a.Fx != nullptr && in_fbox) {
+            Fx = a.Fx[oc];
+            Fy = a.Fy[oc];
+            Fz = a.Fz[oc];
+        }
+        Fz = Fz - (a.bscale * (a.theta[oc] - a.btheta0)) / a.btheta0;
         const R ir = recip(rh);
-        add_feq_diff(ux - a.Fx[oc] * ir, uy - a.Fy[oc] * ir, uz - a.Fz[oc] * ir, std::integral_constant<bool, false>());
+        add_feq_diff(ux - Fx * ir, uy - Fy * ir, uz - Fz * ir, std::integral_constant<bool, false>());
     }
 
     // Solid cells: swap the opposite pairs (2,3),(4,5),... (src/m_solids.F90:19-32).

@@ -85,6 +85,9 @@ struct Launch {
             a.flo[d] = h.flo[d];
             a.fhi[d] = h.fhi[d];
         }
+        a.theta = static_cast<const R*>(h.theta);
+        a.btheta0 = (R)h.theta0;
+        a.bscale = (R)h.bscale;
         a.tau_out = static_cast<R*>(h.tau_out);
         a.fold_dir = h.fold_dir;
         a.fold_opt_lo = h.fold_opt_lo;
@@ -353,8 +356,8 @@ struct Launch {
     static void adj_step_t(const Layout& L, const StepArgs<R>& a, const void* ly, void* lx, const AdjBox& box, R kappa,
                            int coll, double* gF, hipStream_t s) {
         if constexpr (kAdj) {
-            const dim3 block(64, 4, 1);
-            const dim3 grid((L.n(L.ax0) + 63) / 64, (L.n(L.ax1) + 3) / 4, L.n(L.ax2));
+            const dim3 block(256, 1, 1);
+            const dim3 grid((L.n(L.ax0) * L.n(L.ax1) + 255) / 256, L.n(L.ax2), 1);
             const auto* lyp = static_cast<const typename C::raw*>(ly);
             auto* lxp = static_cast<typename C::raw*>(lx);
             // 32-bit byte offsets from one base when a buffer is below 4 GiB

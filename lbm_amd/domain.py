@@ -36,7 +36,7 @@ EXPORTS = ["lbm_create", "lbm_destroy", "lbm_last_error", "lbm_set_state", "lbm_
            "lbm_objective_run", "lbm_adjoint_run", "lbm_set_tangent", "lbm_objective_tangent",
            "lbm_generate_turbulence", "lbm_set_averaging", "lbm_get_averages",
            "lbm_halo_bytes", "lbm_halo_pack", "lbm_halo_unpack", "lbm_step_part",
-           "lbm_set_turbines", "lbm_get_turbine_theta"]
+           "lbm_set_turbines", "lbm_get_turbine_theta", "lbm_set_buoyancy"]
 
 
 class Desc(C.Structure):
@@ -96,6 +96,7 @@ def load_library(path: str = LIB_PATH):
         "lbm_set_timing": ([vp, i32], i32),
         "lbm_timing_read": ([vp, C.POINTER(dbl), C.POINTER(i32), C.POINTER(dbl), C.POINTER(i32)], i32),
         "lbm_get_totals": ([vp, C.POINTER(dbl), C.POINTER(dbl)], i32),
+        "lbm_set_buoyancy": ([vp, vp, dbl, dbl, i32], i32),
         "lbm_get_totals_noalloc": ([vp, C.POINTER(dbl), C.POINTER(dbl)], i32),
         "lbm_get_info": ([vp, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_int64)], i32),
         "lbm_ensemble_create": ([C.POINTER(vp), i32, C.POINTER(vp)], i32),
@@ -124,6 +125,15 @@ def load_library(path: str = LIB_PATH):
         fn.restype = res
     _lib = lib
     return lib
+
+
+def buoyancy_scaling(length=1.0, vel=1.0, dtype=np.float32):
+    """scaling = g * p2l%time**2 / p2l%length with g = 9.81 and p2l%time = length / vel, every
+    operation rounded in dtype as the reference computes it (src/m_buoyancy_forcing.F90:33-34,
+    src/m_readinfile.F90:171)."""
+    g, L, V = dtype(9.81), dtype(length), dtype(vel)
+    t = dtype(L / V)
+    return float(dtype(dtype(g * dtype(t * t)) / L))
 
 
 def _p(a):
@@ -287,6 +297,19 @@ class Domain:
             return
         F = self._arr(force, self.shape_cell + (3,))
         self._chk(self.lib.lbm_set_external_forcing(self.h, _p(F), F.itemsize), "lbm_set_external_forcing")
+
+    def set_buoyancy(self, pottemp, theta0=300.0, scaling=None, length=1.0, vel=1.0):
+        """Buoyancy forcing from the potential temperature field (cell layout); None switches it
+        off.  scaling defaults to the reference's g * p2l%time^2 / p2l%length (g = 9.81,
+        p2l%time = length / vel) in the compute type's rounding."""
+        if pottemp is None:
+            self._chk(self.lib.lbm_set_buoyancy(self.h, None, 300.0, 0.0, 8), "lbm_set_buoyancy")
+            return
+        if scaling is None:
+            scaling = buoyancy_scaling(length, vel, np.float64 if self.real == np.float64 else np.float32)
+        T = self._arr(pottemp, self.shape_cell)
+        self._chk(self.lib.lbm_set_buoyancy(self.h, _p(T), float(theta0), float(scaling), T.itemsize),
+                  "lbm_set_buoyancy")
 
     def set_turbines(self, t):
         """Actuator-line turbines (lbm_set_turbines) from the dict of lbm_amd.turbines.from_infile;

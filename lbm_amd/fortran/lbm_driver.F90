@@ -11,7 +11,9 @@
 ! Time averages (laveraging, avestart, avesave) are accumulated on the device and written to
 ! averages/averages_0000_<avesave>.uf.  Actuator-line turbines (the infile's turbine section, the
 ! NREL 5-MW blade, ./Airfoils/*.dat, restart/theta_0000_<it6>.uf) run on the device.  Not carried
-! over (outside the lattice update): buoyancy/ABL, tracers, Tecplot and NetCDF output, the random
+! over (outside the lattice update): the transport of the potential temperature that iablvisc = 2
+! needs (heatflux + advection; the library's buoyancy forcing, lbm_set_buoyancy, takes the field
+! from its caller), tracers, Tecplot and NetCDF output, the random
 ! perturbation of inipert (nt0 = 0 starts from the inflow profile).  Geometry: 'none' or the
 ! reference's 'cylinder' (src/m_cylinder.F90).
 program lbm_driver
@@ -130,7 +132,7 @@ program lbm_driver
       enddo
    endif
    close(iunit)
-   if (iablvisc == 2) stop 'lbm_driver: the buoyancy scheme is not part of this build'
+   if (iablvisc == 2) stop 'lbm_driver: iablvisc = 2 needs the potential-temperature transport, not in this driver'
    ! conversions as src/m_readinfile.F90:133,171-186
    uini = uini / p_vel
    p_time = p_len / p_vel

@@ -238,3 +238,16 @@ def write_theta(path: str, theta, dtype=np.float32, subrecords: bool = False) ->
 def read_theta(path: str, dtype=np.float32) -> np.ndarray:
     with open(path, "rb") as fh:
         return np.frombuffer(read_record(fh), dtype=dtype).copy()
+
+
+def write_pottemp(path: str, pottemp, subrecords: bool = False) -> None:
+    """restart/pottemp_<tile>_<it>.uf: the potential temperature pottemp(0:nx+1,0:ny+1,0:nz+1), one
+    record (src/m_saverestart.F90:102-114, read back by src/m_readrestart.F90:119-136); the array is
+    given C-contiguous as (nz+2, ny+2, nx+2) and keeps its dtype."""
+    with open(path, "wb") as fh:
+        write_record(fh, np.ascontiguousarray(pottemp).tobytes(), subrecords)
+
+
+def read_pottemp(path: str, shape, dtype=np.float32) -> np.ndarray:
+    with open(path, "rb") as fh:
+        return np.frombuffer(read_record(fh), dtype=dtype).reshape(shape).copy()

@@ -17,6 +17,7 @@
  *   fequil3_block_kernel       src/m_fequil3_block_kernel.F90:7-117
  *   inflow_turbulence_apply    src/m_inflow_turbulence_apply.F90:4-31
  *   turbine_forcing (CPU path) src/m_turbine_forcing.F90:8-140 and the routines it calls
+ *   buoyancy_forcing_kernel    src/m_buoyancy_forcing_kernel.F90:6-49
  *   fequil3_kernel             src/m_fequil3_kernel.F90:7-124 (initialisation)
  *   fillghosts                 src/m_fillghosts.F90:4-28 (initialisation)
  * Array layout is the reference's: f(nl,0:nx+1,0:ny+1,0:nz+1), column major, so element
@@ -341,6 +342,21 @@ OR_EXPORT void or_forcings_apply(const or_params *P, REAL *f, const REAL *F, con
             REAL feq1 = L.w[l] * (tmp1 + tmp2 + tmp3);
             fc[l] = fc[l] + (feq1 - feq0);
         }
+    }
+}
+
+/* buoyancy_forcing_kernel: external_forcing(3,idx) - scaling*(pottemp(idx) - theta0)/theta0 for the
+ * interior cells, evaluated left to right as the Fortran expression reads
+ * (src/m_buoyancy_forcing_kernel.F90:20-37).  pottemp is (0:nx+1,0:ny+1,0:nz+1). */
+OR_EXPORT void or_buoyancy_forcing(const or_params *P, REAL *F, const REAL *pottemp, REAL scaling, REAL theta0)
+{
+    for (int k = 1; k <= P->nz; k++)
+    for (int j = 1; j <= P->ny; j++)
+    for (int i = 1; i <= P->nx; i++) {
+        size_t c = CIDX(P, i, j, k);
+        REAL d = pottemp[c] - theta0;
+        REAL s = scaling * d;
+        F[3 * c + 2] = F[3 * c + 2] - s / theta0;
     }
 }
 
@@ -819,16 +835,21 @@ OR_EXPORT void or_boundarycond(const or_params *P, REAL *f1, REAL *f2, const REA
  * the final f1 is fB, else 0.
  *   uvel_shear[nz], uvel_time[it], udir_time[it] indexed by absolute step it (arrays of
  *   length it1+1); turb (may be NULL): uu,vv,ww as (ny,nz,0:nrturb); F (may be NULL):
- *   external_forcing applied when apply_forcing != 0; blank (may be NULL) enables solids. */
+ *   external_forcing applied when apply_forcing != 0; blank (may be NULL) enables solids.
+ *   pottemp (may be NULL): it1 - it0 potential-temperature fields (0:nx+1,0:ny+1,0:nz+1), the one
+ *   of step it0 + 1 + m at offset m; each step's external_forcing is then the turbines' (or zero)
+ *   plus the buoyancy of its field (src/main.F90:290-293), applied by forcings_apply (:306). */
 OR_EXPORT int or_run(const or_params *P, REAL *fA, REAL *fB, REAL *tau, REAL *rho, REAL *u, REAL *v, REAL *w,
                      const REAL *ablvisc, const uint8_t *blank,
                      const REAL *uvel_shear, const REAL *uvel_time, const REAL *udir_time,
                      const REAL *uu, const REAL *vv, const REAL *ww, int nrturb, REAL ampl,
-                     const REAL *F, int apply_forcing, int it0, int it1, const or_turbines *T)
+                     const REAL *F, int apply_forcing, int it0, int it1, const or_turbines *T,
+                     const REAL *pottemp, REAL bscale, REAL btheta0)
 {
     REAL *f1 = fA, *f2 = fB;
-    REAL *Ft = NULL;   /* external_forcing of the turbines (src/main.F90:291-292) */
-    if (T) {
+    REAL *Ft = NULL;   /* external_forcing of the producers (src/main.F90:290-293) */
+    const size_t ncell = (size_t)(P->nx + 2) * (P->ny + 2) * (P->nz + 2);
+    if (T || pottemp) {
         Ft = (REAL *)malloc(sizeof(REAL) * 3 * (size_t)(P->nx + 2) * (P->ny + 2) * (P->nz + 2));
         F = Ft;
         apply_forcing = 1;
@@ -840,6 +861,8 @@ OR_EXPORT int or_run(const or_params *P, REAL *fA, REAL *fB, REAL *tau, REAL *rh
     int swapped = 0;
     for (int it = it0 + 1; it <= it1; it++) {
         if (T) or_turbine_forcing(P, T, rho, u, v, w, Ft);
+        else if (pottemp) memset(Ft, 0, sizeof(REAL) * 3 * ncell);
+        if (pottemp) or_buoyancy_forcing(P, Ft, pottemp + ncell * (size_t)(it - it0 - 1), bscale, btheta0);
         if (uu) {
             int lit = it % nrturb;
             if (lit == 0) lit = nrturb;

@@ -193,8 +193,10 @@ class OracleSim:
 
     # -- time loop --------------------------------------------------------------------
     def run(self, it0, it1, uini=0.0, udir=0.0, uvel_shear=None, uvel_time=None, udir_time=None,
-            blank=None, turb=None, forcing=None, ablvisc=None, turbines=None):
-        """Steps it0+1..it1.  turbines: the dict of turbines_from_infile; its theta advances."""
+            blank=None, turb=None, forcing=None, ablvisc=None, turbines=None, buoyancy=None):
+        """Steps it0+1..it1.  turbines: the dict of turbines_from_infile; its theta advances.
+        buoyancy: (pottemp, scaling, theta0) with pottemp the it1 - it0 potential-temperature fields
+        (nz+2, ny+2, nx+2) of the steps in order (src/main.F90:293)."""
         nx, ny, nz = self.dims
         dt = self.dt
         shear = np.ones(nz, dt) if uvel_shear is None else np.asarray(uvel_shear, dt)
@@ -211,11 +213,17 @@ class OracleSim:
         real = C.c_float if self.prec == "f32" else C.c_double
         keep = []
         T = None if turbines is None else C.byref(_turbines_struct(real, turbines, keep))
+        pot, bscale, btheta0 = None, 0.0, 1.0
+        if buoyancy is not None:
+            pot, bscale, btheta0 = buoyancy
+            pot = np.ascontiguousarray(pot, dt)
+            if pot.shape != (it1 - it0, nz + 2, ny + 2, nx + 2):
+                raise ValueError(f"buoyancy fields: {pot.shape}, expected {(it1 - it0, nz + 2, ny + 2, nx + 2)}")
         sw = self.lib.or_run(C.byref(self.p), _ptr(self.f1), _ptr(self.f2), _ptr(self.tau),
                              _ptr(self.rho), _ptr(self.u), _ptr(self.v), _ptr(self.w), _ptr(abl),
                              _ptr(bl), _ptr(shear), _ptr(ut), _ptr(dtm), _ptr(uu), _ptr(vv), _ptr(ww),
                              C.c_int(nrturb), real(ampl), _ptr(F), C.c_int(F is not None), C.c_int(it0),
-                             C.c_int(it1), T)
+                             C.c_int(it1), T, _ptr(pot), real(bscale), real(btheta0))
         self.cur ^= sw
         return self.f1
 
@@ -260,13 +268,14 @@ def inflow_series(files, nz, nt0, nt1, uini, udir, prec="f32"):
 def infile_text(*, ibgk=2, ihrr=0, ivreman=0, smagorinsky=0.0, nt0=1, nt1=2, ibnd=0, jbnd=0, kbnd=0,
                 uini=0.0, udir=0.0, turb=False, ampl=0.0, nrturb=100, visc=1.0 / 30.0,
                 experiment="none", ltesting=True, ldump=True, ltiming=False, iout=None,
-                length=1.0, vel=1.0, turbines=None):
+                length=1.0, vel=1.0, turbines=None, iablvisc=0, ablheight=400.0, istable=1):
     """An infile.in in the line order src/m_readinfile.F90:84-160 reads.  With the default
     C_l = C_u = 1 the viscosity line is the non-dimensional kinevisc (src/m_readinfile.F90:182-186)
     and uini the lattice speed; ``length``/``vel`` set C_l [m] and C_u [m/s] (then the caller
     converts: kinevisc = visc / (C_l * C_u), lattice uini = uini / C_u).  ``turbines``: dict with
     pitch [deg], rpm, tsr, tiploss and pos = [(ipos, jpos, kpos, yaw, tilt), ...]
-    (src/m_readinfile.F90:143-161)."""
+    (src/m_readinfile.F90:143-161).  iablvisc, ablheight, istable: the atmospheric boundary layer
+    section (src/m_readinfile.F90:101-103)."""
     iout = iout if iout is not None else 10 ** 7
     L = lambda b: "T" if b else "F"
     smagorinsky, uini, udir, ampl, visc = (float(x) for x in (smagorinsky, uini, udir, ampl, visc))
@@ -276,7 +285,7 @@ def infile_text(*, ibgk=2, ihrr=0, ivreman=0, smagorinsky=0.0, nt0=1, nt1=2, ibn
         "# Experiment configuration",
         f" {experiment}", f" {ibgk}", f" {ihrr}", f" {ivreman} {smagorinsky!r}",
         "# Atmospheric boundary layer",
-        " 0", " 400.0", " 1",
+        f" {int(iablvisc)}", f" {float(ablheight)!r}", f" {int(istable)}",
         "# Time stepping and outputs",
         f" {nt0}", f" {nt1}", f" {iout}", f" {10 ** 7}", f" 0 {10 ** 7} {10 ** 7}", " 2",
         "# Boundary conditions",
@@ -301,13 +310,15 @@ def infile_text(*, ibgk=2, ihrr=0, ivreman=0, smagorinsky=0.0, nt0=1, nt1=2, ibn
 
 
 def run_reference(exe, f_in, *, nt0, nt1, turb_fields=None, workdir=None, threads=1, timeout=3600,
-                  read_output=True, files=None, theta=None, **opts):
+                  read_output=True, files=None, theta=None, pottemp=None, **opts):
     """Run a reference executable (built by oracle/build_ref.py) from restart state ``f_in``
     at step nt0 to nt1.  Returns (f_out, stdout).  f_out is the restart record written at
     nt1 (post-streaming f1 with ghosts, src/main.F90:398).  ``files``: name -> text of input
     files written into the working directory first (uvel_shear.dat, uvel_time.dat, Airfoils/*.dat).
     ``theta``: the turbines' azimuths at nt0 (runs with turbines need that restart record); with it
-    the result is (f_out, theta_out, stdout)."""
+    the result is (f_out, theta_out, stdout).  ``pottemp``: the potential temperature at nt0 (runs with
+    iablvisc = 2 need that restart record, src/m_readrestart.F90:119-136); with it the result is
+    (f_out, pottemp_out, stdout), pottemp_out the record the reference writes at nt1."""
     import sys
     sys.path.insert(0, os.path.dirname(HERE))
     from lbm_amd import fortran_io as fio
@@ -319,6 +330,8 @@ def run_reference(exe, f_in, *, nt0, nt1, turb_fields=None, workdir=None, thread
         if theta is not None:
             fio.write_theta(os.path.join(wd, "restart", fio.restart_name(nt0, prefix="theta")), theta,
                             np.asarray(f_in).dtype)
+        if pottemp is not None:
+            fio.write_pottemp(os.path.join(wd, "restart", fio.restart_name(nt0, prefix="pottemp")), pottemp)
         if turb_fields is not None:
             uu, vv, ww, rr = turb_fields
             fio.write_turbulence(os.path.join(wd, "restart", fio.restart_name(nt0, prefix="turbulence")),
@@ -340,6 +353,10 @@ def run_reference(exe, f_in, *, nt0, nt1, turb_fields=None, workdir=None, thread
         if theta is not None:
             th = fio.read_theta(os.path.join(wd, "restart", fio.restart_name(nt1, prefix="theta")), f_out.dtype)
             return f_out, th, p.stdout
+        if pottemp is not None:
+            pt = fio.read_pottemp(os.path.join(wd, "restart", fio.restart_name(nt1, prefix="pottemp")),
+                                  np.asarray(pottemp).shape, f_out.dtype)
+            return f_out, pt, p.stdout
         return f_out, p.stdout
     finally:
         if own:

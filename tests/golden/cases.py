@@ -63,7 +63,26 @@ CASES = {
     # two yawed and tilted rotors whose deposits overlap, D3Q19 BGK, periodic i and j, no-slip k
     "q19_bgk_turbines2": (19, "sp", (28, 20, 20), 20, 10, "none", 0,
                           dict(ibgk=2, ibnd=0, jbnd=0, kbnd=11, uini=0.04, udir=0.0, visc=0.02)),
+    # buoyancy (iablvisc = 2, stable: no surface heat flux), D3Q27 reg + Vreman, periodic i and j,
+    # no-slip k=1 / free-slip k=nz; the reference advects its potential temperature every step
+    "q27_regvre_buoyancy": (27, "sp", (12, 10, 12), 10, 11, "none", 0,
+                            dict(ibgk=3, ihrr=1, ivreman=1, smagorinsky=0.16733, ibnd=0, jbnd=0, kbnd=12,
+                                 uini=0.03, udir=0.0, visc=0.01)),
 }
+
+# Buoyancy cases: the initial potential temperature 300 K + 0.5 K per cell above k = 6 + U(-2, 2) K
+# (seeded), stability class 1 (src/m_heatflux.F90:30-45 adds no flux), in the units of the
+# turbine cases (exact conversions); scaling = g * p2l%time^2 / p2l%length.
+BUOYANCY = {"q27_regvre_buoyancy": dict(istable=1, ablheight=400.0, theta0=300.0)}
+
+
+def initial_pottemp(name):
+    q, prec, dims, steps, seed, exp, nrturb, o = CASES[name]
+    nx, ny, nz = dims
+    rng = np.random.default_rng(1000 + seed)
+    k = np.arange(nz + 2, dtype=np.float64)[:, None, None]
+    th = 300.0 + 0.5 * (k - 6) + rng.uniform(-2.0, 2.0, (nz + 2, ny + 2, nx + 2))
+    return th.astype(np.float32 if prec == "sp" else np.float64)
 
 # Turbines of a case (infile section src/m_readinfile.F90:143-161): (ipos, jpos, kpos, yaw, tilt [deg]),
 # RPM, pitch [deg], and the azimuths theta [rad] of the restart record.  C_l = 16 m and C_u = 256 m/s
@@ -78,7 +97,14 @@ TURBINE_LENGTH, TURBINE_VEL = 16.0, 256.0
 
 
 def infile_units(name):
-    """Extra infile_text arguments of a case: physical units and the turbine section."""
+    """Extra infile_text arguments of a case: physical units, the turbine section, the atmospheric
+    boundary layer section."""
+    if name in BUOYANCY:
+        q, prec, dims, steps, seed, exp, nrturb, o = CASES[name]
+        b = BUOYANCY[name]
+        return dict(length=TURBINE_LENGTH, vel=TURBINE_VEL, uini=o.get("uini", 0.0) * TURBINE_VEL,
+                    visc=o["visc"] * TURBINE_LENGTH * TURBINE_VEL, iablvisc=2, istable=b["istable"],
+                    ablheight=b["ablheight"])
     if name not in TURBINES:
         return {}
     q, prec, dims, steps, seed, exp, nrturb, o = CASES[name]

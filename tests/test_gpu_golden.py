@@ -68,7 +68,15 @@ def run_case(name):
         foils = [fx["foil"][k, :n] for k, n in enumerate(fx["nang"])]
         dom.set_turbines(turbines.from_infile(t["pos"], t["rpm"], t["pitch"], cases.TURBINE_LENGTH, cases.TURBINE_VEL,
                                               foils, dt, theta=t["theta"]))
-    dom.step(1, steps)
+    if name in cases.BUOYANCY:
+        # the reference's potential temperature of each step, set before that step
+        from lbm_amd.domain import buoyancy_scaling
+        sc = buoyancy_scaling(cases.TURBINE_LENGTH, cases.TURBINE_VEL, dt)
+        for n in range(steps):
+            dom.set_buoyancy(fx["pottemp"][n], theta0=cases.BUOYANCY[name]["theta0"], scaling=sc)
+            dom.step(1 + n, 1)
+    else:
+        dom.step(1, steps)
     f = dom.get_state()
     if name in cases.TURBINES:
         theta = dom.turbine_theta()
@@ -90,3 +98,50 @@ def test_hip_matches_reference_build(name):
     assert d.max() <= tol_f, f"max |df| {d.max():.3e} at {np.unravel_index(d.argmax(), d.shape)}"
     er, eu = rel_errors(f, ref, q)
     assert er <= tol_m and eu <= tol_m, (er, eu)
+
+
+def test_turbines_and_buoyancy_together_match_oracle():
+    """Both external-force producers at once (src/main.F90:290-293: external_forcing = 0, the turbines'
+    deposit, then the buoyancy of every interior cell, applied by forcings_apply): the library against
+    the oracle on the two-rotor turbine case with a seeded potential temperature per step.  The
+    buoyancy moves the result off the turbine-only golden record by far more than the bar."""
+    import oracle
+    from lbm_amd import turbines
+    from lbm_amd.domain import Domain, buoyancy_scaling
+    name = "q19_bgk_turbines2"
+    meta, fx = cases.load(name)
+    q, prec, dims, steps, seed, exp, nrturb, o = cases.CASES[name]
+    nx, ny, nz = dims
+    f0, _ = cases.case_inputs(name)
+    t = cases.TURBINES[name]
+    foils = [fx["foil"][k, :n] for k, n in enumerate(fx["nang"])]
+    rng = np.random.default_rng(77)
+    pot = (300.0 + rng.uniform(-3.0, 3.0, (steps, nz + 2, ny + 2, nx + 2))).astype(np.float32)
+    sc = buoyancy_scaling(cases.TURBINE_LENGTH, cases.TURBINE_VEL)
+    print(f"turbines + buoyancy {name} {dims} {steps}", flush=True)
+    dom = Domain(q, dims, "f32", ibgk=o.get("ibgk", 2), ihrr=o.get("ihrr", 0), ivreman=o.get("ivreman", 0),
+                 smagorinsky=o.get("smagorinsky", 0.0), kinevisc=o["visc"], ibnd=o["ibnd"], jbnd=o["jbnd"],
+                 kbnd=o["kbnd"])
+    dom.set_state(f0)
+    dom.set_inflow(o.get("uini", 0.0), o.get("udir", 0.0))
+    dom.set_turbines(turbines.from_infile(t["pos"], t["rpm"], t["pitch"], cases.TURBINE_LENGTH, cases.TURBINE_VEL,
+                                          foils, np.float32, theta=t["theta"]))
+    for n in range(steps):
+        dom.set_buoyancy(pot[n], theta0=300.0, scaling=sc)
+        dom.step(1 + n, 1)
+    f = dom.get_state()
+    dom.close()
+    sim = oracle.OracleSim(q, dims, "f32", ibgk=o.get("ibgk", 2), ihrr=o.get("ihrr", 0), ivreman=o.get("ivreman", 0),
+                           smagorinsky=float(np.float32(o.get("smagorinsky", 0.0))),
+                           kinevisc=float(np.float32(o["visc"])), ibnd=o["ibnd"], jbnd=o["jbnd"], kbnd=o["kbnd"])
+    sim.restart(f0)
+    T = oracle.turbines_from_infile(t["pos"], t["rpm"], t["pitch"], cases.TURBINE_LENGTH, cases.TURBINE_VEL,
+                                    foils, "f32", theta=t["theta"])
+    ref = sim.run(1, 1 + steps, uini=float(np.float32(o.get("uini", 0.0))), turbines=T, buoyancy=(pot, sc, 300.0))
+    d = np.abs(f.astype(np.float64) - ref.astype(np.float64)).max()
+    off = np.abs(ref.astype(np.float64) - fx["f_out"].astype(np.float64)).max()
+    print(f"max |f - oracle| {d:.3e}; oracle with buoyancy vs turbine-only golden {off:.3e}", flush=True)
+    assert np.isfinite(f).all() and d <= 2e-6
+    er, eu = rel_errors(f, ref, q)
+    assert er <= 1e-5 and eu <= 1e-5, (er, eu)
+    assert off > 1e-5

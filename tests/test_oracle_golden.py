@@ -20,6 +20,18 @@ def turbines_of(name, fx):
                                        foils, prec, theta=t["theta"])
 
 
+def buoyancy_of(name, fx):
+    """(potential temperature of every step, scaling, theta0) of a buoyancy case: the fields the
+    reference's own transport produced (stored in the fixture)."""
+    if name not in cases.BUOYANCY:
+        return None
+    from lbm_amd.domain import buoyancy_scaling
+    prec = cases.CASES[name][1]
+    dt = np.float32 if prec == "sp" else np.float64
+    return (fx["pottemp"], buoyancy_scaling(cases.TURBINE_LENGTH, cases.TURBINE_VEL, dt),
+            float(dt(cases.BUOYANCY[name]["theta0"])))
+
+
 @pytest.mark.parametrize("name", sorted(cases.CASES))
 def test_oracle_matches_reference_build(name):
     meta, fx = cases.load(name)
@@ -40,8 +52,9 @@ def test_oracle_matches_reference_build(name):
     shear, ut, ud = oracle.inflow_series(cases.case_files(name), dims[2], 1, 1 + steps, uini, udir,
                                          "f32" if prec == "sp" else "f64")
     T = turbines_of(name, fx)
+    B = buoyancy_of(name, fx)
     f = sim.run(1, 1 + steps, uvel_shear=shear, uvel_time=ut, udir_time=ud, blank=cases.blanking(name), turb=tb,
-                turbines=T)
+                turbines=T, buoyancy=B)
     ref = fx["f_out"]
     if T is not None:
         assert np.array_equal(T["theta"], fx["theta_out"])
@@ -88,3 +101,28 @@ def test_turbines_drive_the_golden_cases(name):
     f = sim.run(1, 1 + steps, uini=float(np.float32(o.get("uini", 0.0))))
     assert np.abs(f - fx["f_out"]).max() > 1e-4   # measured 2.8e-4 and 3.4e-4
     assert fx["theta_out"].shape == (len(cases.TURBINES[name]["pos"]),)
+
+
+@pytest.mark.parametrize("name", sorted(cases.BUOYANCY))
+def test_buoyancy_drives_the_golden_cases(name):
+    """Without its buoyancy a buoyancy case's run does not reproduce the golden record (the golden
+    pins buoyancy_forcing and forcings_apply, src/main.F90:293-306); the stored fields are the
+    reference's: the first is the seeded initial one and its transport changes them every step."""
+    meta, fx = cases.load(name)
+    q, prec, dims, steps, seed, exp, nrturb, o = cases.CASES[name]
+    assert np.array_equal(fx["pottemp"][0], cases.initial_pottemp(name))
+    assert fx["pottemp"].shape == (steps, dims[2] + 2, dims[1] + 2, dims[0] + 2)
+    assert all(np.abs(fx["pottemp"][n + 1] - fx["pottemp"][n]).max() > 1e-3 for n in range(steps - 1))
+    f0, _ = cases.case_inputs(name)
+    sim = oracle.OracleSim(q, dims, "f32", ibgk=o.get("ibgk", 2), ihrr=o.get("ihrr", 0), ivreman=o.get("ivreman", 0),
+                           smagorinsky=float(np.float32(o.get("smagorinsky", 0.0))),
+                           kinevisc=float(np.float32(o["visc"])), ibnd=o["ibnd"], jbnd=o["jbnd"], kbnd=o["kbnd"])
+    sim.restart(f0)
+    f = sim.run(1, 1 + steps, uini=float(np.float32(o.get("uini", 0.0))))
+    assert np.abs(f - fx["f_out"]).max() > 1e-5   # measured 6.6e-5
+    # the first step's field alone instead of the transported ones: also off the record
+    sim.restart(f0)
+    B = buoyancy_of(name, fx)
+    f1 = sim.run(1, 1 + steps, uini=float(np.float32(o.get("uini", 0.0))),
+                 buoyancy=(np.repeat(fx["pottemp"][:1], steps, axis=0), B[1], B[2]))
+    assert not np.array_equal(f1, fx["f_out"])
