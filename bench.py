@@ -37,6 +37,11 @@ every step; --split steps it in two launches with the transfer during the second
     python bench.py --workload q19 [--storage f32|f16s]
 
 times D3Q19 BGK (tau = 0.6, periodic i and j, no-slip k) at 768x256x192.
+
+    python bench.py --workload large [--storage f32|f16s|bf16s]
+
+times the headline physics at the storage-study size 768x256x192 (D3Q27 FP32 populations: 5.2 GB
+per buffer, row-blocked and addressed per slab of the slowest axis).
 """
 from __future__ import annotations
 
@@ -60,8 +65,8 @@ HBM_PEAK_GBS = 8000.0
 SMAG = math.sqrt(0.07 / 2.5)
 
 
-def workload(storage):
-    return dict(q=27, dims=(384, 192, 128), storage=storage, ibgk=3, ihrr=1, ivreman=1, smagorinsky=SMAG,
+def workload(storage, dims=(384, 192, 128)):
+    return dict(q=27, dims=tuple(dims), storage=storage, ibgk=3, ihrr=1, ivreman=1, smagorinsky=SMAG,
                 kinevisc=1e-4, ibnd=1, jbnd=0, kbnd=12, uini=0.05, udir=0.0, cyl=(96, 96, 12), ip=10,
                 ampl=1e-3, corr=8.0, seed=2)
 
@@ -444,6 +449,28 @@ def bench_q19(a):
         "cpu_baseline": None, "finite": finite}), flush=True)
 
 
+def bench_large(a):
+    """The headline physics (D3Q27 reg + Vreman, inflow/outflow i, periodic j, no-slip k=1 /
+    free-slip k=nz, cylinder r=12 at (96,96), inflow forcing at i=10) at 768x256x192."""
+    w = workload(a.storage, (768, 256, 192))
+    nx, ny, nz = w["dims"]
+    nrturb = max(1001, a.steps + a.warmup + 1)
+    print(f"[bench] building inputs {w['dims']} q={w['q']} storage={a.storage}", file=sys.stderr, flush=True)
+    f0, turb, mask = make_inputs(w, nrturb)
+    res = run_timed(w, a.storage, f0, turb, mask, nrturb, a, info=True)
+    cells = nx * ny * nz
+    print(json.dumps({
+        "metric": "MLUPS + %HBM roofline (D3Q27 reg+Vreman, 768x256x192)", "value": round(res["mlups"], 1),
+        "unit": "MLUPS", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(res["ms_per_step"], 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": STORAGE_DTYPE[a.storage],
+        "data": "synthetic (seeded perturbed uniform flow, seeded smooth inflow forcing)",
+        "config": {"workload": "D3Q27 3rd-order regularized + Vreman (c=0.07), 768x256x192, inflow/outflow i, "
+                               "periodic j, no-slip k=1 / free-slip k=nz, cylinder r=12 at (96,96), inflow forcing at "
+                               "i=10, nu=1e-4", "cells": cells, "storage": a.storage, "parallelism": "single GPU",
+                   "buffer_bytes": res["info"]["device_bytes"] // 2, "row_pitch": res["info"]["row_pitch"]},
+        "roofline": roof_of(res, None, None), "finite": res["finite"], "cpu_baseline": None}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -465,7 +492,7 @@ def main():
     ap.add_argument("--error-steps", type=int, default=1000,
                     help="headline: step count of the storage-error comparison against FP64 (0: skip)")
     ap.add_argument("--block", type=int, nargs=2, default=(0, 0))
-    ap.add_argument("--workload", default="headline", choices=["headline", "ensemble", "adjoint", "tile", "q19"])
+    ap.add_argument("--workload", default="headline", choices=["headline", "ensemble", "adjoint", "tile", "q19", "large"])
     ap.add_argument("--tiles", type=int, default=8)
     ap.add_argument("--split", action="store_true",
                     help="tile workload: split steps (edge rows, pack, inner rows during the transfer)")
@@ -489,6 +516,9 @@ def main():
     if a.workload == "q19":
         bench_q19(a)
         return
+    if a.workload == "large":
+        bench_large(a)
+        return
 
     headline(a)
 
@@ -496,7 +526,7 @@ def main():
 STORAGE_DTYPE = {"f32": "f32", "f64": "f64", "f16s": "f16-store/f32", "bf16s": "bf16-store/f32"}
 
 
-def run_timed(w, storage, f0, turb, mask, nrturb, a):
+def run_timed(w, storage, f0, turb, mask, nrturb, a, info=False):
     """One storage form of the headline workload: untimed clock ramp and warm-up, then exactly
     a.steps timed steps.  HIP events bracket the step kernel of every max(1, steps // 10)-th timed
     step (at least 10 samples from 10 steps on); the wall clock brackets the whole timed region."""
@@ -529,12 +559,13 @@ def run_timed(w, storage, f0, turb, mask, nrturb, a):
     tm = dom.timing_read()
     rho, u, v, ww = dom.macrovars()
     finite = bool(np.isfinite(rho).all() and np.isfinite(u).all())
+    layout = dom.info() if info else None
     dom.close()
     cells = w["dims"][0] * w["dims"][1] * w["dims"][2]
     bpc = bytes_per_cell(w["q"], storage)
     kern_ms = tm["step_ms"] / max(tm["n_step"], 1)
     ms = secs / a.steps * 1e3
-    return {"mlups": cells * a.steps / secs / 1e6, "ms_per_step": ms, "kernel_ms": kern_ms,
+    return {"info": layout, "mlups": cells * a.steps / secs / 1e6, "ms_per_step": ms, "kernel_ms": kern_ms,
             "kernel_samples": tm["n_step"], "bc_ms": tm["bc_ms"] / max(tm["n_bc"], 1),
             "achieved": bpc * cells / (kern_ms * 1e-3) / 1e9,
             "achieved_wall": bpc * cells / (ms * 1e-3) / 1e9, "bpc": bpc, "finite": finite}

@@ -526,8 +526,13 @@ __device__ __forceinline__ void buffer_store(void* base, int nbytes, uint32_t vo
 // One cell of the step at storage coordinates (ca, cb, cc) (1-based along ax0, ax1, ax2):
 // gather the streamed populations, collide, store, and the folded face pass.
 // VAR bit 0: non-temporal stores, bit 1: non-temporal loads (measured slower; kept for tuning),
-// bit 3: the buffer is below 4 GiB (32-bit offsets from one base).
-template <int Q, class C, bool BGK3, bool HRR, bool VRE, int VAR>
+// bit 3: the buffer is below 4 GiB (32-bit offsets from one base), bit 4: row-blocked buffers of
+// any size addressed per slab of the slowest axis (one scalar base per source slab, 32-bit offsets
+// within it; needs AXS >= 0 and a slab below 4 GiB).
+// AXS >= 0: the contiguous storage axis is AXS (known at compile time, the other two in
+// make_layout's order), so i, j, k map to (ca, cb, cc) without run-time selects, and the caller's
+// wave-uniform cb, cc keep the row's offsets, wraps and checks in scalar registers.
+template <int Q, class C, bool BGK3, bool HRR, bool VRE, int VAR, int AXS = -1>
 __device__ __forceinline__ void step_cell(const StepArgs<typename C::real>& a, int ca, int cb, int cc) {
     using R = typename C::real;
     using raw = typename C::raw;
@@ -535,7 +540,14 @@ __device__ __forceinline__ void step_cell(const StepArgs<typename C::real>& a, i
     const Layout& L = a.lay;
     if (ca > L.n(L.ax0) || cb > L.n(L.ax1) || cc > L.n(L.ax2)) return;
     int i, j, k;
-    L.ijk(ca, cb, cc, i, j, k);
+    constexpr int A1 = AXS == 0 ? 1 : 0, A2 = AXS == 2 ? 1 : 2;
+    if constexpr (AXS < 0) {
+        L.ijk(ca, cb, cc, i, j, k);
+    } else {
+        i = AXS == 0 ? ca : A1 == 0 ? cb : cc;
+        j = AXS == 1 ? ca : A1 == 1 ? cb : cc;
+        k = AXS == 2 ? ca : A1 == 2 ? cb : cc;
+    }
 
     // Source offsets: population l comes from (i - cx, j - cy, k - cz), periodic directions
     // folded onto the interior.  Index d = c + 1.
@@ -543,7 +555,10 @@ __device__ __forceinline__ void step_cell(const StepArgs<typename C::real>& a, i
     // for a periodic axis at its first/last cell by the uniform wrap distance (no per-offset
     // multiplies).
     const int s = a.stream;
-    const int oc0 = L.cell(i, j, k);
+    // the contiguous axis has unit stride in both buffer arrangements (make_layout)
+    const int oc0 = AXS < 0 ? L.cell(i, j, k)
+                            : L.base + ca + cb * (A1 == 0 ? L.si : A1 == 1 ? L.sj : L.sk) +
+                                  cc * (A2 == 0 ? L.si : A2 == 1 ? L.sj : L.sk);
     int ox[3], oy[3], oz[3];
     auto offsets = [&](int x, int n, int per, int st, int (&o)[3]) {
         // o[d]: offset change of the source x - s*(d-1) along this axis
@@ -557,9 +572,18 @@ __device__ __forceinline__ void step_cell(const StepArgs<typename C::real>& a, i
     offsets(i, L.nx, L.per_i, L.si, ox);
     offsets(j, L.ny, L.per_j, L.sj, oy);
     offsets(k, L.nz, L.per_k, L.sk, oz);
-    ox[0] += oc0;
-    ox[1] += oc0;
-    ox[2] += oc0;
+    // slab form (VAR bit 4): offsets stay relative, the cell's own offset is split into its slab's
+    // start and its offset within the slab
+#if defined(__HIP_DEVICE_COMPILE__)
+    constexpr bool SLAB = (VAR & 16) != 0 && AXS >= 0 && (sizeof(raw) == 2 || sizeof(raw) == 4);
+#else
+    constexpr bool SLAB = false;
+#endif
+    if constexpr (!SLAB) {
+        ox[0] += oc0;
+        ox[1] += oc0;
+        ox[2] += oc0;
+    }
     // Plane bases are uniform (scalar registers); the per-lane part is a 32-bit byte offset
     // within a plane (planes are below 4 GiB), so loads and stores take the scalar-base form
     // without 64-bit address arithmetic per population.
@@ -579,9 +603,29 @@ __device__ __forceinline__ void step_cell(const StepArgs<typename C::real>& a, i
     // descriptor byte count: an unsigned 32-bit field (the variant runs only below 4 GiB)
     const size_t tbytes = (size_t)L.total * sizeof(raw);
     const int nbytes = (int)(uint32_t)(tbytes < 0xffffffffull ? tbytes : 0xffffffffull);
+    // slab form: the element offset of the cell within its slab of the slowest axis and the slab's
+    // start; a source's slab start moves by the ax2 offset (wrap included), its offset within the
+    // slab by the ax0 and ax1 offsets
+    const int s2 = A2 == 0 ? L.si : A2 == 1 ? L.sj : L.sk;
+    const int (&d0)[3] = AXS == 0 ? ox : AXS == 1 ? oy : oz;
+    const int (&d1)[3] = A1 == 0 ? ox : A1 == 1 ? oy : oz;
+    const int (&d2)[3] = A2 == 0 ? ox : A2 == 1 ? oy : oz;
+    const int w0 = SLAB ? oc0 - cc * s2 : 0;      // cell offset within its slab
+    const size_t slab0 = SLAB ? (size_t)cc * (size_t)s2 : 0;
+    const int sbytes = (int)((uint32_t)s2 * (uint32_t)sizeof(raw));
 #pragma unroll
     for (int l = 0; l < Q; ++l) {
-        const uint32_t off = (uint32_t)(oz[Lt::cz(l) + 1] + oy[Lt::cy(l) + 1] + ox[Lt::cx(l) + 1]) * (uint32_t)sizeof(raw);
+        if constexpr (SLAB) {
+#if defined(__HIP_DEVICE_COMPILE__)
+            const int c0 = Lt::c(l, AXS), c1 = Lt::c(l, A1), c2 = Lt::c(l, A2);
+            const char* sb = srcb + (size_t)((long long)slab0 + d2[c2 + 1]) * sizeof(raw);
+            rv[l] = buffer_load<raw>(sb, sbytes, (uint32_t)(w0 + d0[c0 + 1] + d1[c1 + 1]) * (uint32_t)sizeof(raw),
+                                     (uint32_t)l * pb32);
+#endif
+            continue;
+        }
+        const uint32_t offe = (uint32_t)(oz[Lt::cz(l) + 1] + oy[Lt::cy(l) + 1] + ox[Lt::cx(l) + 1]);
+        const uint32_t off = offe * (uint32_t)sizeof(raw);
         if constexpr (BUF) {
 #if defined(__HIP_DEVICE_COMPILE__)
             rv[l] = buffer_load<raw>(srcb, nbytes, off, (uint32_t)l * pb32);
@@ -590,7 +634,7 @@ __device__ __forceinline__ void step_cell(const StepArgs<typename C::real>& a, i
         }
         const raw* p;
         if constexpr ((VAR & 8) != 0) p = reinterpret_cast<const raw*>(srcb + (off + (uint32_t)l * pb32));
-        else p = reinterpret_cast<const raw*>(srcb + (size_t)l * pbytes + off);
+        else p = reinterpret_cast<const raw*>(srcb + (size_t)l * pbytes + (size_t)offe * sizeof(raw));
         if constexpr ((VAR & 2) != 0) rv[l] = __builtin_nontemporal_load(p);
         else rv[l] = *p;
     }
@@ -598,13 +642,23 @@ __device__ __forceinline__ void step_cell(const StepArgs<typename C::real>& a, i
     R out[Q];
     R tau = 0;
     const int oc = oc0;
-    const int occ = L.cidx(i, j, k);   // per-cell arrays
+    // per-cell arrays
+    const int occ = AXS < 0 ? L.cidx(i, j, k)
+                            : L.base + ca + cb * (A1 == 0 ? L.ci : A1 == 1 ? L.cj : L.ck) +
+                                  cc * (A2 == 0 ? L.ci : A2 == 1 ? L.cj : L.ck);
     collide_cell<Q, C, BGK3, HRR, VRE>(a, rv, i, j, k, ca, cb, cc, occ, out, tau);
 
     char* __restrict__ dstb = static_cast<char*>(a.dst);
     const uint32_t ob = (uint32_t)oc * (uint32_t)sizeof(raw);
 #pragma unroll
     for (int l = 0; l < Q; ++l) {
+        if constexpr (SLAB) {
+#if defined(__HIP_DEVICE_COMPILE__)
+            buffer_store<raw, (VAR & 1) != 0>(dstb + slab0 * sizeof(raw), sbytes, (uint32_t)w0 * (uint32_t)sizeof(raw),
+                                              (uint32_t)l * pb32, C::template enc_dep<Q>(out[l], l));
+#endif
+            continue;
+        }
         if constexpr (BUF) {
 #if defined(__HIP_DEVICE_COMPILE__)
             buffer_store<raw, (VAR & 1) != 0>(dstb, nbytes, ob, (uint32_t)l * pb32, C::template enc_dep<Q>(out[l], l));
@@ -613,7 +667,7 @@ __device__ __forceinline__ void step_cell(const StepArgs<typename C::real>& a, i
         }
         raw* p;
         if constexpr ((VAR & 8) != 0) p = reinterpret_cast<raw*>(dstb + (ob + (uint32_t)l * pb32));
-        else p = reinterpret_cast<raw*>(dstb + (size_t)l * pbytes + ob);
+        else p = reinterpret_cast<raw*>(dstb + (size_t)l * pbytes + (size_t)(uint32_t)oc * sizeof(raw));
         if constexpr ((VAR & 1) != 0) __builtin_nontemporal_store(C::template enc_dep<Q>(out[l], l), p);
         else *p = C::template enc_dep<Q>(out[l], l);
     }
@@ -635,7 +689,9 @@ __device__ __forceinline__ int heavy_order(int x, int n) { return x == 0 ? 1 : x
 // cache lines at their edges (the +-1 shift along ax0), which another XCD's L2 would fetch again;
 // consecutive row groups still go to different XCDs.  Blocks past the last full set of 8 row
 // groups keep their own position.
-template <int Q, class C, bool BGK3, bool HRR, bool VRE, int VAR>
+// AXS >= 0: step_cell's compile-time contiguous axis; the row coordinate is then taken as wave-uniform
+// (block_x is a multiple of 64, so each wave lies in one row).
+template <int Q, class C, bool BGK3, bool HRR, bool VRE, int VAR, int AXS = -1>
 __global__ void __launch_bounds__(sizeof(typename C::raw) >= 8 ? 256 : 1024, sizeof(typename C::raw) >= 8 ? 3 : 1) step_kernel(const StepArgs<typename C::real> a) {
     int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
     // (compiled out for the 16-byte tangent storage, which keeps the plain order: the order code
@@ -653,6 +709,7 @@ __global__ void __launch_bounds__(sizeof(typename C::raw) >= 8 ? 256 : 1024, siz
         }
     }
     int y = by * blockDim.y + threadIdx.y;
+    if constexpr (AXS >= 0) y = __builtin_amdgcn_readfirstlane(y);
     if (a.part) {
         // launch coordinate along the halo axis -> the part's cell (part_row)
         const int n = a.lay.n(a.part_axis == 1 ? a.lay.ax1 : a.lay.ax2);
@@ -663,7 +720,7 @@ __global__ void __launch_bounds__(sizeof(typename C::raw) >= 8 ? 256 : 1024, siz
     const int n1 = a.lay.n(a.lay.ax1);
     const int cb = (a.heavy_first & 1) && y < n1 ? heavy_order(y, n1) : y + 1;
     const int cc = (a.heavy_first & 2) ? heavy_order(bz, a.lay.n(a.lay.ax2)) : bz + 1;
-    step_cell<Q, C, BGK3, HRR, VRE, VAR>(a, bx * blockDim.x + threadIdx.x + 1, cb, cc);
+    step_cell<Q, C, BGK3, HRR, VRE, VAR, AXS>(a, bx * blockDim.x + threadIdx.x + 1, cb, cc);
 }
 
 // ------------------------------------------------------------------------------------------

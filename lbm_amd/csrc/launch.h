@@ -45,7 +45,18 @@ struct Launch {
         // one 32-bit offset range covers every plane of the buffer (var bit 1 forces the large form)
         const bool small = (double)L.total * sizeof(typename C::raw) < 4294967296.0 && !(var & 2);
         if constexpr (kNT) {
-            if (nt && small) hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 9>), grid, block, 0, s, a);
+            // the fast form is compiled per contiguous axis (the others in make_layout's order); its
+            // wave-uniform rows need whole waves along that axis
+            const int ax = L.ax0;
+            const bool axes_std = L.ax1 == (ax == 0 ? 1 : 0) && L.ax2 == (ax == 2 ? 1 : 2) && block.x % 64 == 0;
+            // row-blocked buffers of 4 GiB and more: the slab form (bit 4), one base per slab
+            const bool slab = L.rowblock && axes_std && sizeof(typename C::raw) <= 4;
+            if (nt && small && axes_std && ax == 0) hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 9, 0>), grid, block, 0, s, a);
+            else if (nt && small && axes_std && ax == 1) hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 9, 1>), grid, block, 0, s, a);
+            else if (nt && small && axes_std) hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 9, 2>), grid, block, 0, s, a);
+            else if (nt && slab && ax == 0) hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 17, 0>), grid, block, 0, s, a);
+            else if (nt && slab && ax == 1) hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 17, 1>), grid, block, 0, s, a);
+            else if (nt && slab) hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 17, 2>), grid, block, 0, s, a);
             else if (nt) hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 1>), grid, block, 0, s, a);
             else hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 0>), grid, block, 0, s, a);
         } else {

@@ -7,8 +7,9 @@
 // writes whole cache lines for populations with no velocity along that axis.
 //
 // Two arrangements of the Q populations, both addressed as  l * plane + cell(i, j, k):
-//   row-blocked (default when the buffer is below 4 GiB): [row][population][pitch] - plane is the
-//     row pitch and the row strides are Q times larger; the Q segments of a row are adjacent;
+//   row-blocked (the default; needs element offsets below 2^31 and one slab of the slowest axis
+//     below 4 GiB): [row][population][pitch] - plane is the row pitch and the row strides are Q
+//     times larger; the Q segments of a row are adjacent;
 //   planes: [population][row][pitch] - plane is the whole padded grid.
 // Per-cell arrays (tau, moments, forcing, averages) always use the plane arrangement of one
 // population: cidx(i, j, k), cplane elements.
@@ -69,8 +70,9 @@ __host__ __device__ inline int part_row(int c, int part, int n, int e) {
     return part == 1 ? (c < e ? c : c < 2 * e ? n - 2 * e + c : n) : (c < n - 2 * e ? c + e : n);
 }
 
-// fast < 0: choose (first periodic axis, else i).  q > 0 with rowblock: row-blocked buffers when a
-// buffer stays below 4 GiB (32-bit byte offsets across the whole buffer), planes otherwise.
+// fast < 0: choose (first periodic axis, else i).  q > 0 with rowblock: row-blocked buffers when every
+// element offset fits 31 bits and one slab of the slowest axis (Q rows per ax1 index) stays below
+// 4 GiB (the step kernel's 32-bit offsets within a slab), planes otherwise.
 inline Layout make_layout(int nx, int ny, int nz, int raw_bytes, int per_i, int per_j, int per_k, int fast = -1,
                           int q = 0, bool rowblock = false) {
     Layout L;
@@ -93,7 +95,8 @@ inline Layout make_layout(int nx, int ny, int nz, int raw_bytes, int per_i, int 
     const long long ea = 256 / raw_bytes;
     L.cplane = (pl + ea - 1) / ea * ea;
     const int nq = q > 0 ? q : 1;
-    L.rowblock = rowblock && q > 0 && (double)nq * (double)L.cplane * raw_bytes < 4294967296.0;
+    L.rowblock = rowblock && q > 0 && (double)nq * (double)L.cplane < 2147483648.0 &&
+                 (double)nq * L.pitch * (n1 + 2) * raw_bytes < 4294967296.0;
     if (L.rowblock) {
         st[L.ax1] = nq * L.pitch;
         st[L.ax2] = nq * L.pitch * (n1 + 2);

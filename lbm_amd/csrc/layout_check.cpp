@@ -3,7 +3,9 @@
 //   * every (population, cell) of the padded grid has its own element, inside the buffer;
 //   * the first interior cell of every row and population starts on a 128-byte boundary;
 //   * per-cell indices are distinct and inside cplane;
-//   * row-blocked buffers are chosen only below 4 GiB, where 32-bit byte offsets span them;
+//   * row-blocked buffers are chosen only while element offsets fit 31 bits and a slab of the
+//     slowest axis is below 4 GiB, and every element of a slab lies within [slab start, slab
+//     start + slab stride), the range of the step kernel's per-slab 32-bit offsets;
 //   * the two launches of a split step (part_row) cover every cell of the halo axis exactly once
 //     and map every other launch coordinate past the last cell, for any block height.
 // Prints one JSON line per case; exits non-zero on the first violation.
@@ -32,9 +34,17 @@ static int check(int nx, int ny, int nz, int raw, int q, int pi, int pj, int pk,
                     hit[(size_t)e] = 1;
                     int a0 = L.ax0 == 0 ? i : L.ax0 == 1 ? j : k;
                     if (a0 == 1 && (e * raw) % 128 != 0) return 3;
+                    if (L.rowblock) {
+                        const int a2 = L.ax2 == 0 ? i : L.ax2 == 1 ? j : k;
+                        const long long s2 = L.ax2 == 0 ? L.si : L.ax2 == 1 ? L.sj : L.sk;
+                        const long long w = e - (long long)a2 * s2;
+                        if (w < 0 || w >= s2) return 9;
+                    }
                 }
             }
-    if (L.rowblock && (double)L.total * raw >= 4294967296.0) return 4;
+    if (L.rowblock && ((double)L.total >= 2147483648.0 ||
+                       (double)(L.ax2 == 0 ? L.si : L.ax2 == 1 ? L.sj : L.sk) * raw >= 4294967296.0))
+        return 4;
     std::printf("{\"dims\": [%d, %d, %d], \"raw\": %d, \"q\": %d, \"per\": [%d, %d, %d], \"rowblock\": %d, "
                 "\"ax0\": %d, \"pitch\": %d, \"plane\": %lld, \"total\": %lld, \"cplane\": %lld}\n",
                 nx, ny, nz, raw, q, pi, pj, pk, L.rowblock, L.ax0, L.pitch, L.plane, L.total, L.cplane);
@@ -59,10 +69,12 @@ int main() {
                             }
                             ++n;
                         }
-    // the 4 GiB rule: config 3 of the reference workloads in FP32 D3Q27 exceeds it, config 2 does not
+    // the size rule: config 3 of the reference workloads in FP32 D3Q27 (5.2 GB per buffer) stays
+    // row-blocked; a grid with more than 2^31 elements per buffer does not
     const Layout big = make_layout(768, 256, 192, 4, 0, 1, 0, -1, 27, true);
     const Layout mid = make_layout(384, 192, 128, 4, 0, 1, 0, -1, 27, true);
-    if (big.rowblock || !mid.rowblock) return 5;
+    const Layout huge = make_layout(2048, 1024, 1024, 2, 0, 1, 0, -1, 27, true);
+    if (!big.rowblock || !mid.rowblock || huge.rowblock) return 5;
     // split steps: launch heights rounded up to the block height, as the launcher's grid is
     int nsplit = 0;
     for (int e = 1; e <= 2; ++e)
@@ -87,6 +99,7 @@ int main() {
             ++nsplit;
         }
     std::printf("{\"cases\": %d, \"rowblock_768x256x192_f32\": %d, \"rowblock_384x192x128_f32\": %d, "
-                "\"split_heights\": %d}\n", n, big.rowblock, mid.rowblock, nsplit);
+                "\"rowblock_2048x1024x1024_f16\": %d, \"split_heights\": %d}\n", n, big.rowblock, mid.rowblock,
+                huge.rowblock, nsplit);
     return 0;
 }

@@ -37,7 +37,7 @@ module lbm_hip
              lbm_set_inflow_series, lbm_set_turbulence, lbm_set_external_forcing, lbm_step, &
              lbm_synchronize, lbm_get_totals, lbm_check, lbm_error_message, lbm_generate_turbulence, &
              lbm_set_averaging, lbm_get_averages, lbm_stream, lbm_halo_bytes, lbm_halo_pack, lbm_halo_unpack, &
-             lbm_step_part
+             lbm_step_part, lbm_set_buoyancy, lbm_get_totals_noalloc
 
    interface
       integer(c_int) function lbm_create(desc, ctx) bind(C, name='lbm_create')
@@ -119,6 +119,13 @@ module lbm_hip
          type(c_ptr), value :: ctx, force
          integer(c_int), value :: realkind
       end function
+      ! buoyancy_forcing (src/m_buoyancy_forcing.F90:3-46) from the caller's pottemp(0:nx+1,0:ny+1,0:nz+1)
+      integer(c_int) function lbm_set_buoyancy(ctx, pottemp, theta0, scaling, realkind) bind(C, name='lbm_set_buoyancy')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx, pottemp
+         real(c_double), value :: theta0, scaling
+         integer(c_int), value :: realkind
+      end function
       ! turbine_forcing (src/m_turbine_forcing.F90:8-140) on the device, every step
       integer(c_int) function lbm_set_turbines(ctx, t) bind(C, name='lbm_set_turbines')
          import :: c_int, c_ptr, lbm_turbines
@@ -179,6 +186,13 @@ module lbm_hip
          type(c_ptr), value :: ctx
       end function
       integer(c_int) function lbm_get_totals(ctx, mass, mom) bind(C, name='lbm_get_totals')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(out) :: mass
+         real(c_double), intent(out) :: mom(3)
+      end function
+      ! lbm_get_totals without any device allocation (no counterpart in the reference)
+      integer(c_int) function lbm_get_totals_noalloc(ctx, mass, mom) bind(C, name='lbm_get_totals_noalloc')
          import :: c_int, c_ptr, c_double
          type(c_ptr), value :: ctx
          real(c_double), intent(out) :: mass

@@ -75,15 +75,17 @@ def test_lowprec_error_vs_fp64(case, storage, tol):
     assert e_32 <= 1e-5
 
 
-# The form a population buffer of 4 GiB or more takes (D3Q27 FP32 at 768x256x192 is 4.16 GB): the
-# plane arrangement and 64-bit plane offsets.  LBM_LAYOUT=planes and variant bit 1 force it on a
-# small grid.
+# The forms a population buffer of 4 GiB or more takes (D3Q27 FP32 at 768x256x192 is 5.2 GB): the
+# row-blocked arrangement addressed per slab of the slowest axis (the default above 4 GiB while a
+# slab stays below it), and the plane arrangement with 64-bit plane offsets.  Variant bit 1 (and
+# LBM_LAYOUT=planes) force them on a small grid.
+SLAB = {"LBM_STEP_VARIANT": "3"}
 LARGE = {"LBM_LAYOUT": "planes", "LBM_STEP_VARIANT": "3"}
 
 
 @pytest.mark.parametrize("storage", ["f32", "f16s", "bf16s", "f64"])
 def test_large_buffer_form_matches_default(case, storage):
-    """The >= 4 GiB form gives the default form's states bit for bit (the same arithmetic, other
+    """The >= 4 GiB forms give the default form's states bit for bit (the same arithmetic, other
     addressing), after steps through the folded face, the inflow/outflow planes, the cylinder and
     the forcing slice."""
     dims, f0, tu, mask = case
@@ -91,11 +93,12 @@ def test_large_buffer_form_matches_default(case, storage):
     if storage != "f64":
         f0 = f0.astype(np.float32)
         tu = tuple(a.astype(np.float32) for a in tu)
-    print(f"large-buffer form {storage} {dims} {steps} steps", flush=True)
+    print(f"large-buffer forms {storage} {dims} {steps} steps", flush=True)
     fa, _, _ = run(storage, dims, steps, f0, tu, mask)
-    fb, _, _ = run(storage, dims, steps, f0, tu, mask, env=LARGE)
-    assert np.isfinite(fb).all()
-    assert np.array_equal(fa, fb), float(np.abs(fa - fb).max())
+    for env in (SLAB, LARGE):
+        fb, _, _ = run(storage, dims, steps, f0, tu, mask, env=env)
+        assert np.isfinite(fb).all()
+        assert np.array_equal(fa, fb), (env, float(np.abs(fa - fb).max()))
 
 
 def test_large_buffer_form_fp32_matches_oracle(case):
