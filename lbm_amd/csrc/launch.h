@@ -49,12 +49,21 @@ struct Launch {
             // wave-uniform rows need whole waves along that axis
             const int ax = L.ax0;
             const bool axes_std = L.ax1 == (ax == 0 ? 1 : 0) && L.ax2 == (ax == 2 ? 1 : 2) && block.x % 64 == 0;
-            // row-blocked buffers of 4 GiB and more: the slab form (bit 4), one base per slab
-            const bool slab = L.rowblock && axes_std && sizeof(typename C::raw) <= 4;
-            if (nt && small && axes_std && ax == 0) hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 9, 0>), grid, block, 0, s, a);
-            else if (nt && small && axes_std && ax == 1) hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 9, 1>), grid, block, 0, s, a);
-            else if (nt && small && axes_std) hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 9, 2>), grid, block, 0, s, a);
-            else if (nt && slab && ax == 0) hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 17, 0>), grid, block, 0, s, a);
+            // the slab form (bit 4, one base per slab of the slowest axis): row-blocked 4-byte buffers of
+            // every size (measured on MI355X, D3Q27 384x192x128: 0.717 of the HBM peak against 0.714 for
+            // the one-base form; 768x256x192, 5.2 GB: 0.74 against 0.66 for the plane arrangement), and
+            // 2-byte buffers of 4 GiB and more (below, the one-base form measured 1.5 % faster)
+            const bool slab = L.rowblock && axes_std && sizeof(typename C::raw) <= 4 && (sizeof(typename C::raw) == 4 || !small);
+            bool done = false;
+            if constexpr (sizeof(typename C::raw) != 4) {
+                done = true;
+                if (nt && small && axes_std && ax == 0) hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 9, 0>), grid, block, 0, s, a);
+                else if (nt && small && axes_std && ax == 1) hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 9, 1>), grid, block, 0, s, a);
+                else if (nt && small && axes_std) hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 9, 2>), grid, block, 0, s, a);
+                else done = false;
+            }
+            if (done) {
+            } else if (nt && slab && ax == 0) hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 17, 0>), grid, block, 0, s, a);
             else if (nt && slab && ax == 1) hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 17, 1>), grid, block, 0, s, a);
             else if (nt && slab) hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 17, 2>), grid, block, 0, s, a);
             else if (nt) hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 1>), grid, block, 0, s, a);
