@@ -315,9 +315,9 @@ int lbm_ensemble_timing_read(lbm_ens* ens, double* ms, int* nsteps);
  * The context then holds the state after the nsteps steps, as after lbm_step.  Needs FP32 or
  * FP64 storage, a stepped state (not the first step after set_state / init_equilibrium), no
  * inflow turbulence, no turbines or buoyancy, at most one closed axis and at most 2^24 cells in a
- * plane of the two fastest storage axes.  Device memory: (nsteps/every + 4) state pairs + every
- * single states, kept by the context for the next run (a gradient loop allocates once) and
- * released by lbm_destroy.  Adjoint arithmetic in the storage precision, gradient accumulation
+ * plane of the two fastest storage axes.  Device memory beside the context's own: (ceil(nsteps/every)
+ * + 1) state pairs + (every - 1) single states, kept by the context for the next run (a gradient
+ * loop allocates once) and released by lbm_destroy.  Adjoint arithmetic in the storage precision, gradient accumulation
  * in FP64.
  */
 int lbm_objective_run(lbm_ctx* ctx, int it0, int nsteps, const int32_t* box, double* J);

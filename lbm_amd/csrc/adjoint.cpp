@@ -1,12 +1,12 @@
 // Objective and adjoint entry points (lbm_objective_run, lbm_adjoint_run in include/lbm_hip.h).
 // No counterpart in the reference: they differentiate its time loop.
 //
-// The adjoint run advances the context nsteps steps from its current state, storing the full
-// buffer pair every `every` steps (checkpoints).  The reverse sweep walks the segments from last
-// to first: it restores the segment's checkpoint, recomputes the segment's states (kept as full
-// buffers, ghosts included: they are the inputs of the steps to reverse), and reverses the steps
-// with the kernels of adjoint.h.  At the end the forward state after nsteps is restored, so the
-// context continues as if lbm_step had been called.
+// The adjoint run advances the context nsteps steps from its current state, storing every `every`
+// steps a checkpoint (the state and the ghost shell of the state before it).  The reverse sweep
+// walks the segments from last to first: it recomputes the segment's states from its checkpoint
+// (kept as full buffers, ghosts included: they are the inputs of the steps to reverse) and reverses
+// the steps with the kernels of adjoint.h.  The sweep leaves the context's buffers alone, so the
+// context continues from the forward state after nsteps as if lbm_step had been called.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -155,15 +155,17 @@ int lbm_adjoint_run(lbm_ctx* c, int it0, int nsteps, int every, const int32_t* b
     const size_t plane = (size_t)c->lay.cplane;   // per-cell arrays (force gradient)
     // the workspace stays with the context (a gradient loop calls this repeatedly); a failed
     // allocation frees it all
-    DevBuf &ck = c->adj_ck, &snaps = c->adj_snaps, &lam = c->adj_lam, &fin = c->adj_fin, &gF = c->adj_gF,
-           &gS = c->adj_gS, &acc = c->adj_acc;
+    DevBuf &ck = c->adj_ck, &snaps = c->adj_snaps, &lam = c->adj_lam, &gF = c->adj_gF, &gS = c->adj_gS,
+           &acc = c->adj_acc;
     auto release = [&]() {
-        for (DevBuf* x : {&ck, &snaps, &lam, &fin, &gF, &gS, &acc}) dev_free(*x);
+        for (DevBuf* x : {&ck, &snaps, &lam, &gF, &gS, &acc}) dev_free(*x);
     };
     int rc = LBM_OK;
     const bool want_gF = grad_force && c->has_force;
-    if ((rc = dev_alloc(c, ck, (size_t)nck * 2 * bb)) || (rc = dev_alloc(c, snaps, (size_t)every * bb)) ||
-        (rc = dev_alloc(c, lam, 2 * bb)) || (rc = dev_alloc(c, fin, 2 * bb)) ||
+    // a checkpoint: X(n0) and the ghost shell of X(n0-1); the segment's states X(n0+1) .. X(n1-1) in
+    // `every - 1` slots (X(n0) is read from the checkpoint)
+    if ((rc = dev_alloc(c, ck, (size_t)nck * 2 * bb)) || (rc = dev_alloc(c, snaps, (size_t)std::max(every - 1, 1) * bb)) ||
+        (rc = dev_alloc(c, lam, 2 * bb)) ||
         (rc = dev_alloc(c, acc, 2 * (size_t)objective_blocks(b) * sizeof(double))) ||
         (rc = dev_alloc(c, gS, (size_t)d.ny * d.nz * sizeof(double))) ||
         (want_gF && (rc = dev_alloc(c, gF, 3 * plane * sizeof(double))))) {
@@ -182,17 +184,16 @@ int lbm_adjoint_run(lbm_ctx* c, int it0, int nsteps, int every, const int32_t* b
         for (int n = 1; n <= nsteps; ++n) {
             if ((n - 1) % every == 0) {
                 const int q = (n - 1) / every;
-                ck_cur[q] = c->cur;
-                if (int r = copy_dev(c, ckp + (size_t)q * 2 * bb, c->buf[0], bb)) return r;
-                if (int r = copy_dev(c, ckp + (size_t)q * 2 * bb + bb, c->buf[1], bb)) return r;
+                ck_cur[q] = c->cur;   // the ping-pong parity of X(n0), which the adjoint buffers follow
+                if (int r = copy_dev(c, ckp + (size_t)q * 2 * bb, c->buf[c->cur], bb)) return r;
+                HIPCHK(c, c->ops->ghost_copy(c->lay, c->buf[c->cur ^ 1], ckp + (size_t)q * 2 * bb + bb, c->stream));
             }
             if (int r = lbm_step(c, it0 + n - 1, 1)) return r;
             HIPCHK(c, c->ops->objective(c->lay, c->buf[c->cur], b, static_cast<double*>(acc.p), c->stream));
         }
+        // the reverse sweep reads the context's buffers but writes only the workspace, so the forward
+        // state after nsteps stays where lbm_step left it
         const int fin_cur = c->cur;
-        const bool fin_raw0 = c->ghost_raw[0], fin_raw1 = c->ghost_raw[1];
-        if (int r = copy_dev(c, fin.p, c->buf[0], bb)) return r;
-        if (int r = copy_dev(c, static_cast<char*>(fin.p) + bb, c->buf[1], bb)) return r;
 
         // ---- reverse sweep
         HIPCHK(c, hipMemsetAsync(lam.p, 0, 2 * bb, c->stream));
@@ -215,18 +216,18 @@ int lbm_adjoint_run(lbm_ctx* c, int it0, int nsteps, int every, const int32_t* b
         for (int seg = nck - 1; seg >= 0; --seg) {
             const int n0 = seg * every, n1 = std::min(n0 + every, nsteps);
             // the inputs X(n0) .. X(n1-1) of the segment's steps, slot m holding X(n0+m): slot 0 is
-            // the checkpoint's X(n0); step n0+m is written straight into slot m after that slot
-            // received the ghost shell of X(n0+m-2) (slot m-2, or the checkpoint's X(n0-1)), which
-            // is what the buffer receiving that step holds in the forward run's ping-pong
-            const char* ckx = ckp + (size_t)seg * 2 * bb;
-            const char* x_n0 = ckx + (size_t)ck_cur[seg] * bb;
-            const char* x_before = ckx + (size_t)(ck_cur[seg] ^ 1) * bb;
-            if (int r = copy_dev(c, snp, x_n0, bb)) return r;
+            // the checkpoint's X(n0) (read in place); step n0+m is written straight into slot m after
+            // that slot received the ghost shell of X(n0+m-2) (slot m-2, or the checkpoint's shell of
+            // X(n0-1)), which is what the buffer receiving that step holds in the forward run's
+            // ping-pong
+            char* ckx = ckp + (size_t)seg * 2 * bb;
+            char* x_before = ckx + bb;
+            auto slot = [&](int m) { return m == 0 ? ckx : snp + (size_t)(m - 1) * bb; };
             for (int n = n0 + 1; n < n1; ++n) {
                 const int m = n - n0;
-                char* dst = snp + (size_t)m * bb;
-                HIPCHK(c, c->ops->ghost_copy(c->lay, m >= 2 ? snp + (size_t)(m - 2) * bb : x_before, dst, c->stream));
-                if (int r = step_buffers(c, snp + (size_t)(m - 1) * bb, dst, it0 + n)) return r;
+                char* dst = slot(m);
+                HIPCHK(c, c->ops->ghost_copy(c->lay, m >= 2 ? slot(m - 2) : x_before, dst, c->stream));
+                if (int r = step_buffers(c, slot(m - 1), dst, it0 + n)) return r;
             }
             for (int n = n1; n > n0; --n) {
                 const int bd = ck_cur[seg] ^ ((n - n0) & 1), bs = bd ^ 1;
@@ -238,17 +239,11 @@ int lbm_adjoint_run(lbm_ctx* c, int it0, int nsteps, int every, const int32_t* b
                     HIPCHK(c, c->ops->adj_inflow(c->lay, ly, ut, cs, sn, d.rho0, static_cast<double*>(gS.p), c->stream));
                 }
                 if (c->fold_dir >= 0) HIPCHK(c, c->ops->adj_face(c->lay, ly, lx, c->fold_dir, opt_lo, opt_hi, c->stream));
-                h.src = snp + (size_t)(n - 1 - n0) * bb;
+                h.src = slot(n - 1 - n0);
                 HIPCHK(c, c->ops->adj_step(c->lay, h, ly, lx, b, n - 1 >= 1 ? kappa : 0.0, 1, gFp,
                                            c->stream));
             }
         }
-        // back to the forward state after nsteps
-        if (int r = copy_dev(c, c->buf[0], fin.p, bb)) return r;
-        if (int r = copy_dev(c, c->buf[1], static_cast<char*>(fin.p) + bb, bb)) return r;
-        c->cur = fin_cur;
-        c->ghost_raw[0] = fin_raw0;
-        c->ghost_raw[1] = fin_raw1;
         // results
         double s = 0.0;
         std::vector<double> hs((size_t)d.ny * d.nz), hf, ha(2 * (size_t)nblk);

@@ -363,12 +363,38 @@ __global__ void __launch_bounds__(256) adjoint_step_kernel(const StepArgs<typena
         }
     };
     const raw* __restrict__ src = static_cast<const raw*>(a.src);
+    // 4-byte storage below 4 GiB: raw buffer loads and stores with the population's plane in the
+    // scalar offset (no per-access address arithmetic), as the forward step kernel
+#if defined(__HIP_DEVICE_COMPILE__)
+    constexpr bool BUF = SMALL && sizeof(raw) == 4;
+#else
+    constexpr bool BUF = false;
+#endif
+    const uint32_t pb32 = (uint32_t)(plane * sizeof(raw));
+    const size_t tbytes = (size_t)L.total * sizeof(raw);
+    const int nbytes = (int)(uint32_t)(tbytes < 0xffffffffull ? tbytes : 0xffffffffull);
     R D[Q], Ly[Q];
 #pragma unroll
-    for (int l = 0; l < Q; ++l) D[l] = C::template dep<Q>(*at(src, l, src_off(l)), l);
+    for (int l = 0; l < Q; ++l) {
+        if constexpr (BUF) {
+#if defined(__HIP_DEVICE_COMPILE__)
+            D[l] = C::template dep<Q>(buffer_load<raw>(src, nbytes, (uint32_t)src_off(l) * (uint32_t)sizeof(raw), (uint32_t)l * pb32), l);
+#endif
+        } else {
+            D[l] = C::template dep<Q>(*at(src, l, src_off(l)), l);
+        }
+    }
     if (coll) {
 #pragma unroll
-        for (int l = 0; l < Q; ++l) Ly[l] = C::template decode<Q>(*at(lam_y, l, oc), l);
+        for (int l = 0; l < Q; ++l) {
+            if constexpr (BUF) {
+#if defined(__HIP_DEVICE_COMPILE__)
+                Ly[l] = C::template decode<Q>(buffer_load<raw>(lam_y, nbytes, (uint32_t)oc * (uint32_t)sizeof(raw), (uint32_t)l * pb32), l);
+#endif
+            } else {
+                Ly[l] = C::template decode<Q>(*at(lam_y, l, oc), l);
+            }
+        }
     } else {
 #pragma unroll
         for (int l = 0; l < Q; ++l) Ly[l] = R(0);
@@ -390,7 +416,16 @@ __global__ void __launch_bounds__(256) adjoint_step_kernel(const StepArgs<typena
     raw* __restrict__ lx = lam_x;
     if (!edge) {
 #pragma unroll
-        for (int l = 0; l < Q; ++l) *at(lx, l, src_off(l)) = C::template encode<Q>(lg[l], l);
+        for (int l = 0; l < Q; ++l) {
+            if constexpr (BUF) {
+#if defined(__HIP_DEVICE_COMPILE__)
+                buffer_store<raw, true>(lx, nbytes, (uint32_t)src_off(l) * (uint32_t)sizeof(raw), (uint32_t)l * pb32,
+                                        C::template encode<Q>(lg[l], l));
+#endif
+            } else {
+                *at(lx, l, src_off(l)) = C::template encode<Q>(lg[l], l);
+            }
+        }
     } else {
 #pragma unroll
         for (int l = 0; l < Q; ++l) {

@@ -542,7 +542,7 @@ int lbm_destroy(lbm_ctx* c) {
                      &c->force[1], &c->force[2], &c->tau, &c->ext[0], &c->ext[1], &c->ext[2], &c->ext[3],
                      &c->stage, &c->partial, &c->snap, &c->t_rot, &c->t_theta, &c->t_blade, &c->t_nfoil,
                      &c->t_nang, &c->t_foil, &c->t_pts, &c->t_box, &c->theta, &c->adj_ck, &c->adj_snaps,
-                     &c->adj_lam, &c->adj_fin, &c->adj_gF, &c->adj_gS, &c->adj_acc};
+                     &c->adj_lam, &c->adj_gF, &c->adj_gS, &c->adj_acc};
     for (DevBuf* b : all) dev_free(*b);
     for (hipEvent_t ev : c->ev_step) hipEventDestroy(ev);
     for (hipEvent_t ev : c->ev_bc) hipEventDestroy(ev);

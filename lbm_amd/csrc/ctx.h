@@ -59,7 +59,7 @@ struct lbm_ctx {
     lbm_host::DevBuf t_rot, t_theta, t_blade, t_nfoil, t_nang, t_foil, t_pts, t_box;
     // adjoint workspace (lbm_adjoint_run): checkpoints, recomputed states, adjoint buffers, the final
     // state, gradient and objective accumulators; kept between runs, grown when a run needs more
-    lbm_host::DevBuf adj_ck, adj_snaps, adj_lam, adj_fin, adj_gF, adj_gS, adj_acc;
+    lbm_host::DevBuf adj_ck, adj_snaps, adj_lam, adj_gF, adj_gS, adj_acc;
     // buoyancy forcing (lbm_set_buoyancy): potential temperature, reference temperature, scaling
     lbm_host::DevBuf theta;
     bool buoy_on = false;
