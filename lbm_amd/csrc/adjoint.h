@@ -44,6 +44,15 @@ __host__ __device__ __forceinline__ void collide_adjoint(const StepArgs<R>& a, c
                                                          const R (&Ly)[Q], R kappa, bool coll, R (&lg)[Q], R (&lF)[3]) {
     using Lt = Lattice<Q>;
     constexpr int NP = (Q - 1) / 2;
+    // per-cell inputs read first (their latency overlaps the moments instead of stalling later)
+    R abl = R(0);
+    if (VRE && coll) abl = a.ablvisc[k - 1];
+    R fx = R(0), fy = R(0), fz = R(0);
+    if (a.Fx != nullptr && coll) {
+        fx = a.Fx[oc];
+        fy = a.Fy[oc];
+        fz = a.Fz[oc];
+    }
     // ---- forward recompute (collide_cell's moments)
     R dr = D[0], jx = 0, jy = 0, jz = 0, pxx = 0, pyy = 0, pzz = 0, pxy = 0, pxz = 0, pyz = 0;
 #pragma unroll
@@ -97,7 +106,7 @@ __host__ __device__ __forceinline__ void collide_adjoint(const StepArgs<R>& a, c
         const R bb = b11 * b22 - b12 * b12 + b11 * b33 - b13 * b13 + b22 * b33 - b23 * b23;
         t = bb * recip(amag);
         const R eddy = t > a.eps ? a.vconst * sqrt_r(t) : R(0);
-        tau = R(3) * (a.kinevisc + eddy + a.ablvisc[k - 1]) + R(0.5);
+        tau = R(3) * (a.kinevisc + eddy + abl) + R(0.5);
         fac = R(1) - recip(tau);
     }
     const R T = axx + ayy + azz;
@@ -117,7 +126,7 @@ __host__ __device__ __forceinline__ void collide_adjoint(const StepArgs<R>& a, c
     // Per pair: P2(v) - P2(u) = [-1.5 (v.v - u.u) + 4.5 (cv^2 - cu^2)] + [3 (cv - cu)];
     // grad_u: -3 (v - u) + 9 (cv - cu) c (even); grad_v P2(v) = [-3 v + 9 cv c] + [3 c].
     if (a.Fx != nullptr && rho > R(0)) {
-        const R Fx = a.Fx[oc], Fy = a.Fy[oc], Fz = a.Fz[oc];
+        const R Fx = fx, Fy = fy, Fz = fz;
         const R vx = ux - Fx * inv, vy = uy - Fy * inv, vz = uz - Fz * inv;
         const R vsq = vx * vx + vy * vy + vz * vz;
         const R dsq = R(-1.5) * (vsq - usq);
@@ -399,13 +408,21 @@ __global__ void __launch_bounds__(256) adjoint_step_kernel(const StepArgs<typena
 #pragma unroll
         for (int l = 0; l < Q; ++l) Ly[l] = R(0);
     }
-    bool solid = false;
-    if (a.solid != nullptr) {
-        const uint32_t word = a.solid[((cc - 1) * L.n(L.ax1) + (cb - 1)) * a.solid_wpr + ((ca - 1) >> 5)];
-        solid = (word >> ((ca - 1) & 31)) & 1u;
-    }
+    // the solid word is read behind the populations and used after the gradient loads are issued
+    uint32_t sword = 0;
+    if (a.solid != nullptr) sword = a.solid[((cc - 1) * L.n(L.ax1) + (cb - 1)) * a.solid_wpr + ((ca - 1) >> 5)];
     const bool inbox = i >= box.lo[0] && i <= box.hi[0] && j >= box.lo[1] && j <= box.hi[1] && k >= box.lo[2] && k <= box.hi[2];
     const int ocx = L.cidx(i, j, k);   // per-cell arrays (force, its gradient)
+    // the force-gradient accumulators are read here, with the populations, so that their latency
+    // overlaps the collision instead of following it
+    const bool want_gF = gF != nullptr && a.Fx != nullptr;
+    double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+    if (want_gF) {
+        g0 = gF[ocx];
+        g1 = gF[(size_t)L.cplane + ocx];
+        g2 = gF[2 * (size_t)L.cplane + ocx];
+    }
+    const bool solid = a.solid != nullptr && ((sword >> ((ca - 1) & 31)) & 1u);
     R lg[Q], lF[3];
     collide_adjoint<Q, BGK3, HRR, VRE, R>(a, D, k, ocx, solid, Ly, inbox ? kappa : R(0), coll != 0, lg, lF);
     // a cell next to a non-periodic face: some sources are ghosts, some own slots are unhit
@@ -446,10 +463,10 @@ __global__ void __launch_bounds__(256) adjoint_step_kernel(const StepArgs<typena
             if (unhit) *at(lx, l, oc) = C::template encode<Q>(R(0), l);
         }
     }
-    if (gF != nullptr && a.Fx != nullptr) {
-        gF[ocx] += (double)lF[0];
-        gF[(size_t)L.cplane + ocx] += (double)lF[1];
-        gF[2 * (size_t)L.cplane + ocx] += (double)lF[2];
+    if (want_gF) {
+        gF[ocx] = g0 + (double)lF[0];
+        gF[(size_t)L.cplane + ocx] = g1 + (double)lF[1];
+        gF[2 * (size_t)L.cplane + ocx] = g2 + (double)lF[2];
     }
 }
 
