@@ -301,6 +301,7 @@ StepHost step_host(const lbm_ctx* c, bool fold) {
     h.block_x = d.block_x;
     h.block_y = d.block_y;
     h.variant = c->variant;
+    h.lds_bytes = c->step_lds;
     h.fold_dir = fold ? c->fold_dir : -1;
     h.fold_opt_lo = h.fold_opt_hi = 0;
     if (h.fold_dir >= 0) {
@@ -448,9 +449,18 @@ int lbm_create(const lbm_desc* desc, lbm_ctx** out) {
     if (bx == 0) {
         // 4-byte storage: blocks of 64 cells x 4 rows (measured on MI355X, D3Q27 384x192x128
         // row-blocked: 26.4 k MLUPS against 25.6 k for one 192-cell row per block); 2- and 8-byte
-        // storage: one whole row per block (FP16 47.5 k against 42.8 k for 64 x 3)
+        // storage: one whole row per block (FP16 47.5 k against 42.8 k for 64 x 3), for 2-byte
+        // storage up to 256 cells, longer rows in blocks of 128 cells of one row (D3Q19 BGK
+        // 768x256x192: 515 us per step kernel against 565 us with 256 and 555 us with 128 x 2)
         const int rows = (lay.n(lay.ax0) + 63) / 64;
-        bx = ops->raw_bytes == 4 ? 64 : rows * 64 <= std::min(512, max_threads) ? rows * 64 : 256;
+        if (ops->raw_bytes == 4) {
+            bx = 64;
+        } else if (ops->raw_bytes == 2 && rows * 64 > 256) {
+            bx = 128;
+            if (by == 0) by = 1;
+        } else {
+            bx = rows * 64 <= std::min(512, max_threads) ? rows * 64 : 256;
+        }
     }
     if (by == 0) by = bx >= 256 ? 1 : 256 / bx;
     if (bx % 64 != 0 || bx * by > max_threads || by < 1)
@@ -468,8 +478,9 @@ int lbm_create(const lbm_desc* desc, lbm_ctx** out) {
     c->ops = ops;
     // LBM_STEP_VARIANT overrides the step-kernel variant: bit 0 non-temporal stores (the default),
     // bit 1 the form for population buffers of 4 GiB and more (64-bit plane offsets) whatever the
-    // buffer's size, which lets small grids exercise that path.
-    if (const char* v = getenv("LBM_STEP_VARIANT")) c->variant = atoi(v) & 3;
+    // buffer's size, which lets small grids exercise that path, bit 2 the one-base form with the
+    // storage axes mapped at run time.
+    if (const char* v = getenv("LBM_STEP_VARIANT")) c->variant = atoi(v) & 7;
     {
         // With one closed axis (no edges, corners or snapshots then) the step kernel can perform
         // that axis's face pass itself, and with at most that one closed axis the inflow/outflow
@@ -493,6 +504,22 @@ int lbm_create(const lbm_desc* desc, lbm_ctx** out) {
     hipGetDevice(&c->device);
     // lbm_get_totals_noalloc's scratch on this device (a failed lookup is reported by that call)
     scratch_resolve(c->device);
+    // Resident step-kernel blocks per CU, capped by reserving 1/n of the CU's LDS per block (the
+    // kernel itself uses none).  D3Q27 with 4-byte storage: 5 waves per SIMD (20 per CU; 5 blocks of
+    // 64 x 4 cells) where its 78 VGPRs would allow 6 (measured on MI355X, reg+Vreman 384x192x128:
+    // 351.1 us per step kernel against 358.9 us; 768x256x192: 1373 us against 1444 us); the 16-bit
+    // and D3Q19 kernels run best uncapped.  LBM_STEP_BLOCKS_PER_CU=n overrides (0: no cap).
+    {
+        const int waves_per_block = std::max(1, bx * by / 64);
+        int n = ops->raw_bytes == 4 && d.q == 27 ? std::max(1, 20 / waves_per_block) : 0;
+        if (const char* v = getenv("LBM_STEP_BLOCKS_PER_CU")) n = atoi(v);
+        int lds_cu = 0;
+        if (n >= 1 && hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, c->device) == hipSuccess) {
+            int per_block = 0;
+            if (hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) != hipSuccess) per_block = 0;
+            c->step_lds = std::min(lds_cu / n, per_block);
+        }
+    }
     c->lay = lay;
     c->tau0 = round_real(c, round_real(c, 3.0 * round_real(c, d.kinevisc)) + 0.5);
     c->fac0 = round_real(c, 1.0 - round_real(c, 1.0 / c->tau0));

@@ -75,6 +75,7 @@ struct lbm_ctx {
     bool inflow_fold_on = false;   // single-domain steps write the inflow/outflow planes of i in the
                                    // step kernel (with fold_on, or with no closed axis)
     int variant = 1;               // step-kernel variant (LBM_STEP_VARIANT), see impl.h; non-temporal
+    int step_lds = 0;              // dynamic LDS bytes per step-kernel block (residency cap), see lbm_create
                                    // stores measured fastest on MI355X (5.32 vs 5.11 TB/s)
     // a split step (lbm_step_part) whose first part has run: its step number and arguments
     bool part_open = false;

@@ -57,8 +57,10 @@ struct StepHost {
     int halo_dir;                             // halo axis (-1: none)
     int part, part_axis, part_edge;           // split step along the halo axis (StepArgs::part)
     int bgk3, hrr, vre;
-    int variant;      // step-kernel variant: bit 0 non-temporal stores, bit 1 the >= 4 GiB buffer form
+    int variant;      // step-kernel variant: bit 0 non-temporal stores, bit 1 the >= 4 GiB buffer form,
+                      // bit 2 the one-base form with run-time axis mapping
     int block_x, block_y;
+    int lds_bytes;    // dynamic LDS reserved per step-kernel block (caps the resident blocks per CU; 0: none)
 };
 
 // One ensemble member for the batched kernels: its step parameters (src, dst and the turbulence

@@ -143,29 +143,25 @@ __host__ __device__ constexpr int mirror(int l, int d, int opt) {
 //   fneq_l = 4.5 w [ (cAc - T/3) + (cu (3 cAc - T) - 2 c.g) ],  T = tr a2,  g = a2 u.
 // Working on departures needs no decode/encode arithmetic for the shifted 16-bit formats and
 // forms rho = 1 + sum D/S, j = sum c D/S without the cancellation of sum f.
-// Per-cell inputs of the collision besides the populations: the ABL viscosity of the layer, the word
-// of the row's solid mask holding the cell's bit, the external force.  For 16-bit storage the step
-// kernel reads them ahead of the population loads, so that their latency overlaps those loads'
-// instead of stalling the collision where they are used (measured on MI355X, D3Q27 reg+Vreman
-// 384x192x128: FP16 210 -> 197 us, BF16 214 -> 193 us per step kernel; FP32 354 -> 359 us, so
-// 4- and 8-byte storage read them where they are used).
+// Per-cell inputs of the collision besides the populations: the ABL viscosity of the layer and the
+// word of the row's solid mask holding the cell's bit.  For 16-bit storage the step kernel reads
+// them ahead of the population loads, so that their latency overlaps those loads' instead of
+// stalling the collision where they are used (measured on MI355X, D3Q27 reg+Vreman 384x192x128:
+// FP16 214 -> 198 us, BF16 215 -> 196 us per step kernel; FP32 352 -> 358 us, so 4- and 8-byte
+// storage read them where they are used).  The external force stays where it is used: read ahead,
+// its three registers would be held through every collision, forced or not (they cost the D3Q19
+// BGK kernel its fifth wave per SIMD).
 template <typename R>
 struct CellAux {
     R abl;
     uint32_t sword;
-    R fx, fy, fz;
 };
 template <bool VRE, typename R>
-__host__ __device__ __forceinline__ CellAux<R> cell_aux(const StepArgs<R>& a, int k, int ca, int cb, int cc, int oc) {
+__host__ __device__ __forceinline__ CellAux<R> cell_aux(const StepArgs<R>& a, int k, int ca, int cb, int cc) {
     const Layout& L = a.lay;
-    CellAux<R> x{R(0), 0u, R(0), R(0), R(0)};
+    CellAux<R> x{R(0), 0u};
     if (VRE) x.abl = a.ablvisc[k - 1];
     if (a.solid != nullptr) x.sword = a.solid[((cc - 1) * L.n(L.ax1) + (cb - 1)) * a.solid_wpr + ((ca - 1) >> 5)];
-    if (a.Fx != nullptr) {
-        x.fx = a.Fx[oc];
-        x.fy = a.Fy[oc];
-        x.fz = a.Fz[oc];
-    }
     return x;
 }
 
@@ -361,24 +357,27 @@ __host__ __device__ __forceinline__ void collide_cell(const StepArgs<typename C:
     bool in_fbox = true;
     if constexpr (sizeof(R) <= 8)
         in_fbox = i >= a.flo[0] && i <= a.fhi[0] && j >= a.flo[1] && j <= a.fhi[1] && k >= a.flo[2] && k <= a.fhi[2];
-    if (a.theta == nullptr) {
-        if (a.Fx != nullptr && rh > R(0) && in_fbox) {
-            const R ir = recip(rh);
-            const R fx = AUX ? aux.fx : a.Fx[oc], fy = AUX ? aux.fy : a.Fy[oc], fz = AUX ? aux.fz : a.Fz[oc];
-            add_feq_diff(ux - fx * ir, uy - fy * ir, uz - fz * ir, std::integral_constant<bool, false>());
-        }
-    } else if (rh > R(0)) {
-        // buoyancy added to the forcing of every interior cell (src/m_buoyancy_forcing_kernel.F90:37,
-        // the reference's operation order): F_z - scaling * (theta - theta0) / theta0
+    // One increment for both producers (a second inlined copy of add_feq_diff costs the unforced
+    // kernels registers: D3Q19 BGK FP16 71 -> 101 VGPRs).
+    if ((a.Fx != nullptr || a.theta != nullptr) && rh > R(0)) {
         R Fx = 0, Fy = 0, Fz = 0;
+        bool forced = false;
         if (a.Fx != nullptr && in_fbox) {
-            Fx = AUX ? aux.fx : a.Fx[oc];
-            Fy = AUX ? aux.fy : a.Fy[oc];
-            Fz = AUX ? aux.fz : a.Fz[oc];
+            Fx = a.Fx[oc];
+            Fy = a.Fy[oc];
+            Fz = a.Fz[oc];
+            forced = true;
         }
-        Fz = Fz - (a.bscale * (a.theta[oc] - a.btheta0)) / a.btheta0;
-        const R ir = recip(rh);
-        add_feq_diff(ux - Fx * ir, uy - Fy * ir, uz - Fz * ir, std::integral_constant<bool, false>());
+        if (a.theta != nullptr) {
+            // buoyancy added to the forcing of every interior cell (src/m_buoyancy_forcing_kernel.F90:37,
+            // the reference's operation order): F_z - scaling * (theta - theta0) / theta0
+            Fz = Fz - (a.bscale * (a.theta[oc] - a.btheta0)) / a.btheta0;
+            forced = true;
+        }
+        if (forced) {
+            const R ir = recip(rh);
+            add_feq_diff(ux - Fx * ir, uy - Fy * ir, uz - Fz * ir, std::integral_constant<bool, false>());
+        }
     }
 
     // Solid cells: swap the opposite pairs (2,3),(4,5),... (src/m_solids.F90:19-32).
@@ -629,7 +628,7 @@ __device__ __forceinline__ void step_cell(const StepArgs<typename C::real>& a, i
                                   cc * (A2 == 0 ? L.ci : A2 == 1 ? L.cj : L.ck);
     constexpr bool AUX = sizeof(raw) == 2;
     CellAux<R> aux{};
-    if constexpr (AUX) aux = cell_aux<VRE>(a, k, ca, cb, cc, occ);
+    if constexpr (AUX) aux = cell_aux<VRE>(a, k, ca, cb, cc);
     // Plane bases are uniform (scalar registers); the per-lane part is a 32-bit byte offset
     // within a plane (planes are below 4 GiB), so loads and stores take the scalar-base form
     // without 64-bit address arithmetic per population.

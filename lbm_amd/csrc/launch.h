@@ -40,7 +40,7 @@ struct Launch {
     static constexpr bool kNT = sizeof(typename C::raw) <= 8;
 
     template <bool B3, bool H, bool V>
-    static hipError_t step_t(const Layout& L, const StepArgs<R>& a, dim3 grid, dim3 block, int var, hipStream_t s) {
+    static hipError_t step_t(const Layout& L, const StepArgs<R>& a, dim3 grid, dim3 block, int var, int lds, hipStream_t s) {
         const bool nt = var & 1;
         // one 32-bit offset range covers every plane of the buffer (var bit 1 forces the large form)
         const bool small = (double)L.total * sizeof(typename C::raw) < 4294967296.0 && !(var & 2);
@@ -55,21 +55,27 @@ struct Launch {
             // 2-byte buffers of 4 GiB and more (below, the one-base form measured 1.5 % faster)
             const bool slab = L.rowblock && axes_std && sizeof(typename C::raw) <= 4 && (sizeof(typename C::raw) == 4 || !small);
             bool done = false;
-            if constexpr (sizeof(typename C::raw) != 4) {
+            // variant bit 2: the one-base form with the axes mapped at run time (no per-axis compilation)
+            if (var & 4) {
+                if (nt && small) {
+                    hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 9>), grid, block, lds, s, a);
+                    done = true;
+                }
+            } else if constexpr (sizeof(typename C::raw) != 4) {
                 done = true;
-                if (nt && small && axes_std && ax == 0) hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 9, 0>), grid, block, 0, s, a);
-                else if (nt && small && axes_std && ax == 1) hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 9, 1>), grid, block, 0, s, a);
-                else if (nt && small && axes_std) hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 9, 2>), grid, block, 0, s, a);
+                if (nt && small && axes_std && ax == 0) hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 9, 0>), grid, block, lds, s, a);
+                else if (nt && small && axes_std && ax == 1) hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 9, 1>), grid, block, lds, s, a);
+                else if (nt && small && axes_std) hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 9, 2>), grid, block, lds, s, a);
                 else done = false;
             }
             if (done) {
-            } else if (nt && slab && ax == 0) hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 17, 0>), grid, block, 0, s, a);
-            else if (nt && slab && ax == 1) hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 17, 1>), grid, block, 0, s, a);
-            else if (nt && slab) hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 17, 2>), grid, block, 0, s, a);
-            else if (nt) hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 1>), grid, block, 0, s, a);
-            else hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 0>), grid, block, 0, s, a);
+            } else if (nt && slab && ax == 0) hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 17, 0>), grid, block, lds, s, a);
+            else if (nt && slab && ax == 1) hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 17, 1>), grid, block, lds, s, a);
+            else if (nt && slab) hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 17, 2>), grid, block, lds, s, a);
+            else if (nt) hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 1>), grid, block, lds, s, a);
+            else hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 0>), grid, block, lds, s, a);
         } else {
-            hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 0>), grid, block, 0, s, a);
+            hipLaunchKernelGGL((step_kernel<Q, C, B3, H, V, 0>), grid, block, lds, s, a);
         }
         return hipGetLastError();
     }
@@ -142,14 +148,14 @@ struct Launch {
         a.xcd_groups = xcd && grid.x > 1;
         const int key = (h.bgk3 ? 4 : 0) | (h.hrr ? 2 : 0) | (h.vre ? 1 : 0);
         switch (key) {
-            case 0: return step_t<false, false, false>(L, a, grid, block, h.variant, s);
-            case 1: return step_t<false, false, true>(L, a, grid, block, h.variant, s);
-            case 2: return step_t<false, true, false>(L, a, grid, block, h.variant, s);
-            case 3: return step_t<false, true, true>(L, a, grid, block, h.variant, s);
-            case 4: return step_t<true, false, false>(L, a, grid, block, h.variant, s);
-            case 5: return step_t<true, false, true>(L, a, grid, block, h.variant, s);
-            case 6: return step_t<true, true, false>(L, a, grid, block, h.variant, s);
-            default: return step_t<true, true, true>(L, a, grid, block, h.variant, s);
+            case 0: return step_t<false, false, false>(L, a, grid, block, h.variant, h.lds_bytes, s);
+            case 1: return step_t<false, false, true>(L, a, grid, block, h.variant, h.lds_bytes, s);
+            case 2: return step_t<false, true, false>(L, a, grid, block, h.variant, h.lds_bytes, s);
+            case 3: return step_t<false, true, true>(L, a, grid, block, h.variant, h.lds_bytes, s);
+            case 4: return step_t<true, false, false>(L, a, grid, block, h.variant, h.lds_bytes, s);
+            case 5: return step_t<true, false, true>(L, a, grid, block, h.variant, h.lds_bytes, s);
+            case 6: return step_t<true, true, false>(L, a, grid, block, h.variant, h.lds_bytes, s);
+            default: return step_t<true, true, true>(L, a, grid, block, h.variant, h.lds_bytes, s);
         }
     }
 

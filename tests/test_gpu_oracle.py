@@ -324,9 +324,10 @@ def test_folded_boundary_passes_are_bit_identical(codes, storage, q, monkeypatch
 def test_plane_arrangement_large_form_is_bit_identical(codes, storage, q, monkeypatch):
     """The step-kernel forms of population buffers of 4 GiB and more, forced on a small grid: the
     row-blocked arrangement addressed per slab of the slowest axis (LBM_STEP_VARIANT=3) and the
-    plane arrangement with 64-bit plane offsets (LBM_LAYOUT=planes as well): the same bits as the
-    default form through the folded face, the inflow planes, the separate edge and corner kernels
-    (three closed axes), solids and forcing, and the state and moment transfers."""
+    plane arrangement with 64-bit plane offsets (LBM_LAYOUT=planes as well), and the one-base form
+    with the storage axes mapped at run time (LBM_STEP_VARIANT=5): the same bits as the default form
+    through the folded face, the inflow planes, the separate edge and corner kernels (three closed
+    axes), solids and forcing, and the state and moment transfers."""
     dims, steps = (20, 14, 12), 25
     o = dict(ibgk=3, ihrr=1, ivreman=1, smagorinsky=0.16733, kinevisc=2e-3, ibnd=codes[0], jbnd=codes[1], kbnd=codes[2])
     f0 = inputs.perturbed_uniform_flow(q, dims, seed=17, u0=(0.04, 0.01, -0.01), order=3)
@@ -334,7 +335,8 @@ def test_plane_arrangement_large_form_is_bit_identical(codes, storage, q, monkey
     tu = inputs.turbulence_fields(17, dims[1], dims[2], 10)
     print(f"plane arrangement codes={codes} storage={storage} q={q}", flush=True)
     out = []
-    for env in ({}, {"LBM_LAYOUT": "planes", "LBM_STEP_VARIANT": "3"}, {"LBM_STEP_VARIANT": "3"}):
+    for env in ({}, {"LBM_LAYOUT": "planes", "LBM_STEP_VARIANT": "3"}, {"LBM_STEP_VARIANT": "3"},
+                {"LBM_STEP_VARIANT": "5"}):
         for k in ("LBM_LAYOUT", "LBM_STEP_VARIANT"):
             monkeypatch.delenv(k, raising=False)
         for k, v in env.items():
