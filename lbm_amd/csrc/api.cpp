@@ -447,16 +447,19 @@ int lbm_create(const lbm_desc* desc, lbm_ctx** out) {
     const int max_threads = ops->raw_bytes >= 8 ? 256 : 1024;
     int bx = d.block_x, by = d.block_y;
     if (bx == 0) {
-        // 4-byte storage: blocks of 64 cells x 4 rows (measured on MI355X, D3Q27 384x192x128
-        // row-blocked: 26.4 k MLUPS against 25.6 k for one 192-cell row per block); 2- and 8-byte
-        // storage: one whole row per block (FP16 47.5 k against 42.8 k for 64 x 3), for 2-byte
-        // storage up to 256 cells, longer rows in blocks of 128 cells of one row (D3Q19 BGK
-        // 768x256x192: 515 us per step kernel against 565 us with 256 and 555 us with 128 x 2)
+        // Measured on MI355X (step kernel, bench.py):
+        // 4-byte storage: blocks of 64 cells x 4 rows (D3Q27 reg+Vreman 384x192x128: 352.3 us against
+        //   353-361 us for 64 x 2/3/8 and one 192-cell row, 384 us for 128 x 2);
+        // 2-byte storage: one whole row per block, rows over 512 cells in blocks of 256 (D3Q27
+        //   768x256x192: 732 us against 749 us for 64 x 1; D3Q19 BGK 768x256x192: 484 us against
+        //   487 / 497 us for 128 / 64 cells; on the 192-cell rows of 384x192x128, 64 x 1 is 1 % faster);
+        // 8-byte storage: blocks of one wave (D3Q27 reg+Vreman 384x192x128 FP64: 727 us against 758 us
+        //   for one 192-cell row per block); the 16-byte tangent storage keeps whole rows.
         const int rows = (lay.n(lay.ax0) + 63) / 64;
         if (ops->raw_bytes == 4) {
             bx = 64;
-        } else if (ops->raw_bytes == 2 && rows * 64 > 256) {
-            bx = 128;
+        } else if (ops->raw_bytes == 8) {
+            bx = 64;
             if (by == 0) by = 1;
         } else {
             bx = rows * 64 <= std::min(512, max_threads) ? rows * 64 : 256;
