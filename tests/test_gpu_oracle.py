@@ -325,7 +325,8 @@ def test_plane_arrangement_large_form_is_bit_identical(codes, storage, q, monkey
     """The step-kernel forms of population buffers of 4 GiB and more, forced on a small grid: the
     row-blocked arrangement addressed per slab of the slowest axis (LBM_STEP_VARIANT=3) and the
     plane arrangement with 64-bit plane offsets (LBM_LAYOUT=planes as well), and the one-base form
-    with the storage axes mapped at run time (LBM_STEP_VARIANT=5): the same bits as the default form
+    with the storage axes mapped at run time (LBM_STEP_VARIANT=5), and without the residency cap of
+    the D3Q27 4-byte kernel (LBM_STEP_BLOCKS_PER_CU=0): the same bits as the default form
     through the folded face, the inflow planes, the separate edge and corner kernels (three closed
     axes), solids and forcing, and the state and moment transfers."""
     dims, steps = (20, 14, 12), 25
@@ -336,8 +337,8 @@ def test_plane_arrangement_large_form_is_bit_identical(codes, storage, q, monkey
     print(f"plane arrangement codes={codes} storage={storage} q={q}", flush=True)
     out = []
     for env in ({}, {"LBM_LAYOUT": "planes", "LBM_STEP_VARIANT": "3"}, {"LBM_STEP_VARIANT": "3"},
-                {"LBM_STEP_VARIANT": "5"}):
-        for k in ("LBM_LAYOUT", "LBM_STEP_VARIANT"):
+                {"LBM_STEP_VARIANT": "5"}, {"LBM_STEP_BLOCKS_PER_CU": "0"}):
+        for k in ("LBM_LAYOUT", "LBM_STEP_VARIANT", "LBM_STEP_BLOCKS_PER_CU"):
             monkeypatch.delenv(k, raising=False)
         for k, v in env.items():
             monkeypatch.setenv(k, v)
