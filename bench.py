@@ -157,7 +157,11 @@ def cpu_baseline(w, f0, turb, sample_steps, threads, sample_steps_1t=0):
 def bench_ensemble(a):
     from lbm_amd import inputs, workloads
     from lbm_amd.domain import Ensemble
-    specs = inputs.ensemble_specs(4, a.members)
+    # --small-members: the same physics on members of 24-40 x 16-24 x 16-24 cells, where one launch
+    # per member is launch-bound rather than bandwidth-bound
+    size_sets = (dict(sizes_i=(24, 32, 40), sizes_j=(16, 24), sizes_k=(16, 24), box_side=(4, 8)) if a.small_members
+                 else dict(sizes_i=(96, 128, 160), sizes_j=(64, 96), sizes_k=(64, 96), box_side=(8, 16)))
+    specs = inputs.ensemble_specs(4, a.members, **size_sets)
     cells = workloads.cells(specs)
     print(f"[bench] building {a.members} ensemble members, {cells / 1e6:.1f} M cells", file=sys.stderr, flush=True)
     doms = workloads.ensemble_domains(specs, a.storage, nrturb=100)
@@ -206,14 +210,17 @@ def bench_ensemble(a):
         "dtype": {"f32": "f32", "f64": "f64", "f16s": "f16-store/f32", "bf16s": "bf16-store/f32"}[a.storage],
         "data": "synthetic (seeded sizes, box obstacles, perturbed moments and inflow forcing)",
         "config": {"workload": f"{a.members} D3Q19 regularized + Vreman members, sizes from "
-                               "{96,128,160}x{64,96}x{64,96}, box obstacle side 8-16, inflow/outflow i, periodic j, "
-                               "no-slip k=1 / free-slip k=nz, inflow forcing at i=10, nu=1e-4",
+                               + "x".join("{" + ",".join(str(v) for v in size_sets[k]) + "}"
+                                          for k in ("sizes_i", "sizes_j", "sizes_k"))
+                               + f", box obstacle side {size_sets['box_side'][0]}-{size_sets['box_side'][1]}, "
+                               "inflow/outflow i, periodic j, no-slip k=1 / free-slip k=nz, inflow forcing at i=10, "
+                               "nu=1e-4",
                    "cells": cells, "members": a.members, "storage": a.storage, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      # measured for the default ensemble (64 FP32 members) only
                      "traffic": (traffic_from_profiles("ensemble_step_kernel_batch_q19_64members")
-                                 if a.storage == "f32" and a.members == 64 else None),
+                                 if a.storage == "f32" and a.members == 64 and not a.small_members else None),
                      "kernel": "step_kernel_batch (graph; faces and inflow planes folded)", "bytes_per_cell": bpc_bytes,
                      "kernel_ms": round(step_ms, 5)},
         "tuned_launch": {"block": block, "blocks_per_cu": bpc, "ms_per_step": round(tune_ms, 5)},
@@ -503,6 +510,8 @@ def main():
     ap.add_argument("--split", action="store_true",
                     help="tile workload: split steps (edge rows, pack, inner rows during the transfer)")
     ap.add_argument("--members", type=int, default=64)
+    ap.add_argument("--small-members", action="store_true",
+                    help="ensemble workload: members of 24-40 x 16-24 x 16-24 cells instead of config 4's sizes")
     a = ap.parse_args()
     if a.steps is None:
         a.steps = {"ensemble": 500, "adjoint": 400}.get(a.workload, 1000)
