@@ -14,12 +14,13 @@ kernels) over the grid.  Inputs are synthetic and resident in HBM before timing 
 
 Prints ONE JSON line.  --gpus N > 1 is refused: this build targets one GPU.
 
-    python bench.py --workload ensemble [--members 64] [--steps 500]
+    python bench.py --workload ensemble [--members 64] [--steps 500] [--small-members]
 
 times the batched ensemble instead: 64 D3Q19 regularized + Vreman members with sizes drawn by
 seed from {96,128,160}x{64,96}x{64,96}, a seeded box obstacle (side 8-16) and seeded inflow
 forcing each, FP32, graph-captured batched steps with the autotuned launch shape, against the
-same members stepped with one launch sequence per member.
+same members stepped with one launch sequence per member; --small-members draws the sizes from
+{24,32,40}x{16,24}x{16,24} (box side 4-8) instead.
 
     python bench.py --workload adjoint [--steps 400]
 
