@@ -411,7 +411,7 @@ def bench_tile(a):
 
 
 def bench_q19(a):
-    """The config-1 physics at the storage-study size: D3Q19 BGK (second-order equilibrium), tau = 0.6
+    """The D3Q19 BGK box physics at the storage-study size: D3Q19 BGK (second-order equilibrium), tau = 0.6
     (nu = 1/30), periodic i and j, no-slip k, 768x256x192, u = (0.05, 0, 0) + U(-1e-3, 1e-3) per
     component (seed 1), no solids, no forcing."""
     from lbm_amd import inputs
@@ -512,7 +512,7 @@ def main():
                     help="tile workload: split steps (edge rows, pack, inner rows during the transfer)")
     ap.add_argument("--members", type=int, default=64)
     ap.add_argument("--small-members", action="store_true",
-                    help="ensemble workload: members of 24-40 x 16-24 x 16-24 cells instead of config 4's sizes")
+                    help="ensemble workload: members of 24-40 x 16-24 x 16-24 cells instead of {96,128,160}x{64,96}x{64,96}")
     a = ap.parse_args()
     if a.steps is None:
         a.steps = {"ensemble": 500, "adjoint": 400}.get(a.workload, 1000)

@@ -69,7 +69,7 @@ int main() {
                             }
                             ++n;
                         }
-    // the size rule: config 3 of the reference workloads in FP32 D3Q27 (5.2 GB per buffer) stays
+    // the size rule: D3Q27 FP32 at 768x256x192 (5.2 GB per buffer) stays
     // row-blocked; a grid with more than 2^31 elements per buffer does not
     const Layout big = make_layout(768, 256, 192, 4, 0, 1, 0, -1, 27, true);
     const Layout mid = make_layout(384, 192, 128, 4, 0, 1, 0, -1, 27, true);

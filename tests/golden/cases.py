@@ -17,10 +17,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 # name: (q, prec, dims, steps, seed, experiment, turbulence nrturb or 0, options)
 CASES = {
-    # config-1 physics: D3Q19 BGK, periodic i/j, no-slip k, tau = 0.6
+    # the BGK box: D3Q19 BGK, periodic i/j, no-slip k, tau = 0.6
     "q19_bgk_pp11": (19, "sp", (16, 8, 6), 20, 1, "none", 0,
                      dict(ibgk=2, ihrr=0, ivreman=0, ibnd=0, jbnd=0, kbnd=11, visc=1.0 / 30.0)),
-    # config-2 physics: D3Q27 3rd-order regularized + Vreman, inflow/outflow, periodic j,
+    # the cylinder wake: D3Q27 3rd-order regularized + Vreman, inflow/outflow, periodic j,
     # no-slip k=1, free-slip k=nz, the reference's cylinder (r=7 at i=50, j=ny/2), turbulence
     # forcing at i=10
     "q27_regvre_inflow_cyl": (27, "sp", (60, 14, 5), 20, 2, "cylinder", 25,

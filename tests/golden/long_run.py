@@ -1,4 +1,4 @@
-"""Long-run statistics of the config-2 physics (D3Q27 regularized + Vreman, inflow/outflow i,
+"""Long-run statistics of the cylinder-wake physics (D3Q27 regularized + Vreman, inflow/outflow i,
 periodic j, no-slip k=1 / free-slip k=nz, the reference's cylinder r=7 at (50, ny/2)) at
 96x32x16 over 2000 steps, run in restart segments of 50 steps, sampled at every segment end of
 the second half.  Shared by the fixture writer (the reference's own FP32 and FP64 builds) and the

@@ -1,11 +1,11 @@
 """Hand-written adjoint (lbm_adjoint_run) through the torch.autograd harness, against central
-finite differences of the FP64 forward (h = 1e-6), on small grids with the config-5 physics
+finite differences of the FP64 forward (h = 1e-6), on small grids with the physics of the adjoint workload
 (D3Q19 regularized + Vreman, inflow/outflow i, periodic j, no-slip k=1 / free-slip k=nz, Gaussian
 body-force blobs, power-law inflow profile, kinetic energy over a downstream box).
 
 Tolerances: FP64 adjoint vs FP64 differences 1e-6 relative to the largest checked gradient
 entry (differences of this size are round-off of J); FP32 adjoint vs FP64 differences 1e-3
-The per-cell collision adjoint is checked separately on the CPU
+(the accuracy asked of the FP32 adjoint).  The per-cell collision adjoint is checked separately on the CPU
 (tests/test_adjoint_cell.py).
 """
 import numpy as np

@@ -1,5 +1,5 @@
 """Long runs against the reference build, by statistics: the mean streamwise velocity per k, the
-turbulent kinetic energy per k and the time-mean wake profile behind the cylinder of the config-2
+turbulent kinetic energy per k and the time-mean wake profile behind the cylinder of the cylinder-wake
 physics (96x32x16, 2000 steps, the second half sampled every 50 steps; long_run.py), from the
 reference's own FP32 and FP64 builds (tests/golden/long_run_stats.npz, make_long_run.py).  Both
 implementations restart every 50 steps (the reference through its restart files, the library

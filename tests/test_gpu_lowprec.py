@@ -4,7 +4,7 @@ kernel's form for population buffers of 4 GiB and more.
 Error measure (as for the FP16/BF16 targets): relative L2 of the velocity field against the
 library's FP64 storage run on the same input, ||u - u64||_2 / ||u64||_2 over interior cells (u the
 3-vector), and the same for rho - 1.  The FP64 path is pinned to the FP64 oracle by
-test_gpu_oracle.py::test_config2_physics[f64] at 1e-12.
+test_gpu_oracle.py::test_cylinder_wake_physics[f64] at 1e-12.
 """
 import json
 import os
@@ -19,7 +19,7 @@ from lbm_amd.lattice import lattice
 
 pytestmark = pytest.mark.gpu
 
-CFG2 = dict(ibgk=3, ihrr=1, ivreman=1, smagorinsky=0.16733, kinevisc=1e-4, ibnd=1, jbnd=0, kbnd=12)
+WAKE = dict(ibgk=3, ihrr=1, ivreman=1, smagorinsky=0.16733, kinevisc=1e-4, ibnd=1, jbnd=0, kbnd=12)
 
 
 def run(storage, dims, steps, f0, tu, mask, env=None):
@@ -28,7 +28,7 @@ def run(storage, dims, steps, f0, tu, mask, env=None):
     old = {k: os.environ.get(k) for k in env}
     os.environ.update(env)
     try:
-        dom = Domain(27, dims, storage, **CFG2)
+        dom = Domain(27, dims, storage, **WAKE)
     finally:
         for k, v in old.items():
             if v is None:
@@ -108,7 +108,7 @@ def test_large_buffer_form_fp32_matches_oracle(case):
     f0 = f0.astype(np.float32)
     tu32 = tuple(a.astype(np.float32) for a in tu)
     f, _, _ = run("f32", dims, steps, f0, tu32, mask, env=LARGE)
-    so = {k: (float(np.float32(v)) if isinstance(v, float) else v) for k, v in CFG2.items()}
+    so = {k: (float(np.float32(v)) if isinstance(v, float) else v) for k, v in WAKE.items()}
     sim = oracle.OracleSim(27, dims, "f32", **so)
     sim.restart(f0)
     sim.run(1, 1 + steps, uini=float(np.float32(0.05)), blank=mask,

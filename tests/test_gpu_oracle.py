@@ -84,14 +84,14 @@ def oracle64(q, dims, f0, **o):
     return sim
 
 
-def test_config1_physics_full_size():
+def test_bgk_box_128x64x64_200_steps():
     """D3Q19 BGK, periodic i/j, no-slip k, tau = 0.6, seeded perturbation 1e-3, at the stated
     128x64x64: the 1e-5 bar after 100 steps, then to the stated 200 steps against the FP64 oracle
     (at least as close to it as the reference's FP32 build, within a factor 2).  About 45 s of
     oracle time."""
     q, dims = 19, (128, 64, 64)
     o = dict(ibgk=2, kinevisc=1.0 / 30.0, ibnd=0, jbnd=0, kbnd=11)
-    print(f"config1 {dims} 100 + 100 steps", flush=True)
+    print(f"D3Q19 BGK box {dims} 100 + 100 steps", flush=True)
     dom, sim = pair(q, dims, **o)
     f0 = inputs.perturbed_uniform_flow(q, dims, seed=1)
     dom.set_state(f0)
@@ -110,12 +110,12 @@ def test_config1_physics_full_size():
 
 
 @pytest.mark.parametrize("prec", ["f32", "f64"])
-def test_config2_physics(prec):
+def test_cylinder_wake_physics(prec):
     """D3Q27 3rd-order regularized + Vreman, inflow/outflow, periodic j, no-slip/free-slip k,
     cylinder, inflow-turbulence forcing at i=10."""
     q, dims = 27, (96, 32, 16)
     steps = 100 if prec == "f32" else 40
-    print(f"config2 {prec} {dims} {steps}", flush=True)
+    print(f"cylinder wake {prec} {dims} {steps}", flush=True)
     dt = np.float32 if prec == "f32" else np.float64
     o = dict(ibgk=3, ihrr=1, ivreman=1, smagorinsky=0.16733, kinevisc=1e-4, ibnd=1, jbnd=0, kbnd=12)
     dom, sim = pair(q, dims, prec, **o)

@@ -2,7 +2,7 @@
 (population, cell) of the padded grid gets its own element inside the buffer in both the
 row-blocked and the plane arrangement, rows start 128-byte aligned, per-cell indices are distinct,
 row-blocked buffers are used while element offsets fit 31 bits and a slab of the slowest axis fits
-4 GiB (config 3's 5.2 GB FP32 buffer included) with every element of a slab inside its slab's
+4 GiB (the 5.2 GB buffer of D3Q27 FP32 at 768x256x192 included) with every element of a slab inside its slab's
 range; the two launches of a split step cover each cell
 of the halo axis once (one or two edge rows per side, tile heights up to 1024, block heights 1, 2,
 4, 8)."""
