@@ -281,10 +281,14 @@ __host__ __device__ __forceinline__ void collide_cell(const StepArgs<typename C:
     // are the first of each class), hoisted out of the pair loop:
     //   even part  ev = kE0 + kE2 cu^2 + kF cAc
     //   odd part   od = kO cu pol + kF (cu (3 cAc - T) + c.(-2 g))
-    // with kE2 = wS*4.5rho, kO = wS*rho, kF = fac*4.5wS, kE0 = wS(rho base - 1) - kF T/3.
+    //                 = cu (kA + kB cu^2 + kC cAc) + kF c.(-2 g)      (regularized)
+    // with kE2 = wS*4.5rho, kO = wS*rho, kF = fac*4.5wS, kE0 = wS(rho base - 1) - kF T/3,
+    // pol = 3 - 4.5 u.u + 4.5 cu^2 (third order) or 3, so kA = kO (3 - 4.5 u.u) - kF T or
+    // 3 kO - kF T, kB = 4.5 kO or 0, kC = 3 kF.
     constexpr int NC = Q == 27 ? 3 : 2;
-    R kE0[NC + 1], kE2[NC + 1], kO[NC + 1], kF[NC + 1];
+    R kE0[NC + 1], kE2[NC + 1], kO[NC + 1], kF[NC + 1], kA[NC + 1], kB[NC + 1], kC[NC + 1];
     const R T3 = T * (R(1) / R(3));
+    const R P0 = R(3) - R(4.5) * usq;
 #pragma unroll
     for (int n = 1; n <= NC; ++n) {
         const R wS = weight<Q, R>(n == 1 ? 1 : n == 2 ? 7 : 19) * S;
@@ -292,10 +296,17 @@ __host__ __device__ __forceinline__ void collide_cell(const StepArgs<typename C:
         kO[n] = wS * rh;
         kF[n] = fac * (R(4.5) * wS);
         kE0[n] = HRR ? wS * rb1 - kF[n] * T3 : wS * rb1;
+        kA[n] = (BGK3 ? kO[n] * P0 : R(3) * kO[n]) - kF[n] * T;
+        kB[n] = R(4.5) * kO[n];
+        kC[n] = R(3) * kF[n];
     }
     const R a2xy = R(2) * axy, a2xz = R(2) * axz, a2yz = R(2) * ayz;
     const R gmx = R(-2) * gx, gmy = R(-2) * gy, gmz = R(-2) * gz;
-    const R P0 = R(3) - R(4.5) * usq;
+    // c.A.c by the axes c spans: one axis a_dd; two axes (a_dd + a_ee) +- 2 a_de; three axes
+    // T + sxy 2a_xy + sxz 2a_xz + syz 2a_yz, whose signs multiply to +1, so with sxy fixed the
+    // other two are (s, s) or (s, -s): T +- 2a_xy +- (2a_xz +- 2a_yz)
+    const R exy = axx + ayy, exz = axx + azz, eyz = ayy + azz;
+    const R tpp = T + a2xy, tmm = T - a2xy, up = a2xz + a2yz, um = a2xz - a2yz;
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
         const int l = 2 * p + 1;
@@ -303,19 +314,21 @@ __host__ __device__ __forceinline__ void collide_cell(const StepArgs<typename C:
         const int n2 = Lt::norm2(l);
         const R cu = sdot<R>(cx, cy, cz, ux, uy, uz);
         const R cu2 = cu * cu;
-        const R pol = BGK3 ? P0 + R(4.5) * cu2 : R(3);
-        const R cp = cu * pol;
         if (HRR) {
-            R cac = sdot<R>(cx * cy, cx * cz, cy * cz, a2xy, a2xz, a2yz);
-            if (cx != 0) cac += axx;
-            if (cy != 0) cac += ayy;
-            if (cz != 0) cac += azz;
+            R cac;
+            if (n2 == 1) cac = cx != 0 ? axx : cy != 0 ? ayy : azz;
+            else if (n2 == 2) cac = cz == 0 ? (cx * cy > 0 ? exy + a2xy : exy - a2xy)
+                                  : cy == 0 ? (cx * cz > 0 ? exz + a2xz : exz - a2xz)
+                                            : (cy * cz > 0 ? eyz + a2yz : eyz - a2yz);
+            else cac = cx * cy > 0 ? (cx * cz > 0 ? tpp + up : tpp - up) : (cx * cz > 0 ? tmm + um : tmm - um);
             const R ev = kE0[n2] + kE2[n2] * cu2 + kF[n2] * cac;
-            const R t2 = cu * (R(3) * cac - T) + sdot<R>(cx, cy, cz, gmx, gmy, gmz);
-            const R od = kO[n2] * cp + kF[n2] * t2;
+            const R in = BGK3 ? kA[n2] + kB[n2] * cu2 + kC[n2] * cac : kA[n2] + kC[n2] * cac;
+            const R od = cu * in + kF[n2] * sdot<R>(cx, cy, cz, gmx, gmy, gmz);
             dev[l] = ev + od;
             dev[l + 1] = ev - od;
         } else {
+            const R pol = BGK3 ? P0 + R(4.5) * cu2 : R(3);
+            const R cp = cu * pol;
             const R e = kE0[n2] + kE2[n2] * cu2;   // S (E - w)
             const R o = kO[n2] * cp;               // S O
             dev[l] = (e + o) + fac * (D[l] - (e + o));
