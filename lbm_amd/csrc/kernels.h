@@ -185,27 +185,33 @@ __host__ __device__ __forceinline__ void collide_cell(const StepArgs<typename C:
     // accumulators start at -0, the additive identity the compiler may fold (0 + x is not x for
     // x = -0)
     const R z0 = -R(0);
-    R dr = D[0], jx = z0, jy = z0, jz = z0;
-    R pxx = z0, pyy = z0, pzz = z0, pxy = z0, pxz = z0, pyz = z0;
+    R jx = z0, jy = z0, jz = z0;
+    R pxy = z0, pxz = z0, pyz = z0;
+    // pair sums by the axes c spans (a face axis, an edge plane, the corners): the density and
+    // the diagonal second moments are sums of these (a corner counts in all three diagonals, an
+    // edge plane in two)
+    R sfx = z0, sfy = z0, sfz = z0, sexy = z0, sexz = z0, seyz = z0, scn = z0;
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
         const int l = 2 * p + 1;
         const int cx = Lt::cx(l), cy = Lt::cy(l), cz = Lt::cz(l);
+        const int n2 = Lt::norm2(l);
         const R sp = D[l] + D[l + 1];
         const R dp = D[l] - D[l + 1];
-        dr += sp;
+        if (n2 == 1) (cx != 0 ? sfx : cy != 0 ? sfy : sfz) += sp;
+        else if (n2 == 2) (cz == 0 ? sexy : cy == 0 ? sexz : seyz) += sp;
+        else scn += sp;
         if (cx > 0) jx += dp; else if (cx < 0) jx -= dp;
         if (cy > 0) jy += dp; else if (cy < 0) jy -= dp;
         if (cz > 0) jz += dp; else if (cz < 0) jz -= dp;
         if (HRR || VRE) {
-            if (cx != 0) pxx += sp;
-            if (cy != 0) pyy += sp;
-            if (cz != 0) pzz += sp;
             if (cx * cy > 0) pxy += sp; else if (cx * cy < 0) pxy -= sp;
             if (cx * cz > 0) pxz += sp; else if (cx * cz < 0) pxz -= sp;
             if (cy * cz > 0) pyz += sp; else if (cy * cz < 0) pyz -= sp;
         }
     }
+    R dr = D[0] + (((sfx + sfy) + sfz) + ((sexy + sexz) + seyz) + scn);
+    const R pxx = sfx + sexy + sexz + scn, pyy = sfy + sexy + seyz + scn, pzz = sfz + sexz + seyz + scn;
     dr *= invS;
     R rh = R(1) + dr;
     const R invrho = recip(rh);
