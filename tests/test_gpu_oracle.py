@@ -134,6 +134,26 @@ def test_cylinder_wake_physics(prec):
     check(dom.get_state(), sim.f1, q, prec)
 
 
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("q,vre", [(19, 1), (27, 0)])
+def test_second_order_equilibrium_regularized(q, vre, prec):
+    """Regularization over the second-order equilibrium (ibgk = 2, ihrr = 1: the reference applies
+    the regularized non-equilibrium whatever ibgk is, src/m_postcoll_kernel.F90:189-244), with and
+    without Vreman, periodic i and j, no-slip k."""
+    dims = (40, 24, 16)
+    steps = 60 if prec == "f32" else 30
+    print(f"ibgk=2 ihrr=1 q={q} ivreman={vre} {prec} {dims} {steps}", flush=True)
+    dt = np.float32 if prec == "f32" else np.float64
+    o = dict(ibgk=2, ihrr=1, ivreman=vre, smagorinsky=0.16733, kinevisc=2e-3, ibnd=0, jbnd=0, kbnd=11)
+    dom, sim = pair(q, dims, prec, **o)
+    f0 = inputs.perturbed_uniform_flow(q, dims, seed=5, order=2, dtype=dt)
+    dom.set_state(f0)
+    sim.restart(f0)
+    dom.step(1, steps)
+    sim.run(1, 1 + steps)
+    check(dom.get_state(), sim.f1, q, prec)
+
+
 @pytest.mark.parametrize("codes", [(11, 21, 22), (22, 11, 12), (12, 22, 21)])
 def test_closed_box_edges_corners(codes):
     q, dims, steps = 27, (20, 12, 10), 60
